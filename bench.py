@@ -1,0 +1,179 @@
+"""Benchmark: batched forward + adjoint gradient of the 2D acoustic propagator.
+
+Workload (one "step" = one full gradient evaluation):
+  350 x 1700 grid (+20 absorbing layers per side -> 390 x 1740 padded), dx 5 m,
+  dt 0.5 ms, 4000 time steps, 64 shots evenly spaced along the surface (z = 2),
+  1700 receivers at z = 2, 15 Hz Ricker, velocity = depth-linear 1500-4500 m/s
+  background + anisotropic Gaussian random field (lh 300 m, lv 50 m, std 200,
+  seed 0) clipped to [1500, 4700], Gardner density, L2 misfit against data
+  modelled in a smoothed copy of the model, forward checkpoints every 50 steps.
+  The step is forward (seismograms + checkpoints) + misfit + hand-written
+  adjoint (recompute + backward sweep + imaging condition) -> vp.grad.
+
+Metric: Gcell-updates/s = shots * padded cells * nt / step time.
+
+Also reported (one JSON line): the roofline of the dominant kernel measured
+live with events on the launch stream, the job-level HBM roofline fraction at
+the algorithmic 80 + 24/k bytes per cell-update, and the CPU oracle (FP32, the
+host's cores) timed on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s fwd+adjoint (64-shot batch vs per-shot) and % of HBM roofline"
+
+
+def workload(n_shots=64, nz=350, nx=1700, nt=4000, nabc=20, dx=5.0, dt=5e-4, f0=15.0):
+    from geobrain_amd import synthetic
+    from geobrain_amd.physics import wave
+    vp = synthetic.grf_model(nz, nx, dx, seed=0)
+    return dict(
+        nz=nz, nx=nx, nt=nt, nabc=nabc, dx=dx, dt=dt, vp=vp, vp_obs=synthetic.smooth_model(vp, 0.05),
+        rho=synthetic.gardner(vp),
+        damp=wave.bc_pml(nx, nz, dx, dx, nabc, float(vp.max()), False).astype(np.float32),
+        wavelet=np.tile(synthetic.ricker_padded(f0, dt, nt), (n_shots, 1)),
+        src_x=synthetic.surface_shots(nx, n_shots), src_z=np.full(n_shots, 2, np.int64),
+        rcv_x=np.arange(nx, dtype=np.int64), rcv_z=np.full(nx, 2, np.int64))
+
+
+def cpu_baseline(w, steps=400, shots=1):
+    """FP32 oracle gradient on `shots` shot(s) x `steps` time steps of the same grid."""
+    import oracle
+    oracle.build()
+    sl = slice(0, shots)
+    args = dict(nz=w["nz"], nx=w["nx"], nabc=w["nabc"], free_surface=False, dt=w["dt"],
+                dz=w["dx"], src_z=w["src_z"][sl], src_x=w["src_x"][sl],
+                src_v=w["wavelet"][sl, :steps], rcv_z=w["rcv_z"], rcv_x=w["rcv_x"],
+                vp=w["vp"], rho=w["rho"], damp=w["damp"])
+    t0 = time.perf_counter()
+    f = oracle.forward(**args, precision="f32")["p"]
+    g = np.random.default_rng(0).standard_normal(f.shape).astype(np.float32)
+    oracle.gradient(**args, g_p=g, chunk=50, precision="f32")
+    t = time.perf_counter() - t0
+    cells = shots * (w["nz"] + 2 * w["nabc"]) * (w["nx"] + 2 * w["nabc"]) * steps
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(cells / t / 1e9, 5), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP32 C++ oracle (OpenMP), forward + checkpointed adjoint gradient, "
+                      f"{shots} shot x {steps} steps of the {w['nz']}x{w['nx']} grid, "
+                      f"{t:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=400)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots shard across GPUs with one gradient all-reduce per iteration, "
+              "not implemented here)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from geobrain_amd.physics import wave
+
+    dev = torch.device("cuda:0")
+    w = workload(n_shots=args.shots, nt=args.nt)
+    S, nt = args.shots, args.nt
+    NZ, NX = w["nz"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]
+    k = 50
+    segments = -(-nt // k)
+    T = lambda a, **kw: torch.as_tensor(a, device=dev, **kw)  # noqa: E731
+    common = dict(nx=w["nx"], nz=w["nz"], dx=w["dx"], dz=w["dx"], nt=nt, dt=w["dt"],
+                  nabc=w["nabc"], free_surface=False, src_x=w["src_x"], src_z=w["src_z"],
+                  src_n=S, src_v=T(w["wavelet"]), rcv_x=w["rcv_x"], rcv_z=w["rcv_z"],
+                  rcv_n=w["nx"], damp=T(w["damp"]), rho=T(w["rho"]))
+    with torch.no_grad():
+        d_obs = wave.acoustic_forward_kernel(vp=T(w["vp_obs"]), **common)["p"]
+    vp = T(w["vp"]).requires_grad_(True)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(ev=None):
+        vp.grad = None
+        if ev:
+            ev[0].record(stream)
+        out = wave.acoustic_forward_kernel(vp=vp, checkpoint_segments=segments, **common)
+        if ev:
+            ev[1].record(stream)
+        loss = ((out["p"] - d_obs) ** 2).sum()
+        if ev:
+            ev[2].record(stream)
+        loss.backward()
+        if ev:
+            ev[3].record(stream)
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_fwd = np.mean([e[0].elapsed_time(e[1]) for e in evs]) / 1e3
+    t_bwd = np.mean([e[2].elapsed_time(e[3]) for e in evs]) / 1e3
+    cells = S * NZ * NX * nt
+    step_s = wall / args.steps
+    value = cells / step_s / 1e9
+    # forward kernel: one launch per time step, 24 algorithmic bytes per cell-update
+    fwd_launch_s = t_fwd / nt
+    fwd_bytes = 24.0 * S * NZ * NX
+    achieved = fwd_bytes / fwd_launch_s / 1e9
+    job_bytes = (80.0 + 24.0 / k) * cells
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"batched gradient: {S} shots, {w['nz']}x{w['nx']} grid "
+                               f"(+{w['nabc']} PML), {nt} steps, checkpoint every {k}",
+                   "shots": S, "grid": [w["nz"], w["nx"]], "padded": [NZ, NX], "nt": nt,
+                   "checkpoint_every": k},
+        "roofline": {"bound": "hbm", "kernel": "fwd_step_kernel (forward pass)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(fwd_launch_s * 1e6, 2),
+                     "bytes_per_launch": fwd_bytes},
+        "phases": {"forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4),
+                   "forward_gcells": round(cells / t_fwd / 1e9, 3),
+                   "backward_gcells": round(cells / t_bwd / 1e9, 3),
+                   "job_hbm_frac": round(job_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                   "job_bytes_per_cell_update": 80.0 + 24.0 / k},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(w, steps=args.cpu_steps)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

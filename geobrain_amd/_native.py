@@ -1,0 +1,123 @@
+"""ctypes binding of ``libgbw_acoustic2d.so`` (declared in ``include/gbw_acoustic2d.h``).
+
+The library is the only compute path: if it is missing, or no ROCm GPU is
+visible, calls raise ``RuntimeError``.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgbw_acoustic2d.so")
+
+c_i32 = ctypes.c_int32
+c_f32 = ctypes.c_float
+c_vp = ctypes.c_void_p
+
+ABI_VERSION = 1
+
+EXPORTS = (
+    "gbw_abi_version",
+    "gbw_last_error",
+    "gbw_acoustic2d_workspace_bytes",
+    "gbw_acoustic2d_checkpoint_bytes",
+    "gbw_acoustic2d_forward",
+    "gbw_acoustic2d_backward",
+)
+
+
+class Problem(ctypes.Structure):
+    """``gbw_acoustic2d_problem``."""
+    _fields_ = [
+        ("nx", c_i32), ("nz", c_i32), ("dx", c_f32), ("dz", c_f32), ("nt", c_i32),
+        ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32), ("src_n", c_i32),
+        ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
+        ("snapshot_every", c_i32), ("checkpoint_every", c_i32), ("checkpoint_bf16", c_i32),
+        ("tile_rows", c_i32), ("shots_per_block", c_i32),
+    ]
+
+
+class Inputs(ctypes.Structure):
+    """``gbw_acoustic2d_inputs``."""
+    _fields_ = [
+        ("src_x", c_vp), ("src_z", c_vp), ("rcv_x", c_vp), ("rcv_z", c_vp),
+        ("src_v", c_vp), ("damp", c_vp), ("vp", c_vp), ("rho", c_vp),
+    ]
+
+
+class Outputs(ctypes.Structure):
+    """``gbw_acoustic2d_outputs``."""
+    _fields_ = [
+        ("rec_p", c_vp), ("rec_vx", c_vp), ("rec_vz", c_vp),
+        ("energy_p", c_vp), ("energy_vx", c_vp), ("energy_vz", c_vp),
+        ("snapshots", c_vp), ("checkpoints", c_vp),
+    ]
+
+
+class Adjoint(ctypes.Structure):
+    """``gbw_acoustic2d_adjoint``."""
+    _fields_ = [
+        ("g_p", c_vp), ("g_vx", c_vp), ("g_vz", c_vp), ("checkpoints", c_vp),
+        ("grad_vp", c_vp), ("grad_rho", c_vp), ("grad_src_v", c_vp),
+    ]
+
+
+_lock = threading.Lock()
+_lib = None
+
+STATUS = {
+    -1: ValueError,
+    -2: IndexError,
+    -3: RuntimeError,
+    -4: RuntimeError,
+    -5: NotImplementedError,
+}
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the library once; raises ``RuntimeError`` if it was not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"geobrain_amd: HIP library not found at {path}; build it with "
+                "`python -m geobrain_amd.build` (there is no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        lib.gbw_abi_version.restype = ctypes.c_int
+        lib.gbw_abi_version.argtypes = []
+        lib.gbw_last_error.restype = ctypes.c_char_p
+        lib.gbw_last_error.argtypes = []
+        lib.gbw_acoustic2d_workspace_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic2d_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_int]
+        lib.gbw_acoustic2d_checkpoint_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic2d_checkpoint_bytes.argtypes = [ctypes.POINTER(Problem)]
+        lib.gbw_acoustic2d_forward.restype = ctypes.c_int
+        lib.gbw_acoustic2d_forward.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
+            c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_acoustic2d_backward.restype = ctypes.c_int
+        lib.gbw_acoustic2d_backward.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Adjoint),
+            c_vp, ctypes.c_size_t, c_vp]
+        ver = lib.gbw_abi_version()
+        if ver != ABI_VERSION:
+            raise RuntimeError(f"geobrain_amd: ABI version {ver} != expected {ABI_VERSION}")
+        _lib = lib
+        return lib
+
+
+def check(rc: int) -> None:
+    """Raise the Python exception matching a non-zero gbw_status."""
+    if rc == 0:
+        return
+    msg = load().gbw_last_error().decode(errors="replace")
+    raise STATUS.get(rc, RuntimeError)(msg)
+
+
+def last_error() -> str:
+    return load().gbw_last_error().decode(errors="replace")

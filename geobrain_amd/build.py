@@ -1,0 +1,53 @@
+"""Build recipe for the HIP library (gfx950 only).
+
+``python -m geobrain_amd.build`` compiles ``csrc/gbw_acoustic2d.hip`` into
+``libgbw_acoustic2d.so`` next to this file.  The flags:
+
+* ``--offload-arch=gfx950``: MI355X (CDNA4) code objects only.
+* ``-ffp-contract=off``: no fused multiply-add contraction, so every
+  floating-point operation rounds once, in the order the reference's FP32
+  PyTorch ops use.
+* ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed FP32 math.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc", "gbw_acoustic2d.hip")
+LIB = os.path.join(HERE, "libgbw_acoustic2d.so")
+
+HIPCC_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the ROCm toolchain is required to build geobrain_amd")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the library; returns its path.  Always recompiles."""
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,302 @@
+"""2D acoustic FDTD operator on MI355X, drop-in for ``geobrain.physics.wave.kernels``.
+
+``acoustic_forward_kernel`` keeps the reference's signature, argument meaning,
+defaults and output keys (``geobrain/physics/wave/kernels/acoustic_2d.py:294-501``),
+but every time step, the receiver sampling, the energy maps and the gradient
+run in the hand-written HIP library ``libgbw_acoustic2d.so``:
+
+* forward: ``gbw_acoustic2d_forward`` -- one fused stencil launch per step;
+* gradient: a ``torch.autograd.Function`` whose backward calls
+  ``gbw_acoustic2d_backward`` -- the exact discrete adjoint with forward
+  checkpoints every ``k`` steps and recomputation in between, replacing
+  autograd through ``torch.utils.checkpoint`` (``acoustic_2d.py:448-460``).
+
+``checkpoint_segments`` keeps its meaning: the checkpoint interval is the
+``torch.chunk`` chunk length ``ceil(nt / checkpoint_segments)`` and the energy
+maps are summed per chunk as the reference sums them.  With
+``checkpoint_segments == 1`` the interval is chosen to balance checkpoint and
+recompute memory (``~sqrt(3 nt)``); results do not depend on it.
+
+Inputs must live on a ROCm GPU.  There is no CPU path: a CPU tensor, a
+missing library or a missing GPU raises ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+from torch import Tensor
+
+from ... import _native
+
+
+def pad_model(v: Tensor, pml: int, nz: int, nx: int,
+              device: torch.device = torch.device("cpu")) -> Tensor:
+    """Edge-replicating pad of ``(nz, nx)`` to ``(nz + 2 pml, nx + 2 pml)``
+    (``acoustic_2d.py:18-66``): cell ``(i, j)`` takes ``v[clamp(i - pml), clamp(j - pml)]``.
+    The HIP library applies the same map on the fly; this is the tensor form."""
+    ii = torch.arange(nz + 2 * pml, device=v.device).sub(pml).clamp_(0, nz - 1)
+    jj = torch.arange(nx + 2 * pml, device=v.device).sub(pml).clamp_(0, nx - 1)
+    return v.to(torch.float32)[ii][:, jj].to(device)
+
+
+def chunk_length(nt: int, checkpoint_segments: int) -> int:
+    """Chunk size ``torch.chunk(src_v, checkpoint_segments, -1)`` uses."""
+    return -(-nt // max(1, int(checkpoint_segments)))
+
+
+def auto_checkpoint_interval(nt: int) -> int:
+    """Interval minimising checkpoints (12 B/cell each, ``nt/k`` of them) plus the
+    per-interval bracket store (4 B/cell per step, ``k`` of them)."""
+    return max(1, min(nt, int(round(math.sqrt(3.0 * nt)))))
+
+
+@dataclass
+class LaunchChoice:
+    """Launch parameters the library accepts (``tile_rows``, ``shots_per_block``);
+    zeros select the library's defaults.  The autotuner fills these."""
+    tile_rows: int = 0
+    shots_per_block: int = 0
+
+
+_launch_override: Optional[LaunchChoice] = None
+
+
+def set_launch_choice(choice: Optional[LaunchChoice]) -> None:
+    """Pin the launch parameters for subsequent calls (``None`` = library default)."""
+    global _launch_override
+    _launch_override = choice
+
+
+@dataclass
+class _Spec:
+    prob: _native.Problem
+    src_x: np.ndarray
+    src_z: np.ndarray
+    rcv_x: np.ndarray
+    rcv_z: np.ndarray
+    n_snap: int
+    keep: Dict[str, object] = field(default_factory=dict)
+
+
+def _stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def _ptr(t: Optional[Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _inputs(spec: _Spec, src_v: Tensor, damp: Tensor, vp: Tensor, rho: Tensor) -> _native.Inputs:
+    ins = _native.Inputs()
+    ins.src_x = spec.src_x.ctypes.data
+    ins.src_z = spec.src_z.ctypes.data
+    ins.rcv_x = spec.rcv_x.ctypes.data if spec.rcv_x.size else None
+    ins.rcv_z = spec.rcv_z.ctypes.data if spec.rcv_z.size else None
+    ins.src_v = src_v.data_ptr()
+    ins.damp = damp.data_ptr()
+    ins.vp = vp.data_ptr()
+    ins.rho = rho.data_ptr()
+    return ins
+
+
+class AcousticFunction(torch.autograd.Function):
+    """Seismograms and energy maps as a differentiable function of
+    ``(vp, rho, src_v)``; ``damp`` and the geometry are constants."""
+
+    @staticmethod
+    def forward(ctx, vp, rho, src_v, damp, spec: _Spec):
+        lib = _native.load()
+        prob = spec.prob
+        dev = vp.device
+        S, nt, R, nz, nx = prob.src_n, prob.nt, prob.rcv_n, prob.nz, prob.nx
+        need_grad = any(ctx.needs_input_grad[:3])
+        if not need_grad:
+            prob.checkpoint_every = 0
+        ws_bytes = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(prob), 0)
+        if ws_bytes == 0:
+            _native.check(-1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        ck = None
+        if need_grad:
+            ck = torch.empty(lib.gbw_acoustic2d_checkpoint_bytes(ctypes.byref(prob)),
+                             dtype=torch.uint8, device=dev)
+        f32 = dict(dtype=torch.float32, device=dev)
+        p = torch.empty((S, nt, R), **f32)
+        vx = torch.empty((S, nt, R), **f32)
+        vz = torch.empty((S, nt, R), **f32)
+        ep = torch.empty((nz, nx), **f32)
+        evx = torch.empty((nz, nx), **f32)
+        evz = torch.empty((nz, nx), **f32)
+        snaps = torch.empty((spec.n_snap, nz, nx), **f32) if spec.n_snap else torch.empty(0, **f32)
+        outs = _native.Outputs()
+        outs.rec_p, outs.rec_vx, outs.rec_vz = _ptr(p), _ptr(vx), _ptr(vz)
+        outs.energy_p, outs.energy_vx, outs.energy_vz = _ptr(ep), _ptr(evx), _ptr(evz)
+        outs.snapshots = _ptr(snaps) if spec.n_snap else None
+        outs.checkpoints = _ptr(ck)
+        ins = _inputs(spec, src_v, damp, vp, rho)
+        rc = lib.gbw_acoustic2d_forward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(outs),
+                                        ws.data_ptr(), ws_bytes, _stream_handle(dev))
+        _native.check(rc)
+        ctx.mark_non_differentiable(ep, evx, evz, snaps)
+        ctx.spec = spec
+        ctx.ck = ck
+        ctx.save_for_backward(vp, rho, src_v, damp)
+        return p, vx, vz, ep, evx, evz, snaps
+
+    @staticmethod
+    def backward(ctx, g_p, g_vx, g_vz, *unused):
+        lib = _native.load()
+        vp, rho, src_v, damp = ctx.saved_tensors
+        spec = ctx.spec
+        prob = spec.prob
+        dev = vp.device
+
+        def prep(g):
+            return None if g is None else g.to(torch.float32).contiguous()
+
+        g_p, g_vx, g_vz = prep(g_p), prep(g_vx), prep(g_vz)
+        want_vp, want_rho, want_w = ctx.needs_input_grad[:3]
+        grad_vp = torch.empty_like(vp) if want_vp else None
+        grad_rho = torch.empty_like(rho) if want_rho else None
+        grad_w = torch.empty_like(src_v) if want_w else None
+        adj = _native.Adjoint()
+        adj.g_p, adj.g_vx, adj.g_vz = _ptr(g_p), _ptr(g_vx), _ptr(g_vz)
+        adj.checkpoints = _ptr(ctx.ck)
+        adj.grad_vp, adj.grad_rho, adj.grad_src_v = _ptr(grad_vp), _ptr(grad_rho), _ptr(grad_w)
+        ws_bytes = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(prob), 2 if want_rho else 1)
+        if ws_bytes == 0:
+            _native.check(-1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        ins = _inputs(spec, src_v, damp, vp, rho)
+        rc = lib.gbw_acoustic2d_backward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(adj),
+                                         ws.data_ptr(), ws_bytes, _stream_handle(dev))
+        _native.check(rc)
+        ctx.ck = None
+        return grad_vp, grad_rho, grad_w, None, None
+
+
+def _host_index(t, name: str) -> np.ndarray:
+    if torch.is_tensor(t):
+        a = t.detach().to("cpu").numpy()
+    else:
+        a = np.asarray(t)
+    if a.ndim != 1:
+        raise NotImplementedError(
+            f"{name}: only one point per shot (1-D positions) is supported; got shape {a.shape}")
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _on_gpu(t: Tensor, name: str, device: torch.device) -> Tensor:
+    if not torch.is_tensor(t):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise RuntimeError(
+            f"geobrain_amd runs the acoustic propagator on a ROCm GPU; {name} is on {t.device}")
+    if t.device != device:
+        raise RuntimeError(f"{name} is on {t.device}, expected {device}")
+    return t
+
+
+def acoustic_forward_kernel(
+    nx: int,
+    nz: int,
+    dx: float,
+    dz: float,
+    nt: int,
+    dt: float,
+    nabc: int,
+    free_surface: bool,
+    src_x: Tensor,
+    src_z: Tensor,
+    src_n: int,
+    src_v: Tensor,
+    rcv_x: Tensor,
+    rcv_z: Tensor,
+    rcv_n: int,
+    damp: Tensor,
+    vp: Tensor,
+    rho: Tensor,
+    checkpoint_segments: int = 1,
+    device: torch.device = torch.device('cpu'),
+    dtype: torch.dtype = torch.float32,
+    snapshot_every: int = 0,
+) -> Dict[str, Tensor]:
+    """Forward simulation (signature of ``acoustic_2d.py:294-317``).
+
+    Returns ``'p', 'vx', 'vz'`` ``(src_n, nt, rcv_n)``, the energy maps
+    ``'forward_wavefield_p/vx/vz'`` ``(nz, nx)`` (detached) and, when
+    ``snapshot_every > 0``, ``'snapshots'`` ``(n_snaps, nz, nx)`` on the CPU.
+    Gradients flow to ``vp``, ``rho`` and ``src_v`` when they require grad.
+    """
+    if dtype != torch.float32:
+        raise NotImplementedError("geobrain_amd computes in float32 only (the reference default)")
+    dev = vp.device
+    for name, t in (("vp", vp), ("rho", rho), ("src_v", src_v), ("damp", damp)):
+        _on_gpu(t, name, dev)
+    if not torch.cuda.is_available():
+        raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
+    sx, sz = _host_index(src_x, "src_x"), _host_index(src_z, "src_z")
+    rx, rz = _host_index(rcv_x, "rcv_x"), _host_index(rcv_z, "rcv_z")
+    if sx.shape != sz.shape or sx.shape[0] != src_n:
+        raise ValueError(f"src_x/src_z must both have src_n={src_n} entries")
+    if rx.shape != rz.shape or rx.shape[0] != rcv_n:
+        raise ValueError(f"rcv_x/rcv_z must both have rcv_n={rcv_n} entries")
+    if tuple(vp.shape) != (nz, nx) or tuple(rho.shape) != (nz, nx):
+        raise ValueError(f"vp and rho must have shape ({nz}, {nx})")
+    if tuple(damp.shape) != (nz + 2 * nabc, nx + 2 * nabc):
+        raise ValueError(f"damp must have shape ({nz + 2 * nabc}, {nx + 2 * nabc})")
+    if src_v.dim() == 1:
+        shared = 1
+        if src_v.shape[0] != nt:
+            raise ValueError(f"src_v has {src_v.shape[0]} samples, nt={nt}")
+    elif src_v.dim() == 2:
+        shared = 0
+        if tuple(src_v.shape) != (src_n, nt):
+            raise ValueError(f"src_v must have shape ({src_n}, {nt}) or ({nt},)")
+    else:
+        raise NotImplementedError("src_v must be (src_n, nt) or (nt,)")
+
+    prob = _native.Problem()
+    prob.nx, prob.nz = int(nx), int(nz)
+    prob.dx, prob.dz = float(dx), float(dz)
+    prob.nt, prob.dt = int(nt), float(dt)
+    prob.nabc, prob.free_surface = int(nabc), int(bool(free_surface))
+    prob.src_n, prob.rcv_n = int(src_n), int(rcv_n)
+    prob.wavelet_shared = shared
+    if snapshot_every > 0:
+        prob.energy_chunk = int(snapshot_every)
+        prob.snapshot_every = int(snapshot_every)
+        n_snap = -(-nt // snapshot_every)
+    else:
+        prob.energy_chunk = chunk_length(nt, checkpoint_segments)
+        prob.snapshot_every = 0
+        n_snap = 0
+    if snapshot_every <= 0 and checkpoint_segments > 1:
+        prob.checkpoint_every = chunk_length(nt, checkpoint_segments)
+    else:
+        prob.checkpoint_every = auto_checkpoint_interval(nt)
+    prob.checkpoint_bf16 = 0
+    choice = _launch_override
+    prob.tile_rows = choice.tile_rows if choice else 0
+    prob.shots_per_block = choice.shots_per_block if choice else 0
+    spec = _Spec(prob=prob, src_x=sx, src_z=sz, rcv_x=rx, rcv_z=rz, n_snap=n_snap)
+
+    f32 = torch.float32
+    p, vx, vz, ep, evx, evz, snaps = AcousticFunction.apply(
+        vp.to(f32).contiguous(), rho.to(f32).contiguous(), src_v.to(f32).contiguous(),
+        damp.to(f32).contiguous(), spec)
+    out = {
+        'p': p, 'vx': vx, 'vz': vz,
+        'forward_wavefield_p': ep,
+        'forward_wavefield_vx': evx,
+        'forward_wavefield_vz': evz,
+    }
+    if snapshot_every > 0:
+        out['snapshots'] = snaps.cpu()
+    return out

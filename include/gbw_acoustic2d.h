@@ -1,0 +1,142 @@
+/*
+ * gbw_acoustic2d.h -- C ABI of the MI355X (gfx950) 2D acoustic propagator.
+ *
+ * The library replaces the time-stepping operator of GeoBrain's 2D acoustic
+ * propagator:
+ *   acoustic_forward_kernel(nx, nz, dx, dz, nt, dt, nabc, free_surface,
+ *                           src_x, src_z, src_n, src_v, rcv_x, rcv_z, rcv_n,
+ *                           damp, vp, rho, checkpoint_segments, device, dtype,
+ *                           snapshot_every)
+ *       geobrain/physics/wave/kernels/acoustic_2d.py:294-501
+ * together with the per-step body it drives,
+ *   acoustic_step_forward(...)   geobrain/physics/wave/kernels/acoustic_2d.py:123-291
+ * and the gradient PyTorch autograd computes for it through
+ * torch.utils.checkpoint (acoustic_2d.py:448-460).  The reference is pure
+ * Python and binds no C interface; these entry points are what its Python
+ * layer would bind with ctypes (see INTEGRATION.md for the stub).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative gbw_status; the
+ *    message of the last failure on the calling thread is gbw_last_error().
+ *  - Arrays marked DEVICE are device pointers (e.g. torch tensor data_ptr()),
+ *    contiguous, row-major, float32 unless stated.  Arrays marked HOST are host
+ *    pointers read during the call only.
+ *  - All device work is enqueued on `stream`; nothing is allocated: the caller
+ *    passes a workspace of at least *_workspace_bytes() bytes and, for the
+ *    gradient, a checkpoint buffer of gbw_acoustic2d_checkpoint_bytes() bytes.
+ *  - Inputs are never modified.
+ */
+#ifndef GBW_ACOUSTIC2D_H
+#define GBW_ACOUSTIC2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBW_ABI_VERSION 1
+
+typedef enum {
+  GBW_OK = 0,
+  GBW_ERR_INVALID = -1,     /* bad sizes / flags (reference: ValueError)              */
+  GBW_ERR_INDEX = -2,       /* source or receiver index out of range (IndexError)     */
+  GBW_ERR_WORKSPACE = -3,   /* workspace or checkpoint buffer too small               */
+  GBW_ERR_HIP = -4,         /* a HIP runtime call failed                              */
+  GBW_ERR_UNSUPPORTED = -5  /* valid request the library does not implement           */
+} gbw_status;
+
+/* Scalars of acoustic_forward_kernel (acoustic_2d.py:294-317) plus launch
+ * choices.  dx is accepted for signature parity; the scheme uses dz on both
+ * axes, as the reference does (acoustic_2d.py:407,410). */
+typedef struct gbw_acoustic2d_problem {
+  int32_t nx, nz;            /* model grid (unpadded)                                  */
+  float dx, dz;              /* grid spacing (m)                                       */
+  int32_t nt;                /* time steps                                             */
+  float dt;                  /* time step (s)                                          */
+  int32_t nabc;              /* absorbing layers per side (>= 1)                       */
+  int32_t free_surface;      /* 0 / 1                                                  */
+  int32_t src_n;             /* shots (one point source each)                          */
+  int32_t rcv_n;             /* receivers shared by every shot                         */
+  int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
+  int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
+  int32_t snapshot_every;    /* >0: copy shot-0 pressure after every chunk of this size */
+  int32_t checkpoint_every;  /* k>0: keep the state every k steps for the gradient     */
+  int32_t checkpoint_bf16;   /* 1: store checkpoints as bf16 with per-field fp32 scale */
+  int32_t tile_rows;         /* launch choice: rows per tile (8 or 16; 0 = default)    */
+  int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
+} gbw_acoustic2d_problem;
+
+/* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model
+ * indices, as the reference's kernel receives them. */
+typedef struct gbw_acoustic2d_inputs {
+  const int64_t* src_x;      /* HOST (src_n,)                                          */
+  const int64_t* src_z;      /* HOST (src_n,)                                          */
+  const int64_t* rcv_x;      /* HOST (rcv_n,)                                          */
+  const int64_t* rcv_z;      /* HOST (rcv_n,)                                          */
+  const float* src_v;        /* DEVICE (src_n, nt) or (nt,)                            */
+  const float* damp;         /* DEVICE (nz + 2 nabc, nx + 2 nabc)                      */
+  const float* vp;           /* DEVICE (nz, nx)                                        */
+  const float* rho;          /* DEVICE (nz, nx)                                        */
+} gbw_acoustic2d_inputs;
+
+/* Outputs of the forward pass (the dict of acoustic_2d.py:491-500).  Any
+ * pointer may be NULL to skip that output. */
+typedef struct gbw_acoustic2d_outputs {
+  float* rec_p;              /* DEVICE (src_n, nt, rcv_n)  key 'p'                     */
+  float* rec_vx;             /* DEVICE (src_n, nt, rcv_n)  key 'vx'                    */
+  float* rec_vz;             /* DEVICE (src_n, nt, rcv_n)  key 'vz'                    */
+  float* energy_p;           /* DEVICE (nz, nx)  key 'forward_wavefield_p'             */
+  float* energy_vx;          /* DEVICE (nz, nx)  key 'forward_wavefield_vx'            */
+  float* energy_vz;          /* DEVICE (nz, nx)  key 'forward_wavefield_vz'            */
+  float* snapshots;          /* DEVICE (ceil(nt/snapshot_every), nz, nx)  'snapshots'  */
+  void* checkpoints;         /* DEVICE checkpoint_bytes(), needed when checkpoint_every */
+} gbw_acoustic2d_outputs;
+
+/* Gradient request: adjoint sources are dLoss/d(seismogram).  Any g_* may be
+ * NULL (zero); any grad_* may be NULL (not wanted). */
+typedef struct gbw_acoustic2d_adjoint {
+  const float* g_p;          /* DEVICE (src_n, nt, rcv_n)                              */
+  const float* g_vx;         /* DEVICE (src_n, nt, rcv_n)                              */
+  const float* g_vz;         /* DEVICE (src_n, nt, rcv_n)                              */
+  const void* checkpoints;   /* DEVICE buffer filled by gbw_acoustic2d_forward         */
+  float* grad_vp;            /* DEVICE (nz, nx)   overwritten                          */
+  float* grad_rho;           /* DEVICE (nz, nx)   overwritten; density as an input     */
+  float* grad_src_v;         /* DEVICE shape of src_v; overwritten                     */
+} gbw_acoustic2d_adjoint;
+
+int gbw_abi_version(void);
+const char* gbw_last_error(void);
+
+/* Bytes of device workspace for gbw_acoustic2d_forward (pass 0) or
+ * gbw_acoustic2d_backward (pass 1); 0 if the problem is invalid. */
+size_t gbw_acoustic2d_workspace_bytes(const gbw_acoustic2d_problem* prob, int pass);
+
+/* Bytes of the checkpoint buffer (0 when checkpoint_every == 0). */
+size_t gbw_acoustic2d_checkpoint_bytes(const gbw_acoustic2d_problem* prob);
+
+/* Forward modelling: all nt steps, seismograms, energy maps, snapshots and,
+ * when checkpoint_every > 0, the checkpoints the gradient needs.
+ * Replaces acoustic_forward_kernel (acoustic_2d.py:294-501). */
+int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob,
+                           const gbw_acoustic2d_inputs* in,
+                           const gbw_acoustic2d_outputs* out,
+                           void* workspace, size_t workspace_bytes,
+                           void* stream /* hipStream_t */);
+
+/* Gradient of sum(g_p*p + g_vx*vx + g_vz*vz) w.r.t. vp (through alpha1 and
+ * pad_model), rho and src_v: the hand-written adjoint of the forward step,
+ * recomputing each checkpoint interval.  Replaces PyTorch autograd through
+ * torch.utils.checkpoint(acoustic_step_forward) (acoustic_2d.py:448-460). */
+int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob,
+                            const gbw_acoustic2d_inputs* in,
+                            const gbw_acoustic2d_adjoint* adj,
+                            void* workspace, size_t workspace_bytes,
+                            void* stream /* hipStream_t */);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBW_ACOUSTIC2D_H */

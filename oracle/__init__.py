@@ -1,0 +1,221 @@
+"""CPU oracle for the 2D acoustic propagator -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker (or, for the
+baseline, as the thing timed on the host).  The product path
+(``geobrain_amd``) never calls it.
+
+Contents
+--------
+* ``forward`` / ``gradient``: ctypes front-end to ``acoustic2d_oracle.cpp``,
+  a C++ restatement of ``geobrain/physics/wave/kernels/acoustic_2d.py``
+  (FP32 in the reference's op order, or FP64) and of its exact adjoint.
+* ``bc_pml``: numpy restatement of ``geobrain/physics/wave/boundary/pml.py:16-98``.
+* ``pad_model``: clamp-index restatement of ``kernels/acoustic_2d.py:18-66``.
+* ``ricker``: float64 closed form of ``wavelets/ricker.py:82-93``.
+* ``gardner``: ``models/acoustic.py:169`` (``310 * vp ** 0.25`` in float32 numpy).
+* ``chunk_len``: the chunk size ``torch.chunk`` uses (``acoustic_2d.py:430``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_acoustic2d.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle shared library with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        D = ctypes.c_double
+        fwd_args = [I, I, I, I, D, D, I, I, P, P, P, I, P, P, P, P, P, I,
+                    P, P, P, P, P, P, P, P]
+        grad_args = [I, I, I, I, D, D, I, I, P, P, P, I, I, P, P, P, P, P, I,
+                     P, P, P, P, P, P]
+        for name in ("oracle_forward_f32", "oracle_forward_f64"):
+            fn = getattr(lib, name)
+            fn.argtypes = fwd_args
+            fn.restype = I
+        for name in ("oracle_gradient_f32", "oracle_gradient_f64"):
+            fn = getattr(lib, name)
+            fn.argtypes = grad_args
+            fn.restype = I
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def chunk_len(nt: int, checkpoint_segments: int) -> int:
+    """Size of every chunk but the last from ``torch.chunk(x, segs, -1)``."""
+    segs = max(1, int(checkpoint_segments))
+    return -(-nt // segs)
+
+
+def _common(nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp):
+    src_z = np.ascontiguousarray(src_z, dtype=np.int64).reshape(-1)
+    src_x = np.ascontiguousarray(src_x, dtype=np.int64).reshape(-1)
+    rcv_z = np.ascontiguousarray(rcv_z, dtype=np.int64).reshape(-1)
+    rcv_x = np.ascontiguousarray(rcv_x, dtype=np.int64).reshape(-1)
+    S = src_z.shape[0]
+    src_v = np.asarray(src_v, dtype=np.float32)
+    if src_v.ndim == 1:
+        src_v = np.broadcast_to(src_v, (S, src_v.shape[0]))
+    src_v = np.ascontiguousarray(src_v, dtype=np.float32)
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    damp = np.ascontiguousarray(damp, dtype=np.float32)
+    assert vp.shape == (nz, nx) and rho.shape == (nz, nx)
+    assert damp.shape == (nz + 2 * nabc, nx + 2 * nabc)
+    return src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp
+
+
+def forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
+            vp, rho, damp, chunk=0, snapshot_every=0, precision="f64",
+            return_state=False):
+    """Forward modelling; returns a dict with the reference's output keys.
+
+    ``chunk`` is the time-chunk length (``chunk_len(nt, segments)``); it only
+    changes how the energy maps are summed.  ``snapshot_every > 0`` selects the
+    reference's snapshot mode (fixed chunks, shot-0 pressure after each).
+    """
+    src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp = _common(
+        nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp)
+    S, nt = src_v.shape
+    R = rcv_z.shape[0]
+    dtp = np.float32 if precision == "f32" else np.float64
+    fn = getattr(_load(), "oracle_forward_" + precision)
+    out = {k: np.zeros((S, nt, R), dtp) for k in ("p", "vx", "vz")}
+    for k in ("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"):
+        out[k] = np.zeros((nz, nx), dtp)
+    clen = chunk
+    snaps = None
+    if snapshot_every > 0:
+        clen = snapshot_every
+        snaps = np.zeros((-(-nt // snapshot_every), nz, nx), dtp)
+    NZ, NX = nz + 2 * nabc, nx + 2 * nabc
+    state = np.zeros((3, S, NZ, NX), dtp) if return_state else None
+    rc = fn(nz, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+            _ptr(src_z), _ptr(src_x), _ptr(src_v), R, _ptr(rcv_z), _ptr(rcv_x),
+            _ptr(vp), _ptr(rho), _ptr(damp), int(clen),
+            _ptr(out["p"]), _ptr(out["vx"]), _ptr(out["vz"]),
+            _ptr(out["forward_wavefield_p"]), _ptr(out["forward_wavefield_vx"]),
+            _ptr(out["forward_wavefield_vz"]), _ptr(snaps), _ptr(state))
+    if rc != 0:
+        raise ValueError(f"oracle forward rejected its inputs (code {rc})")
+    if snaps is not None:
+        out["snapshots"] = snaps
+    if state is not None:
+        out["state"] = state
+    return out
+
+
+def gradient(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
+             vp, rho, damp, g_p=None, g_vx=None, g_vz=None, chunk=0,
+             precision="f64", wavelet_shared=False):
+    """Gradient of ``sum(g_p*p + g_vx*vx + g_vz*vz)`` w.r.t. vp, rho and the wavelet.
+
+    ``g_*`` are the adjoint sources (the loss's derivative w.r.t. each
+    seismogram), shape ``(S, nt, R)``.  Returns ``(grad_vp, grad_rho, grad_w)``;
+    ``grad_rho`` is the gradient for a density that is an independent input
+    (the reference's ``rho_grad=True`` case).
+    """
+    src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp = _common(
+        nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp)
+    S, nt = src_v.shape
+    R = rcv_z.shape[0]
+    dtp = np.float32 if precision == "f32" else np.float64
+    fn = getattr(_load(), "oracle_gradient_" + precision)
+
+    def conv(g):
+        if g is None:
+            return None
+        g = np.ascontiguousarray(g, dtype=dtp)
+        assert g.shape == (S, nt, R)
+        return g
+
+    g_p, g_vx, g_vz = conv(g_p), conv(g_vx), conv(g_vz)
+    gvp = np.zeros((nz, nx), dtp)
+    grho = np.zeros((nz, nx), dtp)
+    gw = np.zeros((nt,) if wavelet_shared else (S, nt), dtp)
+    rc = fn(nz, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+            _ptr(src_z), _ptr(src_x), _ptr(src_v), int(bool(wavelet_shared)), R,
+            _ptr(rcv_z), _ptr(rcv_x), _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
+            _ptr(g_p), _ptr(g_vx), _ptr(g_vz), _ptr(gvp), _ptr(grho), _ptr(gw))
+    if rc != 0:
+        raise ValueError(f"oracle gradient rejected its inputs (code {rc})")
+    return gvp, grho, gw
+
+
+# --------------------------------------------------------------------------
+# Host-side pieces of the reference path, restated in numpy.
+# --------------------------------------------------------------------------
+
+def bc_pml(nx, nz, dx, dz, pml, vmax, free_surface=True):
+    """Quadratic PML damping, ``geobrain/physics/wave/boundary/pml.py:16-98``.
+
+    kappa = -3 vmax ln(1e-6) / (2 a), a = (pml-1) dx; profile d[i] = kappa (i dx / a)^2
+    placed on the left/right columns over all rows, then top/bottom trapezoids
+    (the top one only without a free surface).  Returns float64 (NZ, NX).
+    """
+    NX = nx + 2 * pml
+    NZ = nz + pml if free_surface else nz + 2 * pml
+    a = (pml - 1) * dx
+    kappa = -3.0 * vmax * math.log(1e-6) / (2.0 * a)
+    prof = np.array([kappa * (i * dx / a) * (i * dx / a) for i in range(pml)])
+    d = np.zeros((NX, NZ))  # the reference fills an (x, z) array and transposes
+    for i in range(pml):
+        d[pml - i - 1, :] = prof[i]
+        d[NX - pml + i, :] = prof[i]
+    for i in range(pml):
+        x0, x1 = pml - i, NX - (pml - i)
+        if not free_surface:
+            d[x0:x1, pml - i - 1] = prof[i]
+        d[x0:x1, NZ - pml + i] = prof[i]
+    return d.T
+
+
+def pad_model(v, nabc):
+    """Edge-replicate padding (``kernels/acoustic_2d.py:18-66``) via clamped indices."""
+    v = np.asarray(v)
+    nz, nx = v.shape
+    ii = np.clip(np.arange(nz + 2 * nabc) - nabc, 0, nz - 1)
+    jj = np.clip(np.arange(nx + 2 * nabc) - nabc, 0, nx - 1)
+    return v[np.ix_(ii, jj)]
+
+
+def ricker(f0, dt, length_factor=2.2):
+    """Ricker wavelet in float64, ``wavelets/ricker.py:82-93``."""
+    nw = length_factor / f0 / dt
+    nw = 2 * int(nw / 2) + 1
+    nc = nw // 2
+    t = (np.arange(nw) - nc) * dt
+    arg = np.pi * f0 * t
+    return (1.0 - 2.0 * arg ** 2) * np.exp(-arg ** 2)
+
+
+def gardner(vp):
+    """Gardner density ``310 * vp ** 0.25`` in float32 numpy (``models/acoustic.py:169``)."""
+    return (310.0 * np.power(np.asarray(vp, dtype=np.float32), 0.25)).astype(np.float32)

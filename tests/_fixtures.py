@@ -1,0 +1,46 @@
+"""Helpers shared by the tests: golden-fixture loading and error metrics."""
+
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+FWD_CASES = ["tiny0_fwd", "tiny1_fwd", "tiny2_fwd", "tiny3_fwd", "tiny_snap", "c1_fwd"]
+GRAD_CASES = ["tiny0_grad", "tiny1_grad", "tiny2_grad", "tiny3_grad", "c1_grad", "c2r_grad"]
+OUT_KEYS = ("p", "vx", "vz", "forward_wavefield_p", "forward_wavefield_vx",
+            "forward_wavefield_vz", "snapshots")
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    return z, meta
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def chunk_len(nt, segments):
+    return -(-nt // max(1, int(segments)))
+
+
+def oracle_args(z, m):
+    return dict(nz=m["nz"], nx=m["nx"], nabc=m["nabc"], free_surface=m["free_surface"],
+                dt=m["dt"], dz=m["dx"], src_z=z["in_src_z"], src_x=z["in_src_x"],
+                src_v=z["in_wavelet"], rcv_z=z["in_rcv_z"], rcv_x=z["in_rcv_x"],
+                vp=z["in_vp"], rho=z["in_rho"], damp=z["in_damp"])
+
+
+def adjoint_sources(z, m, fwd_p=None, obs_p=None):
+    """The loss behind a gradient fixture, as adjoint sources (g_p, g_vx, g_vz)."""
+    S, R, nt = len(z["in_src_z"]), len(z["in_rcv_z"]), m["nt"]
+    if "g_seed" in m:
+        rng = np.random.default_rng(m["g_seed"])
+        return tuple(rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(3))
+    return (2.0 * (fwd_p - obs_p), None, None)

@@ -1,0 +1,229 @@
+"""Write golden vectors from the reference's own implementation.
+
+Run ONLY in the build container, where the reference checkout exists:
+
+    python tests/golden/make_golden.py [--only NAME ...]
+
+It loads two leaf modules of the reference by file path -- the FDTD kernel
+``geobrain/physics/wave/kernels/acoustic_2d.py`` (imports torch only) and the
+PML builder ``geobrain/physics/wave/boundary/pml.py`` (imports numpy only) --
+runs them on seeded inputs and stores inputs and outputs as ``.npz`` fixtures
+next to this script.  Nothing from the reference is copied; the fixtures are
+data.  These vectors pin what no closed form or invariant can: the reference's
+exact FP32 arithmetic of the time step and the gradient that PyTorch autograd
+produces through its reentrant ``torch.utils.checkpoint`` chunks.
+
+Gradient fixtures use the loss ``sum(g_p*p + g_vx*vx + g_vz*vz)`` with
+adjoint sources drawn from ``numpy.random.default_rng(seed)`` (stored), or,
+for the config-2-shaped cases, the L2 misfit ``sum((p - d_obs)**2)`` where
+``d_obs`` is the reference's own forward on a smoothed copy of the model.
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from geobrain_amd import synthetic  # noqa: E402  (our own input generators)
+
+
+def _load(name, rel):
+    spec = importlib.util.spec_from_file_location(name, os.path.join(REF, rel))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+ker = _load("_gb_ref_acoustic_2d", "geobrain/physics/wave/kernels/acoustic_2d.py")
+pml = _load("_gb_ref_pml", "geobrain/physics/wave/boundary/pml.py")
+
+
+def damp_for(nz, nx, dx, nabc, vmax):
+    # the propagator always asks for free_surface=False (propagators/acoustic.py:140)
+    d = pml.bc_pml(nx, nz, dx, dx, nabc, vmax, False)
+    return torch.tensor(d, dtype=torch.float32)
+
+
+def run_case(name, *, nz, nx, nabc, fs, dx, dt, nt, vp, rho, src_z, src_x, wavelet,
+             rcv_z, rcv_x, segments=1, snapshot_every=0, grad=None, g_seed=None,
+             d_obs_vp=None, rho_grad=False, w_grad=False, extra_meta=None):
+    vmax = float(np.max(vp))
+    damp = damp_for(nz, nx, dx, nabc, vmax)
+    vp_t = torch.tensor(vp, dtype=torch.float32)
+    rho_t = torch.tensor(rho, dtype=torch.float32)
+    w_t = torch.tensor(wavelet, dtype=torch.float32)
+    S = len(src_z)
+    R = len(rcv_z)
+    common = dict(nx=nx, nz=nz, dx=dx, dz=dx, nt=nt, dt=dt, nabc=nabc, free_surface=fs,
+                  src_x=torch.tensor(src_x, dtype=torch.long),
+                  src_z=torch.tensor(src_z, dtype=torch.long), src_n=S,
+                  rcv_x=torch.tensor(rcv_x, dtype=torch.long),
+                  rcv_z=torch.tensor(rcv_z, dtype=torch.long), rcv_n=R,
+                  damp=damp, device=torch.device("cpu"), dtype=torch.float32)
+    out = {}
+    meta = dict(name=name, nz=nz, nx=nx, nabc=nabc, free_surface=bool(fs), dx=dx, dt=dt,
+                nt=nt, segments=segments, snapshot_every=snapshot_every, vmax=vmax,
+                loss=None, torch=torch.__version__)
+    if extra_meta:
+        meta.update(extra_meta)
+    t0 = time.time()
+    if grad is None:
+        with torch.no_grad():
+            res = ker.acoustic_forward_kernel(src_v=w_t, vp=vp_t, rho=rho_t,
+                                              checkpoint_segments=segments,
+                                              snapshot_every=snapshot_every, **common)
+        for k, v in res.items():
+            out["out_" + k] = v.numpy()
+    else:
+        vp_t.requires_grad_(True)
+        if rho_grad:
+            rho_t.requires_grad_(True)
+        if w_grad:
+            w_t.requires_grad_(True)
+        if grad == "random":
+            rng = np.random.default_rng(g_seed)
+            gs = [rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(3)]
+            res = ker.acoustic_forward_kernel(src_v=w_t, vp=vp_t, rho=rho_t,
+                                              checkpoint_segments=segments, **common)
+            loss = sum((torch.tensor(g) * res[k]).sum() for g, k in zip(gs, ("p", "vx", "vz")))
+            meta["g_seed"] = g_seed
+        elif grad == "misfit":
+            with torch.no_grad():
+                obs = ker.acoustic_forward_kernel(
+                    src_v=w_t.detach(), vp=torch.tensor(d_obs_vp, dtype=torch.float32),
+                    rho=rho_t.detach(), checkpoint_segments=1, **common)["p"]
+            res = ker.acoustic_forward_kernel(src_v=w_t, vp=vp_t, rho=rho_t,
+                                              checkpoint_segments=segments, **common)
+            loss = ((res["p"] - obs) ** 2).sum()
+            out["in_vp_obs"] = d_obs_vp.astype(np.float32)
+        else:
+            raise ValueError(grad)
+        loss.backward()
+        meta["loss"] = float(loss.detach())
+        out["grad_vp"] = vp_t.grad.numpy()
+        if rho_grad:
+            out["grad_rho"] = rho_t.grad.numpy()
+        if w_grad:
+            out["grad_w"] = w_t.grad.numpy()
+        if grad == "random" or R * nt * S <= 400_000:
+            for k in ("p", "vx", "vz"):
+                out["out_" + k] = res[k].detach().numpy()
+        else:  # keep big fixtures small: a strided sample of the seismogram
+            out["out_p_sub"] = res["p"].detach().numpy()[:, ::10, ::10]
+        for k in ("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"):
+            out["out_" + k] = res[k].detach().numpy()
+    meta["ref_seconds"] = round(time.time() - t0, 2)
+    out.update(in_vp=vp.astype(np.float32), in_rho=rho.astype(np.float32),
+               in_damp=damp.numpy(), in_wavelet=np.asarray(wavelet, np.float32),
+               in_src_z=np.asarray(src_z, np.int64), in_src_x=np.asarray(src_x, np.int64),
+               in_rcv_z=np.asarray(rcv_z, np.int64), in_rcv_x=np.asarray(rcv_x, np.int64),
+               meta=np.array(json.dumps(meta)))
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: {meta['ref_seconds']} s, {os.path.getsize(path) / 1e6:.2f} MB", flush=True)
+
+
+def tiny_cases():
+    rng = np.random.default_rng(1234)
+    cases = []
+    for idx, (nz, nx, nabc, fs, S, R, nt, segs, sz) in enumerate([
+        (8, 12, 4, False, 2, 5, 40, 3, 2),
+        (9, 13, 4, True, 2, 6, 45, 4, 0),   # source on the free-surface row (never updated)
+        (10, 11, 3, True, 3, 4, 50, 1, 1),  # source just below the surface
+        (7, 16, 5, False, 1, 16, 30, 7, 3),
+    ]):
+        vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
+        rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+        src_x = rng.integers(0, nx, S)
+        src_z = np.full(S, sz)
+        rcv_x = rng.integers(0, nx, R)
+        rcv_z = rng.integers(0, nz, R)
+        w = rng.standard_normal((S, nt)).astype(np.float32)
+        base = dict(nz=nz, nx=nx, nabc=nabc, fs=fs, dx=10.0, dt=1e-3, nt=nt, vp=vp, rho=rho,
+                    src_z=src_z, src_x=src_x, wavelet=w, rcv_z=rcv_z, rcv_x=rcv_x)
+        cases.append((f"tiny{idx}_fwd", dict(base, segments=segs)))
+        cases.append((f"tiny{idx}_grad", dict(base, segments=segs, grad="random",
+                                              g_seed=100 + idx, rho_grad=(idx % 2 == 1),
+                                              w_grad=True)))
+    # snapshot mode and a shared (1-D) wavelet
+    nz, nx, nabc, nt = 9, 14, 4, 33
+    vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+    cases.append(("tiny_snap", dict(nz=nz, nx=nx, nabc=nabc, fs=True, dx=10.0, dt=1e-3,
+                                    nt=nt, vp=vp, rho=rho, src_z=np.array([1, 2]),
+                                    src_x=np.array([3, 9]),
+                                    wavelet=rng.standard_normal(nt).astype(np.float32),
+                                    rcv_z=np.array([0, 1, 4]), rcv_x=np.array([0, 7, 13]),
+                                    snapshot_every=7)))
+    return cases
+
+
+def c1_case(grad=False):
+    # 100 x 200, dx 10 m, dt 1 ms, 1000 steps, 5 layers from seed 0, 10 Hz Ricker
+    nz, nx, nt, dx, dt = 100, 200, 1000, 10.0, 1e-3
+    vp = synthetic.layered_model(nz, nx, n_layers=5, vmin=1500.0, vmax=3500.0, seed=0)
+    rho = synthetic.gardner(vp)
+    w = synthetic.ricker_padded(10.0, dt, nt)
+    base = dict(nz=nz, nx=nx, nabc=20, fs=False, dx=dx, dt=dt, nt=nt, vp=vp, rho=rho,
+                src_z=np.array([2]), src_x=np.array([100]), wavelet=w[None, :],
+                rcv_z=np.full(nx, 2), rcv_x=np.arange(nx))
+    if grad:
+        return ("c1_grad", dict(base, segments=20, grad="random", g_seed=7))
+    return ("c1_fwd", dict(base, segments=1))
+
+
+def c2_case(nz, nx, nt, name):
+    dx, dt, f0 = 5.0, 5e-4, 15.0
+    vp = synthetic.grf_model(nz, nx, dx, seed=0)
+    vp_obs = synthetic.smooth_model(vp, 0.05)
+    rho = synthetic.gardner(vp)
+    w = synthetic.ricker_padded(f0, dt, nt)
+    segments = -(-nt // 50)
+    return (name, dict(nz=nz, nx=nx, nabc=20, fs=False, dx=dx, dt=dt, nt=nt, vp=vp, rho=rho,
+                       src_z=np.array([2]), src_x=np.array([nx // 2]), wavelet=w[None, :],
+                       rcv_z=np.full(nx, 2), rcv_x=np.arange(nx), segments=segments,
+                       grad="misfit", d_obs_vp=vp_obs,
+                       extra_meta=dict(recipe="synthetic.grf_model(seed=0); obs model "
+                                              "synthetic.smooth_model(vp, 0.05)")))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*")
+    ap.add_argument("--threads", type=int, default=8)
+    args = ap.parse_args()
+    torch.set_num_threads(args.threads)
+    cases = tiny_cases() + [c1_case(False), c1_case(True),
+                            c2_case(120, 400, 1200, "c2r_grad")]
+    if args.only and "c2_full" in args.only:
+        cases.append(c2_case(350, 1700, 4000, "c2_full_grad"))
+    # PML arrays for several shapes (free-surface variants included)
+    for name, kw in cases:
+        if args.only and name not in args.only:
+            continue
+        run_case(name, **kw)
+    if not args.only or "pml" in args.only:
+        out = {}
+        for i, (nx, nz, dx, n, vmax, fs) in enumerate([(200, 100, 10.0, 20, 3500.0, False),
+                                                        (200, 88, 40.0, 30, 4700.0, True),
+                                                        (13, 9, 5.0, 4, 2000.0, False),
+                                                        (17, 6, 7.5, 3, 1234.5, True)]):
+            out[f"args{i}"] = np.array([nx, nz, dx, n, vmax, float(fs)])
+            out[f"pml{i}"] = pml.bc_pml(nx, nz, dx, dx, n, vmax, fs)
+        np.savez_compressed(os.path.join(HERE, "pml.npz"), **out)
+        print("pml: done")
+
+
+if __name__ == "__main__":
+    main()

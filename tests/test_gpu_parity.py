@@ -1,0 +1,242 @@
+"""GPU parity of the HIP propagator against the reference's golden vectors and
+the CPU oracle.  Bar: relative L2 <= 1e-5 for FP32 seismograms, energy maps
+and gradients (the tolerance is written in each assert); the error against the
+FP64 oracle is printed for each result."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from geobrain_amd import _native
+from geobrain_amd.physics import wave
+from geobrain_amd.physics.wave import kernels as K
+from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len,
+                             load, oracle_args, rel_l2)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+DEV = torch.device("cuda:0")
+
+
+def _t(a, **kw):
+    return torch.as_tensor(np.asarray(a), device=DEV, **kw)
+
+
+def run(z, m, *, vp=None, segments=None, snapshot_every=None, grad=(), launch=None, wavelet=None):
+    """Runs acoustic_forward_kernel on a fixture's inputs; returns (outputs, tensors)."""
+    K.set_launch_choice(launch)
+    try:
+        vp_t = _t(z["in_vp"] if vp is None else vp, dtype=torch.float32).clone()
+        rho_t = _t(z["in_rho"], dtype=torch.float32).clone()
+        w_t = _t(z["in_wavelet"] if wavelet is None else wavelet, dtype=torch.float32).clone()
+        vp_t.requires_grad_("vp" in grad)
+        rho_t.requires_grad_("rho" in grad)
+        w_t.requires_grad_("w" in grad)
+        out = wave.acoustic_forward_kernel(
+            nx=m["nx"], nz=m["nz"], dx=m["dx"], dz=m["dx"], nt=m["nt"], dt=m["dt"],
+            nabc=m["nabc"], free_surface=m["free_surface"],
+            src_x=_t(z["in_src_x"]), src_z=_t(z["in_src_z"]), src_n=len(z["in_src_x"]),
+            src_v=w_t, rcv_x=_t(z["in_rcv_x"]), rcv_z=_t(z["in_rcv_z"]),
+            rcv_n=len(z["in_rcv_x"]), damp=_t(z["in_damp"], dtype=torch.float32),
+            vp=vp_t, rho=rho_t,
+            checkpoint_segments=m["segments"] if segments is None else segments,
+            snapshot_every=m["snapshot_every"] if snapshot_every is None else snapshot_every)
+    finally:
+        K.set_launch_choice(None)
+    return out, dict(vp=vp_t, rho=rho_t, w=w_t)
+
+
+def np_(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("name", FWD_CASES + GRAD_CASES)
+def test_forward_matches_reference(name):
+    print(f"[case] forward {name}", flush=True)
+    z, m = load(name)
+    out, _ = run(z, m)
+    torch.cuda.synchronize()
+    o64 = oracle.forward(**oracle_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
+                         snapshot_every=m["snapshot_every"], precision="f64")
+    n = 0
+    for k in OUT_KEYS:
+        if "out_" + k in z:
+            got = np_(out[k])
+            e_ref = rel_l2(got, z["out_" + k])
+            e64 = rel_l2(got, o64[k])
+            same = np.array_equal(got, z["out_" + k])
+            print(f"  {k}: rel-L2 vs reference {e_ref:.2e} (bitwise={same}), vs FP64 oracle {e64:.2e}")
+            assert e_ref <= TOL, k
+            n += 1
+    if "out_p_sub" in z:
+        assert rel_l2(np_(out["p"])[:, ::10, ::10], z["out_p_sub"]) <= TOL
+    assert n >= 3 or "out_p_sub" in z
+
+
+@pytest.mark.parametrize("name", GRAD_CASES)
+def test_gradient_matches_reference_autograd(name):
+    print(f"[case] gradient {name}", flush=True)
+    z, m = load(name)
+    want = ["vp"] + (["rho"] if "grad_rho" in z else []) + (["w"] if "grad_w" in z else [])
+    obs = None
+    if "g_seed" not in m:
+        with torch.no_grad():
+            obs, _ = run(z, m, vp=z["in_vp_obs"], segments=1)
+            obs = obs["p"]
+    out, ts = run(z, m, grad=want)
+    if "g_seed" in m:
+        g = adjoint_sources(z, m)
+        loss = sum((_t(gk) * out[k]).sum() for gk, k in zip(g, ("p", "vx", "vz")))
+    else:
+        loss = ((out["p"] - obs) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    if m.get("loss") is not None:
+        print(f"  loss {float(loss):.7e} vs reference {m['loss']:.7e}")
+    args = oracle_args(z, m)
+    if "g_seed" in m:
+        g64 = adjoint_sources(z, m)
+    else:
+        f64 = oracle.forward(**args, precision="f64")["p"]
+        o64 = oracle.forward(**dict(args, vp=z["in_vp_obs"]), precision="f64")["p"]
+        g64 = (2.0 * (f64 - o64), None, None)
+    gv64, gr64, gw64 = oracle.gradient(**args, g_p=g64[0], g_vx=g64[1], g_vz=g64[2],
+                                       chunk=chunk_len(m["nt"], m["segments"]), precision="f64",
+                                       wavelet_shared=z["in_wavelet"].ndim == 1)
+    for key, gkey, g64k in (("vp", "grad_vp", gv64), ("rho", "grad_rho", gr64), ("w", "grad_w", gw64)):
+        if key in want:
+            got = np_(ts[key].grad)
+            e_ref = rel_l2(got, z[gkey])
+            e64 = rel_l2(got, g64k)
+            print(f"  grad {key}: rel-L2 vs reference autograd {e_ref:.2e}, vs FP64 oracle {e64:.2e}"
+                  f" (reference vs FP64: {rel_l2(z[gkey], g64k):.2e})")
+            assert e_ref <= TOL, key
+
+
+@pytest.mark.parametrize("tile_rows,spb", [(8, 1), (16, 0), (16, 2), (8, 3)])
+def test_launch_choices_do_not_change_results(tile_rows, spb):
+    print(f"[case] launch tile_rows={tile_rows} spb={spb}", flush=True)
+    z, m = load("tiny2_grad")
+    base, tb = run(z, m, grad=("vp", "w"))
+    alt, ta = run(z, m, grad=("vp", "w"), launch=K.LaunchChoice(tile_rows, spb))
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_array_equal(np_(base[k]), np_(alt[k]))
+    for k in ("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"):
+        assert rel_l2(np_(alt[k]), np_(base[k])) <= 1e-6
+    g = adjoint_sources(z, m)
+    for out, ts in ((base, tb), (alt, ta)):
+        sum((_t(gk) * out[k]).sum() for gk, k in zip(g, ("p", "vx", "vz"))).backward()
+    assert rel_l2(np_(ta["vp"].grad), np_(tb["vp"].grad)) <= 1e-6
+    np.testing.assert_array_equal(np_(ta["w"].grad), np_(tb["w"].grad))
+
+
+def test_batched_shots_equal_single_shots():
+    print("[case] batch vs single shots", flush=True)
+    z, m = load("tiny2_grad")
+    full, _ = run(z, m)
+    for s in range(len(z["in_src_x"])):
+        zz = {k: z[k] for k in z.files}
+        zz["in_src_x"], zz["in_src_z"] = z["in_src_x"][s:s + 1], z["in_src_z"][s:s + 1]
+        zz["in_wavelet"] = z["in_wavelet"][s:s + 1]
+        one, _ = run(zz, m)
+        for k in ("p", "vx", "vz"):
+            np.testing.assert_array_equal(np_(one[k])[0], np_(full[k])[s])
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_propagator_end_to_end(fs):
+    """AcousticPropagator as examples 07/09 drive it (free surface, Gardner
+    density, integrated Ricker), against the FP32 oracle."""
+    print(f"[case] propagator fs={fs}", flush=True)
+    from scipy.integrate import cumulative_trapezoid
+    nz, nx, nt, dx, dt = 40, 90, 600, 10.0, 1e-3
+    vp = oracle_vp = np.asarray(
+        np.linspace(1500, 3000, nz)[:, None] * np.ones((1, nx)), np.float32)
+    grid = wave.GridConfig(nx=nx, nz=nz, dx=dx, dz=dx)
+    bnd = wave.BoundaryConfig(type="pml", n_layers=12, free_surface=fs)
+    model = wave.AcousticModel(grid, bnd, vp=vp, rho=np.ones_like(vp), device="cuda")
+    w, _ = wave.RickerWavelet()(f0=10.0, dt=dt)
+    w = np.pad(w.numpy(), (0, nt - len(w)))
+    w = cumulative_trapezoid(w, initial=0)
+    src = wave.Source(nt=nt, dt=dt, f0=10.0)
+    for x in (5, 45, 80):
+        src.add_source(x=x, z=1, wavelet=w)
+    rcv = wave.Receiver(nt=nt, dt=dt)
+    rcv.add_receivers(x=np.arange(nx), z=np.ones(nx, int), rcv_type="pr")
+    prop = wave.AcousticPropagator(model, wave.Survey(src, rcv), device="cuda")
+    res = prop.forward(checkpoint_segments=4)
+    rho = synthetic_rho = (310.0 * np.power(oracle_vp, 0.25)).astype(np.float32)
+    damp = oracle.bc_pml(nx, nz, dx, dx, 12, float(vp.max()), False).astype(np.float32)
+    ref = oracle.forward(nz=nz, nx=nx, nabc=12, free_surface=fs, dt=dt, dz=dx,
+                         src_z=[1, 1, 1], src_x=[5, 45, 80],
+                         src_v=np.stack([w] * 3).astype(np.float32), rcv_z=np.ones(nx, int),
+                         rcv_x=np.arange(nx), vp=vp, rho=synthetic_rho, damp=damp,
+                         chunk=chunk_len(nt, 4), precision="f32")
+    np.testing.assert_array_equal(model.rho.detach().cpu().numpy(), rho)
+    for k in ("p", "vx", "vz", "forward_wavefield_p"):
+        e = rel_l2(np_(res[k]), ref[k])
+        print(f"  {k}: rel-L2 vs FP32 oracle {e:.2e}")
+        assert e <= TOL
+    snap = prop.forward(shot_index=1, snapshot_every=10)
+    assert snap.snapshots.shape == (60, nz, nx) and snap.snapshots.device.type == "cpu"
+    np.testing.assert_array_equal(np_(snap.p)[0], np_(res.p)[1])
+
+
+def test_dot_product_fp32():
+    """<F w, g> vs <w, F^T g> through the HIP forward and adjoint."""
+    print("[case] dot product", flush=True)
+    z, m = load("c2r_grad")
+    rng = np.random.default_rng(0)
+    out, ts = run(z, m, grad=("w",))
+    S, nt, R = out["p"].shape
+    gs = [_t(rng.standard_normal((S, nt, R)).astype(np.float32)) for _ in range(3)]
+    lhs = sum(float((g.double() * out[k].double()).sum()) for g, k in zip(gs, ("p", "vx", "vz")))
+    sum((g * out[k]).sum() for g, k in zip(gs, ("p", "vx", "vz"))).backward()
+    rhs = float((ts["w"].grad.double() * ts["w"].detach().double()).sum())
+    print(f"  lhs {lhs:.9e} rhs {rhs:.9e} rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_invalid_geometry_raises_before_launch():
+    z, m = load("tiny0_fwd")
+    bad = {k: z[k] for k in z.files}
+    bad["in_src_x"] = np.array([0, m["nx"] + m["nabc"]])
+    with pytest.raises(IndexError, match="index out of range"):
+        run(bad, m)
+    bad = {k: z[k] for k in z.files}
+    bad["in_rcv_z"] = bad["in_rcv_z"].copy()
+    bad["in_rcv_z"][0] = m["nz"] + m["nabc"] - 1  # vz has one row fewer than p
+    with pytest.raises(IndexError, match="receiver 0"):
+        run(bad, m)
+    with pytest.raises(ValueError, match="src_v"):
+        run(z, m, wavelet=z["in_wavelet"][:, :5])
+
+
+def test_full_size_config2_exact_scaling_and_batch():
+    """350 x 1700, 4000 steps: doubling the wavelet doubles every output bit for
+    bit (scaling by 2 is exact in binary floating point), and a 4-shot batch
+    reproduces each shot run alone."""
+    print("[case] full-size config 2 scaling", flush=True)
+    from geobrain_amd import synthetic
+    nz, nx, nt = 350, 1700, 4000
+    vp = synthetic.grf_model(nz, nx, 5.0, seed=0)
+    w = synthetic.ricker_padded(15.0, 5e-4, nt)
+    damp = wave.bc_pml(nx, nz, 5.0, 5.0, 20, float(vp.max()), False).astype(np.float32)
+    xs = synthetic.surface_shots(nx, 4)
+    z = dict(in_vp=vp, in_rho=synthetic.gardner(vp), in_damp=damp,
+             in_wavelet=np.stack([w] * 4), in_src_x=xs, in_src_z=np.full(4, 2),
+             in_rcv_x=np.arange(nx), in_rcv_z=np.full(nx, 2))
+    m = dict(nz=nz, nx=nx, nt=nt, dx=5.0, dt=5e-4, nabc=20, free_surface=False, segments=80,
+             snapshot_every=0)
+    with torch.no_grad():
+        a, _ = run(z, m)
+        b, _ = run(z, m, wavelet=2 * z["in_wavelet"])
+        for k in ("p", "vx", "vz"):
+            assert torch.equal(2 * a[k], b[k]), k
+        z1 = dict(z, in_src_x=xs[2:3], in_src_z=np.full(1, 2), in_wavelet=z["in_wavelet"][2:3])
+        c, _ = run(z1, m)
+        assert torch.equal(c["p"][0], a["p"][2])
+    assert float(a["p"].abs().max()) > 0

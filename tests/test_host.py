@@ -1,0 +1,180 @@
+"""CPU-side checks of the package: the C-ABI library loads and exports every
+declared symbol, host-side mirrors of the reference's types behave as the
+reference's do, and the product path refuses to run without a GPU."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from geobrain_amd import _native, synthetic
+from geobrain_amd.physics import wave
+from tests._fixtures import load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "gbw_acoustic2d.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gbw_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    names = _declared_functions()
+    assert set(names) == set(_native.EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.gbw_abi_version() == _native.ABI_VERSION
+
+
+def _problem(**kw):
+    p = _native.Problem()
+    base = dict(nx=200, nz=100, dx=10.0, dz=10.0, nt=1000, dt=1e-3, nabc=20, free_surface=0,
+                src_n=1, rcv_n=200, wavelet_shared=0, energy_chunk=0, snapshot_every=0,
+                checkpoint_every=50)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_workspace_queries():
+    lib = _native.load()
+    p = _problem()
+    fwd = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 0)
+    bwd = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 1)
+    bwd_rho = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 2)
+    ck = lib.gbw_acoustic2d_checkpoint_bytes(ctypes.byref(p))
+    plane = 140 * 256 * 4
+    assert fwd >= 2 * 3 * plane + 5 * plane
+    assert bwd >= fwd + 50 * plane
+    assert bwd_rho > bwd
+    assert ck == 20 * 3 * plane
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(nx=0)), 0) == 0
+    assert "Grid dimensions must be positive" in _native.last_error()
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(tile_rows=5)), 0) == 0
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(checkpoint_every=0)), 1) == 0
+
+
+def test_status_codes_map_to_reference_exceptions():
+    assert _native.STATUS[-1] is ValueError
+    assert _native.STATUS[-2] is IndexError
+    assert _native.STATUS[-5] is NotImplementedError
+
+
+def test_bc_pml_matches_reference():
+    z = np.load(os.path.join(ROOT, "tests", "golden", "pml.npz"))
+    for i in range(4):
+        nx, nz, dx, n, vmax, fs = z[f"args{i}"]
+        got = wave.bc_pml(int(nx), int(nz), dx, dx, int(n), vmax, bool(fs))
+        np.testing.assert_array_equal(got, z[f"pml{i}"])
+    # the fixtures' damping arrays were built by the reference's bc_pml too
+    zz, m = load("c1_fwd")
+    d = wave.create_boundary('pml', m["nx"], m["nz"], m["dx"], m["dx"], m["nabc"], m["vmax"])
+    np.testing.assert_array_equal(d.astype(np.float32), zz["in_damp"])
+
+
+def test_create_boundary_errors():
+    with pytest.raises(ValueError, match="vmax is required"):
+        wave.create_boundary('pml', 10, 10, 5.0, 5.0, 4)
+    with pytest.raises(ValueError, match="Unknown boundary type"):
+        wave.create_boundary('nope', 10, 10, 5.0, 5.0, 4, vmax=2000.0)
+    with pytest.raises(NotImplementedError):
+        wave.create_boundary('gerjan', 10, 10, 5.0, 5.0, 4, vmax=2000.0)
+
+
+def test_configs_validate_like_reference():
+    with pytest.raises(ValueError, match="Grid spacing must be equal"):
+        wave.GridConfig(nx=10, nz=10, dx=5.0, dz=4.0)
+    with pytest.raises(ValueError, match="Grid dimensions must be positive"):
+        wave.GridConfig(nx=0, nz=10, dx=5.0, dz=5.0)
+    with pytest.raises(ValueError, match="Unknown boundary type"):
+        wave.BoundaryConfig(type="abc")
+    b = wave.BoundaryConfig()
+    assert (b.type, b.n_layers, b.free_surface, b.alpha) == ("pml", 20, False, 0.0053)
+    g = wave.GridConfig(nx=200, nz=100, dx=10.0, dz=10.0)
+    assert g.shape == (100, 200) and g.ndim == 2
+
+
+def test_wave_result_dict_access():
+    t = torch.zeros(2, 3, 4)
+    r = wave.WaveResult.from_dict({k: t for k in ("p", "vx", "vz", "forward_wavefield_p",
+                                                  "forward_wavefield_vx", "forward_wavefield_vz")})
+    assert r["p"] is t and "p" in r and "snapshots" not in r
+    assert r.n_shots == 2 and r.nt == 3 and r.n_receivers == 4
+    with pytest.raises(KeyError):
+        r["nope"]
+
+
+def test_survey_and_source():
+    src = wave.Source(nt=5, dt=1e-3, f0=10.0)
+    with pytest.raises(ValueError, match="Wavelet length"):
+        src.add_source(1, 2, np.zeros(4))
+    src.add_sources(np.array([1, 2]), np.array([3, 3]), np.ones(5))
+    assert src.get_loc().tolist() == [[1, 3], [2, 3]] and src.get_wavelet().shape == (2, 5)
+    rcv = wave.Receiver(nt=5, dt=1e-3)
+    with pytest.raises(ValueError, match="Receiver type"):
+        rcv.add_receiver(1, 1, "xx")
+    rcv.add_receivers(np.arange(4), np.zeros(4, int), "pr")
+    s = wave.Survey(src, rcv)
+    assert (s.n_shots, s.n_receivers, s.nt) == (2, 4, 5)
+    with pytest.raises(ValueError, match="Receiver mask shape"):
+        s.set_receiver_masks(np.ones((3, 4)))
+
+
+def test_acoustic_model_gardner_and_clip():
+    g = wave.GridConfig(nx=6, nz=4, dx=5.0, dz=5.0)
+    vp = np.full((4, 6), 2500.0, np.float32)
+    vp[0, 0] = 9000.0
+    m = wave.AcousticModel(g, wave.BoundaryConfig(), vp=vp, rho=np.ones((4, 6), np.float32),
+                           vp_bound=(1500.0, 5000.0), vp_grad=True)
+    m.forward()
+    assert float(m.vp[0, 0]) == 5000.0
+    np.testing.assert_array_equal(m.rho.detach().numpy()[1:, 1:],
+                                  synthetic.gardner(vp)[1:, 1:])
+    assert not m.rho.requires_grad and m.vp.requires_grad
+
+
+def test_ricker_closed_form():
+    w, t = wave.RickerWavelet()(f0=15.0, dt=5e-4)
+    tt = t.numpy().astype(np.float64)
+    ref = (1 - 2 * (np.pi * 15.0 * tt) ** 2) * np.exp(-(np.pi * 15.0 * tt) ** 2)
+    np.testing.assert_allclose(w.numpy(), ref, atol=2e-6)
+    assert len(w) % 2 == 1 and float(w[len(w) // 2]) == 1.0
+
+
+def test_synthetic_generators_are_seeded():
+    a = synthetic.grf_model(60, 90, 5.0, seed=3)
+    b = synthetic.grf_model(60, 90, 5.0, seed=3)
+    np.testing.assert_array_equal(a, b)
+    assert a.min() >= 1500 and a.max() <= 4700
+    lay = synthetic.layered_model(100, 200, seed=0)
+    assert len(np.unique(lay)) == 5 and np.all(lay == lay[:, :1])
+    assert synthetic.surface_shots(1700, 64)[0] == 13
+    probs = synthetic.uneven_batch()
+    assert len(probs) == 96 and probs == synthetic.uneven_batch()
+    assert all(200 <= p["nz"] <= 600 and 200 <= p["nx"] <= 1200 and 1 <= p["n_shots"] <= 8
+               for p in probs)
+
+
+def test_product_path_refuses_cpu_tensors():
+    z, m = load("tiny0_fwd")
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        wave.acoustic_forward_kernel(
+            nx=m["nx"], nz=m["nz"], dx=m["dx"], dz=m["dx"], nt=m["nt"], dt=m["dt"],
+            nabc=m["nabc"], free_surface=m["free_surface"],
+            src_x=torch.tensor(z["in_src_x"]), src_z=torch.tensor(z["in_src_z"]),
+            src_n=len(z["in_src_x"]), src_v=torch.tensor(z["in_wavelet"]),
+            rcv_x=torch.tensor(z["in_rcv_x"]), rcv_z=torch.tensor(z["in_rcv_z"]),
+            rcv_n=len(z["in_rcv_x"]), damp=torch.tensor(z["in_damp"]),
+            vp=torch.tensor(z["in_vp"]), rho=torch.tensor(z["in_rho"]))
+
+
+def test_pad_model_tensor_form():
+    v = torch.arange(12, dtype=torch.float32).reshape(3, 4)
+    p = wave.pad_model(v, 2, 3, 4)
+    assert p.shape == (7, 8) and p[0, 0] == 0 and p[-1, -1] == 11 and p[3, 0] == 4

@@ -1,0 +1,206 @@
+"""Pins the CPU oracle: against the reference's own golden vectors, and against
+what the mathematics fixes independently of the reference (dot-product test,
+finite differences in FP64, linearity, translation and mirror symmetry,
+the energy-map / seismogram identity).  CPU only."""
+
+import numpy as np
+import pytest
+
+import oracle
+from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len,
+                             load, oracle_args, rel_l2)
+
+
+@pytest.mark.parametrize("name", FWD_CASES + GRAD_CASES)
+def test_oracle_fp32_forward_is_bitwise_reference(name):
+    z, m = load(name)
+    out = oracle.forward(**oracle_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
+                         snapshot_every=m["snapshot_every"], precision="f32")
+    checked = 0
+    for k in OUT_KEYS:
+        if "out_" + k in z:
+            np.testing.assert_array_equal(out[k], z["out_" + k], err_msg=k)
+            checked += 1
+    assert checked >= 3
+
+
+@pytest.mark.parametrize("name", FWD_CASES)
+def test_oracle_fp64_forward_close_to_reference(name):
+    z, m = load(name)
+    out = oracle.forward(**oracle_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
+                         snapshot_every=m["snapshot_every"], precision="f64")
+    for k in OUT_KEYS:
+        if "out_" + k in z:
+            assert rel_l2(out[k], z["out_" + k]) < 1e-5, k
+
+
+@pytest.mark.parametrize("name", GRAD_CASES)
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_oracle_gradient_matches_reference_autograd(name, precision):
+    z, m = load(name)
+    args = oracle_args(z, m)
+    obs = fwd = None
+    if "g_seed" not in m:
+        fwd = oracle.forward(**args, precision="f32")["p"]
+        obs = oracle.forward(**dict(args, vp=z["in_vp_obs"]), precision="f32")["p"]
+    g = adjoint_sources(z, m, fwd, obs)
+    gv, gr, gw = oracle.gradient(**args, g_p=g[0], g_vx=g[1], g_vz=g[2],
+                                 chunk=chunk_len(m["nt"], m["segments"]), precision=precision,
+                                 wavelet_shared=z["in_wavelet"].ndim == 1)
+    assert rel_l2(gv, z["grad_vp"]) < 1e-5
+    if "grad_rho" in z:
+        assert rel_l2(gr, z["grad_rho"]) < 1e-5
+    if "grad_w" in z:
+        assert rel_l2(gw, z["grad_w"]) < 1e-5
+
+
+def _small_problem(seed=0, nz=12, nx=15, nabc=4, fs=False, S=2, R=7, nt=60):
+    rng = np.random.default_rng(seed)
+    vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+    damp = oracle.bc_pml(nx, nz, 10.0, 10.0, nabc, float(vp.max()), False).astype(np.float32)
+    return dict(nz=nz, nx=nx, nabc=nabc, free_surface=fs, dt=1e-3, dz=10.0,
+                src_z=rng.integers(0, nz, S), src_x=rng.integers(0, nx, S),
+                src_v=rng.standard_normal((S, nt)).astype(np.float32),
+                rcv_z=rng.integers(0, nz, R), rcv_x=rng.integers(0, nx, R),
+                vp=vp, rho=rho, damp=damp)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_oracle_dot_product(fs):
+    """<F w, g> == <w, F^T g>: the adjoint propagation is the exact transpose."""
+    args = _small_problem(seed=3, fs=fs)
+    rng = np.random.default_rng(5)
+    S, nt, R = args["src_v"].shape[0], args["src_v"].shape[1], len(args["rcv_z"])
+    gs = [rng.standard_normal((S, nt, R)) for _ in range(3)]
+    out = oracle.forward(**args, precision="f64")
+    lhs = sum(float(np.sum(out[k] * g)) for k, g in zip(("p", "vx", "vz"), gs))
+    _, _, gw = oracle.gradient(**args, g_p=gs[0], g_vx=gs[1], g_vz=gs[2], chunk=7,
+                               precision="f64")
+    rhs = float(np.sum(args["src_v"].astype(np.float64) * gw))
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_oracle_gradient_finite_differences(fs):
+    """Central differences in FP64 on vp and rho (the imaging condition and chain rule)."""
+    args = _small_problem(seed=11, fs=fs, nt=50)
+    rng = np.random.default_rng(2)
+    S, nt, R = args["src_v"].shape[0], args["src_v"].shape[1], len(args["rcv_z"])
+    gs = [rng.standard_normal((S, nt, R)) for _ in range(3)]
+
+    def loss(vp=None, rho=None):
+        a = dict(args)
+        if vp is not None:
+            a["vp"] = vp
+        if rho is not None:
+            a["rho"] = rho
+        out = oracle.forward(**a, precision="f64")
+        return sum(float(np.sum(out[k] * g)) for k, g in zip(("p", "vx", "vz"), gs))
+
+    gv, gr, _ = oracle.gradient(**args, g_p=gs[0], g_vx=gs[1], g_vz=gs[2], chunk=9,
+                                precision="f64")
+    # the oracle reads float32 model values, so perturb by an exactly representable step
+    for (a, b) in [(0, 0), (5, 7), (11, 14), (3, 0), (6, 9)]:
+        for name, grad, h in (("vp", gv, 0.5), ("rho", gr, 0.25)):
+            base = args[name].copy()
+            up, dn = base.copy(), base.copy()
+            up[a, b] += h
+            dn[a, b] -= h
+            fd = (loss(**{name: up}) - loss(**{name: dn})) / (2 * h)
+            assert abs(fd - grad[a, b]) <= 1e-5 * abs(grad).max(), (name, a, b, fd, grad[a, b])
+
+
+def test_oracle_linear_and_zero_wavelet():
+    args = _small_problem(seed=4)
+    zero = oracle.forward(**dict(args, src_v=np.zeros_like(args["src_v"])), precision="f32")
+    for k in ("p", "vx", "vz", "forward_wavefield_p"):
+        assert not np.any(zero[k])
+    w2 = np.random.default_rng(9).standard_normal(args["src_v"].shape).astype(np.float32)
+    o1 = oracle.forward(**args, precision="f64")
+    o2 = oracle.forward(**dict(args, src_v=w2), precision="f64")
+    o12 = oracle.forward(**dict(args, src_v=args["src_v"] + w2), precision="f64")
+    for k in ("p", "vx", "vz"):
+        assert rel_l2(o12[k], o1[k] + o2[k]) < 1e-7
+
+
+def test_oracle_translation_invariance_is_bitwise():
+    """Homogeneous interior, no PML contact: shifting source and receivers shifts
+    the traces exactly (the undamped update multiplies by exactly 1)."""
+    nz, nx, nabc, nt = 40, 80, 10, 40
+    vp = np.full((nz, nx), 2000.0, np.float32)
+    rho = np.full((nz, nx), 2100.0, np.float32)
+    damp = oracle.bc_pml(nx, nz, 10.0, 10.0, nabc, 2000.0, False).astype(np.float32)
+    w = np.random.default_rng(1).standard_normal((1, nt)).astype(np.float32)
+    base = dict(nz=nz, nx=nx, nabc=nabc, free_surface=False, dt=1e-3, dz=10.0, src_v=w,
+                vp=vp, rho=rho, damp=damp, precision="f32")
+    rx = np.arange(30, 36)
+    o1 = oracle.forward(**base, src_z=[20], src_x=[33], rcv_z=np.full(6, 18), rcv_x=rx)
+    o2 = oracle.forward(**base, src_z=[20], src_x=[45], rcv_z=np.full(6, 18), rcv_x=rx + 12)
+    assert np.any(o1["p"])
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_array_equal(o1[k], o2[k])
+
+
+def test_oracle_mirror_symmetry():
+    """Model symmetric in x, source on the centre column: p is even about the
+    source column and vx (staggered half a cell right) is odd."""
+    nz, nx, nabc, nt = 20, 31, 6, 80
+    rng = np.random.default_rng(3)
+    half = rng.uniform(1500, 3000, (nz, nx // 2 + 1)).astype(np.float32)
+    vp = np.concatenate([half, half[:, -2::-1]], axis=1)
+    rho = np.full((nz, nx), 2000.0, np.float32)
+    damp = oracle.bc_pml(nx, nz, 10.0, 10.0, nabc, float(vp.max()), False).astype(np.float32)
+    xc = nx // 2
+    d = np.arange(1, 8)
+    out = oracle.forward(nz=nz, nx=nx, nabc=nabc, free_surface=True, dt=1e-3, dz=10.0,
+                         src_z=[3], src_x=[xc],
+                         src_v=rng.standard_normal((1, nt)).astype(np.float32),
+                         rcv_z=np.full(3 * len(d), 5),
+                         rcv_x=np.concatenate([xc + d, xc - d, xc - d - 1]),
+                         vp=vp, rho=rho, damp=damp, precision="f64")
+    n = len(d)
+    p, vx = out["p"][0], out["vx"][0]
+    scale = np.abs(p).max()
+    np.testing.assert_allclose(p[:, :n], p[:, n:2 * n], atol=1e-12 * scale)
+    np.testing.assert_allclose(vx[:, :n], -vx[:, 2 * n:], atol=1e-12 * np.abs(vx).max())
+
+
+def test_oracle_energy_equals_sum_of_squared_records():
+    args = _small_problem(seed=8, S=3, R=5)
+    out = oracle.forward(**args, precision="f64")
+    for k in ("p", "vx", "vz"):
+        e = out["forward_wavefield_" + k]
+        for r in range(len(args["rcv_z"])):
+            want = float(np.sum(out[k][:, :, r] ** 2))
+            assert abs(e[args["rcv_z"][r], args["rcv_x"][r]] - want) <= 1e-12 * max(want, 1e-300)
+
+
+def test_oracle_pml_matches_reference():
+    z = np.load("tests/golden/pml.npz")
+    for i in range(4):
+        nx, nz, dx, n, vmax, fs = z[f"args{i}"]
+        got = oracle.bc_pml(int(nx), int(nz), dx, dx, int(n), vmax, bool(fs))
+        np.testing.assert_array_equal(got, z[f"pml{i}"])
+
+
+def test_oracle_pml_closed_form():
+    d = oracle.bc_pml(30, 20, 5.0, 5.0, 8, 3000.0, False)
+    kappa = -3.0 * 3000.0 * np.log(1e-6) / (2.0 * 7 * 5.0)
+    assert d.shape == (36, 46)
+    assert np.all(d[8:28, 8:38] == 0)                       # interior undamped
+    assert np.isclose(d[18, 0], kappa) and np.isclose(d[18, 45], kappa)  # outermost = kappa
+    np.testing.assert_allclose(d[18, :8][::-1], kappa * (np.arange(8) / 7.0) ** 2)
+    np.testing.assert_array_equal(d, d[::-1, :])             # top/bottom symmetric
+    np.testing.assert_array_equal(d, d[:, ::-1])             # left/right symmetric
+
+
+def test_oracle_pad_model_and_ricker():
+    v = np.arange(12, dtype=np.float32).reshape(3, 4)
+    p = oracle.pad_model(v, 2)
+    assert p.shape == (7, 8)
+    assert p[0, 0] == v[0, 0] and p[-1, -1] == v[-1, -1] and p[3, 0] == v[1, 0]
+    np.testing.assert_array_equal(p[2:5, 2:6], v)
+    w = oracle.ricker(10.0, 1e-3)
+    assert len(w) == 221 and w[110] == 1.0
+    np.testing.assert_allclose(w, w[::-1])
