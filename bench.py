@@ -48,6 +48,88 @@ def workload(n_shots=64, nz=350, nx=1700, nt=4000, nabc=20, dx=5.0, dt=5e-4, f0=
         rcv_x=np.arange(nx, dtype=np.int64), rcv_z=np.full(nx, 2, np.int64))
 
 
+def eager_forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
+                  damp, vp, rho, checkpoint_segments=1):
+    """Plain PyTorch restatement of the reference operator (one ATen launch per
+    slice operation, autograd through reentrant torch.utils.checkpoint chunks),
+    as geobrain/physics/wave/kernels/acoustic_2d.py:123-501 runs it.  It is the
+    comparison baseline timed beside the HIP path ("plain per-shot GPU path");
+    the product never calls it."""
+    import torch
+    from torch.utils.checkpoint import checkpoint
+    dev = vp.device
+    NZ, NX = nz + 2 * nabc, nx + 2 * nabc
+    f = nabc if free_surface else 1
+    ii = torch.arange(NZ, device=dev).sub(nabc).clamp(0, nz - 1)
+    jj = torch.arange(NX, device=dev).sub(nabc).clamp(0, nx - 1)
+    c = vp[ii][:, jj]
+    den = rho[ii][:, jj]
+    a1 = den * c * c * dt / dz
+    k1 = damp * dt
+    a2 = dt / (den * dz)
+    k2 = torch.zeros_like(damp)
+    k2[:, 1:NX - 2] = 0.5 * (damp[:, 1:NX - 2] + damp[:, 2:NX - 1]) * dt
+    k3 = torch.zeros_like(damp)
+    k3[f:NZ - 2, :] = 0.5 * (damp[f:NZ - 2, :] + damp[f + 1:NZ - 1, :]) * dt
+    S = src_z.shape[0]
+    sz, sx = src_z.to(dev) + nabc, src_x.to(dev) + nabc
+    rz, rx = rcv_z.to(dev) + nabc, rcv_x.to(dev) + nabc
+    shots = torch.arange(S, device=dev)
+    c1, c2 = 9.0 / 8.0, -1.0 / 24.0
+    P = (slice(None), slice(f + 1, NZ - 2), slice(2, NX - 2))
+    X = (slice(None), slice(f, NZ - 1), slice(1, NX - 2))
+    Z = (slice(None), slice(f, NZ - 2), slice(1, NX - 1))
+    inner = (slice(nabc, nabc + nz), slice(nabc, nabc + nx))
+
+    def chunk_fn(a1, a2, k1, k2, k3, w, p, vx, vz):
+        p, vx, vz = p.clone(), vx.clone(), vz.clone()
+        n = w.shape[-1]
+        rec = [torch.zeros((S, n, rz.shape[0]), device=dev) for _ in range(3)]
+        en = [torch.zeros((nz, nx), device=dev) for _ in range(3)]
+        for it in range(n):
+            p[P] = (1.0 - k1[P[1:]]) * p[P] - a1[P[1:]] * (
+                c1 * (vx[:, f + 1:NZ - 2, 2:NX - 2] - vx[:, f + 1:NZ - 2, 1:NX - 3]
+                      + vz[:, f + 1:NZ - 2, 2:NX - 2] - vz[:, f:NZ - 3, 2:NX - 2])
+                + c2 * (vx[:, f + 1:NZ - 2, 3:NX - 1] - vx[:, f + 1:NZ - 2, 0:NX - 4]
+                        + vz[:, f + 2:NZ - 1, 2:NX - 2] - vz[:, f - 1:NZ - 4, 2:NX - 2]))
+            upd = dt * (w[it] if w.dim() == 1 else w[:, it])
+            p[shots, sz, sx] = p[shots, sz, sx] + upd
+            if free_surface:
+                p[:, f - 1, :] = -p[:, f + 1, :]
+            vx[X] = (1.0 - k2[X[1:]]) * vx[X] - a2[X[1:]] * (
+                c1 * (p[:, f:NZ - 1, 2:NX - 1] - p[:, f:NZ - 1, 1:NX - 2])
+                + c2 * (p[:, f:NZ - 1, 3:NX] - p[:, f:NZ - 1, 0:NX - 3]))
+            vz[Z] = (1.0 - k3[Z[1:]]) * vz[Z] - a2[Z[1:]] * (
+                c1 * (p[:, f + 1:NZ - 1, 1:NX - 1] - p[:, f:NZ - 2, 1:NX - 1])
+                + c2 * (p[:, f + 2:NZ, 1:NX - 1] - p[:, f - 1:NZ - 3, 1:NX - 1]))
+            if free_surface:
+                vz[:, f - 1, :] = vz[:, f, :]
+            for r, fld in zip(rec, (p, vx, vz)):
+                r[:, it, :] = fld[:, rz, rx]
+            for e, fld in zip(en, (p, vx, vz)):
+                e += torch.sum(fld * fld, dim=0)[inner].detach()
+        return (p, vx, vz, *rec, *en)
+
+    p = torch.zeros((S, NZ, NX), device=dev)
+    vx = torch.zeros((S, NZ, NX - 1), device=dev)
+    vz = torch.zeros((S, NZ - 1, NX), device=dev)
+    recs, ens = [[], [], []], [torch.zeros((nz, nx), device=dev) for _ in range(3)]
+    for w in torch.chunk(src_v, checkpoint_segments, dim=-1):
+        if checkpoint_segments > 1 and torch.is_grad_enabled():
+            res = checkpoint(chunk_fn, a1, a2, k1, k2, k3, w, p, vx, vz, use_reentrant=True)
+        else:
+            res = chunk_fn(a1, a2, k1, k2, k3, w, p, vx, vz)
+        p, vx, vz = res[:3]
+        for lst, r in zip(recs, res[3:6]):
+            lst.append(r)
+        for e, r in zip(ens, res[6:9]):
+            e += r.detach()
+    out = {k: torch.cat(v, dim=1) for k, v in zip(("p", "vx", "vz"), recs)}
+    for k, e in zip(("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"), ens):
+        out[k] = e
+    return out
+
+
 def cpu_baseline(w, steps=400, shots=1):
     """FP32 oracle gradient on `shots` shot(s) x `steps` time steps of the same grid."""
     import oracle

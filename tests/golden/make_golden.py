@@ -198,6 +198,60 @@ def c2_case(nz, nx, nt, name):
                                               "synthetic.smooth_model(vp, 0.05)")))
 
 
+def read_segy_ibm(raw: bytes):
+    """Minimal SEG-Y reader: 3600-byte file header, then traces of a 240-byte
+    header + n_samples big-endian 4-byte IBM floats (format 1).  Returns
+    (n_traces, n_samples) float32.  IBM single: sign, 7-bit base-16 exponent
+    biased by 64, 24-bit fraction: value = (-1)^s * 0.f * 16^(e - 64)."""
+    n_samples = int.from_bytes(raw[3220:3222], "big")
+    fmt = int.from_bytes(raw[3224:3226], "big")
+    assert fmt == 1, fmt
+    trace_bytes = 240 + 4 * n_samples
+    n_traces = (len(raw) - 3600) // trace_bytes
+    buf = np.frombuffer(raw, dtype=np.uint8, offset=3600,
+                        count=n_traces * trace_bytes).reshape(n_traces, trace_bytes)
+    words = buf[:, 240:].copy().view(">u4").astype(np.uint64)
+    sign = np.where((words >> 31) & 1, -1.0, 1.0)
+    expo = ((words >> 24) & 0x7F).astype(np.int64) - 64
+    frac = (words & 0xFFFFFF).astype(np.float64) / float(1 << 24)
+    return (sign * frac * np.power(16.0, expo)).astype(np.float32)
+
+
+def marmousi_case():
+    """Marmousi2 true model of examples/09_acoustic_fwi.py:75-146 (88 x 200 at 40 m,
+    x from 5000 m), resampled with the same steps as that example."""
+    import zipfile
+    from scipy import interpolate
+    with zipfile.ZipFile(os.path.join(REF, "examples", "data", "marmousi.zip")) as zf:
+        raw = zf.read("marmousi/vp_marmousi-ii.segy")
+    vp_full = read_segy_ibm(raw).T  # (nz, nx) = (samples, traces)
+    if vp_full.max() < 10:
+        vp_full = vp_full * 1000.0
+    dxo = 1.25
+    nz_o, nx_o = vp_full.shape
+    x_o, z_o = np.arange(nx_o) * dxo, np.arange(nz_o) * dxo
+    NX, NZ, DX, X0 = 200, 88, 40.0, 5000
+    xt = np.linspace(X0, X0 + NX * DX - DX, NX)
+    zt = np.linspace(0, NZ * DX - DX, NZ)
+    xt = np.clip(xt, x_o.min(), x_o.max() - DX)
+    zt = np.clip(zt, z_o.min(), z_o.max() - DX)
+    f = interpolate.RegularGridInterpolator((z_o, x_o), vp_full, method="linear",
+                                            bounds_error=False, fill_value=None)
+    Zt, Xt = np.meshgrid(zt, xt, indexing="ij")
+    vp = f(np.stack([Zt.ravel(), Xt.ravel()], axis=-1)).reshape(NZ, NX)
+    meta = dict(name="marmousi2_fwi", source="examples/data/marmousi.zip vp_marmousi-ii.segy",
+                nz=NZ, nx=NX, dx=DX, x_start=X0, full_shape=[int(nz_o), int(nx_o)],
+                notebook_loss={"0": 7.480488e+04, "10": 5.605745e+04, "100": 1.241548e+04,
+                               "299": 2.053805e+03},
+                notebook_loss_source="examples/notebooks/09_acoustic_fwi.ipynb:420-450",
+                gardner_range=[1929, 2567],
+                gardner_range_source="examples/notebooks/07_acoustic_forward.ipynb:254")
+    np.savez_compressed(os.path.join(HERE, "marmousi2_fwi.npz"), vp_true=vp,
+                        meta=np.array(json.dumps(meta)))
+    print(f"marmousi2_fwi: vp {vp.min():.1f}..{vp.max():.1f}, "
+          f"Gardner {310 * vp.min() ** 0.25:.0f}..{310 * vp.max() ** 0.25:.0f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*")
@@ -213,6 +267,8 @@ def main():
         if args.only and name not in args.only:
             continue
         run_case(name, **kw)
+    if not args.only or "marmousi" in args.only:
+        marmousi_case()
     if not args.only or "pml" in args.only:
         out = {}
         for i, (nx, nz, dx, n, vmax, fs) in enumerate([(200, 100, 10.0, 20, 3500.0, False),
