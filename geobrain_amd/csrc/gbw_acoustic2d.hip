@@ -147,7 +147,7 @@ __global__ void rmap_scatter_kernel(int* __restrict__ rhead, const int* __restri
 // ---------------------------------------------------------------------------
 struct FwdArgs {
   Dims d;
-  int S, spb, tiles_x;           // shots, shots per block, tiles along x
+  int S, spb, tiles_x, ntiles;   // shots, shots per workgroup, tiles along x, tiles
   const float* P0; const float* VX0; const float* VZ0;  // input state fields (shot-major)
   float* P1; float* VX1; float* VZ1;                    // output state fields
   const float* coef;
@@ -161,160 +161,318 @@ struct FwdArgs {
   float* gxbuf; float* gzbuf;    // [S][plane] vx / vz brackets of this step, or null
 };
 
+// ---------------------------------------------------------------------------
+// Staging geometry shared by the step kernels
+//
+// A workgroup (256 threads = four 64-lane wavefronts) owns a tile of TZ rows
+// x 64 columns.  Every staged row covers columns x0-4 .. x0+67 as 18 float4
+// groups, loaded with 16-byte global loads; in LDS a row holds 20 float4
+// (columns x0-8 .. x0+71) so that the 4-wide stencil reads of the outermost
+// groups stay inside the row (guard groups 0 and 19 are zero).  Work items are
+// (row, float4 group) pairs: 4 cells along x per thread and item.
+// ---------------------------------------------------------------------------
+constexpr int NG = 18;  // float4 groups staged per row
+constexpr int LG = 20;  // float4 groups per LDS row
+
+__device__ __forceinline__ float4 f4zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ __forceinline__ float f4get(const float4& v, int e) {
+  return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void f4set(float4& v, int e, float s) {
+  if (e == 0) v.x = s; else if (e == 1) v.y = s; else if (e == 2) v.z = s; else v.w = s;
+}
+
+// float4 index of row gz, first column x (a multiple of 4), clamped into the
+// plane; ok tells whether the group is real (rows outside the grid and the
+// columns left of 0 read as zero; columns NX..PX-1 are zero padding in memory)
+__device__ __forceinline__ unsigned group_index(const Dims& d, int gz, int x, bool& ok) {
+  ok = gz >= 0 && gz < d.NZ && x >= 0 && x <= d.PX - 4;
+  const int zc = min(max(gz, 0), d.NZ - 1);
+  const int xc = min(max(x, 0), d.PX - 4);
+  return (unsigned)((zc * d.PX + xc) >> 2);
+}
+
+// tile index of this workgroup: consecutive workgroups go to different XCDs
+// (round-robin over 8), so hand each XCD a contiguous band of tiles and
+// neighbouring tiles share that XCD's L2
+__device__ __forceinline__ int xcd_tile() {
+  const int b = blockIdx.x;
+  return (b & 7) * (gridDim.x >> 3) + (b >> 3);
+}
+
 // RECOMP = false: forward pass (receivers, energy); true: checkpoint-interval
 // recompute for the adjoint (stores the brackets the adjoint needs instead).
 template <int TZ, bool RECOMP>
 __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
-  __shared__ float sVX[TZ + 3][TX + 6];  // rows z0-1..z0+TZ+1, cols x0-3..x0+TX+2
-  __shared__ float sVZ[TZ + 6][TX + 3];  // rows z0-3..z0+TZ+2, cols x0-1..x0+TX+1
-  __shared__ float sP[TZ + 3][TX + 3];   // rows z0-1..z0+TZ+1, cols x0-1..x0+TX+1
-  constexpr int RPT = TZ / 4;            // tile rows per wavefront
+  constexpr int RV = TZ + 3;  // vx rows z0-1 .. z0+TZ+1
+  constexpr int RZ = TZ + 6;  // vz rows z0-3 .. z0+TZ+2
+  constexpr int RP = TZ + 3;  // p' rows z0-1 .. z0+TZ+1
+  constexpr int NV = (RV * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NZZ = (RZ * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NP = (RP * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NT = TZ * 16 / NTHREADS;  // tile items (TZ rows x 16 groups)
+  static_assert(TZ % 16 == 0, "TZ must be a multiple of 16");
+  __shared__ float4 sVX[RV][LG];
+  __shared__ float4 sVZ[RZ][LG];
+  __shared__ float4 sP[RP][LG];
   const Dims d = a.d;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tile = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile();
+  if (tile >= a.ntiles) return;
   const int x0 = (tile % a.tiles_x) * TX;
   const int z0 = (tile / a.tiles_x) * TZ;
   const int s_begin = blockIdx.y * a.spb;
   const int s_end = min(a.S, s_begin + a.spb);
-  const float* omk1 = a.coef;
-  const float* a1 = a.coef + d.plane;
-  const float* omk2 = a.coef + 2 * d.plane;
-  const float* omk3 = a.coef + 3 * d.plane;
-  const float* a2 = a.coef + 4 * d.plane;
-  const long long fstride = (long long)a.S * d.plane;  // between P, VX, VZ of one state
+  if (s_begin >= s_end) return;  // uniform per workgroup
+  const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
+  const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
+  const float4* __restrict__ omk3 = reinterpret_cast<const float4*>(a.coef + 3 * d.plane);
+  const float4* __restrict__ a2 = reinterpret_cast<const float4*>(a.coef + 4 * d.plane);
 
-  float e_p[RPT], e_vx[RPT], e_vz[RPT];
+  for (int i = tid; i < RV; i += NTHREADS) sVX[i][0] = sVX[i][LG - 1] = f4zero();
+  for (int i = tid; i < RZ; i += NTHREADS) sVZ[i][0] = sVZ[i][LG - 1] = f4zero();
+
+  // --- per-workgroup constants (registers across the shot loop) ---
+  unsigned vx_idx[NV], vz_idx[NZZ], p_idx[NP];
+  unsigned vx_ok = 0, vz_ok = 0, p_ok = 0;
 #pragma unroll
-  for (int k = 0; k < RPT; ++k) e_p[k] = e_vx[k] = e_vz[k] = 0.f;
+  for (int n = 0; n < NV; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    bool ok;
+    vx_idx[n] = group_index(d, z0 - 1 + r, x0 - 4 + 4 * g, ok);
+    if (ok && it < RV * NG) vx_ok |= 1u << n;
+  }
+#pragma unroll
+  for (int n = 0; n < NZZ; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    bool ok;
+    vz_idx[n] = group_index(d, z0 - 3 + r, x0 - 4 + 4 * g, ok);
+    if (ok && it < RZ * NG) vz_ok |= 1u << n;
+  }
+  // p' items: region mask per cell (4 bits per item), coefficients
+  float4 pc_omk[NP], pc_a1[NP];
+  unsigned p_reg = 0, p_in = 0;
+#pragma unroll
+  for (int n = 0; n < NP; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    const int gz = z0 - 1 + r, xb = x0 - 4 + 4 * g;
+    bool ok;
+    p_idx[n] = group_index(d, gz, xb, ok);
+    ok = ok && it < RP * NG;
+    if (ok) p_ok |= 1u << n;
+    const bool zr = gz >= d.f + 1 && gz <= d.NZ - 3;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      if (ok && x < d.NX) p_in |= 1u << (4 * n + e);
+      if (ok && zr && x >= 2 && x <= d.NX - 3) p_reg |= 1u << (4 * n + e);
+    }
+    pc_omk[n] = omk1[p_idx[n]];
+    pc_a1[n] = a1[p_idx[n]];
+  }
+  // tile items
+  unsigned t_idx[NT];
+  float4 tc_omk2[NT], tc_omk3[NT], tc_a2[NT];
+  unsigned t_ok = 0, t_x = 0, t_z = 0, t_e = 0;
+  int4 t_rh[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+    const int gz = z0 + tr, xb = x0 + 4 * h;
+    bool ok;
+    t_idx[n] = group_index(d, gz, xb, ok);
+    if (ok) t_ok |= 1u << n;
+    int4 rh = make_int4(-1, -1, -1, -1);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      const bool in = ok && x < d.NX;
+      if (in && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) t_x |= 1u << (4 * n + e);
+      if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) t_z |= 1u << (4 * n + e);
+      if (in && gz >= d.nabc && gz < d.nabc + d.nz && x >= d.nabc && x < d.nabc + d.nx)
+        t_e |= 1u << (4 * n + e);
+    }
+    if (!RECOMP && a.rec_p && ok && a.rrow[gz]) rh = reinterpret_cast<const int4*>(a.rhead)[t_idx[n]];
+    t_rh[n] = rh;
+    tc_omk2[n] = omk2[t_idx[n]];
+    tc_omk3[n] = omk3[t_idx[n]];
+    tc_a2[n] = a2[t_idx[n]];
+  }
+
+  float4 e_p[NT], e_vx[NT], e_vz[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) e_p[n] = e_vx[n] = e_vz[n] = f4zero();
+
+  float4 rvx[NV], rvz[NZZ], rp[NP];
+  auto load_v = [&](int s) {
+    const long long so = (long long)s * d.plane;
+    const float4* __restrict__ VX = reinterpret_cast<const float4*>(a.VX0 + so);
+    const float4* __restrict__ VZ = reinterpret_cast<const float4*>(a.VZ0 + so);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) rvx[n] = VX[vx_idx[n]];
+#pragma unroll
+    for (int n = 0; n < NZZ; ++n) rvz[n] = VZ[vz_idx[n]];
+  };
+  auto load_p = [&](int s) {
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(a.P0 + (long long)s * d.plane);
+#pragma unroll
+    for (int n = 0; n < NP; ++n) rp[n] = P[p_idx[n]];
+  };
+  load_v(s_begin);
+  load_p(s_begin);
 
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const float* P0 = a.P0 + so;
-    const float* VX0 = a.VX0 + so;
-    const float* VZ0 = a.VZ0 + so;
-    for (int r = wave; r < TZ + 3; r += 4) {
-      const int gz = z0 - 1 + r;
-      const bool zin = gz >= 0 && gz < d.NZ;
-      for (int c = lane; c < TX + 6; c += 64) {
-        const int gx = x0 - 3 + c;
-        sVX[r][c] = (zin && gx >= 0 && gx < d.NX) ? VX0[(long long)gz * d.PX + gx] : 0.f;
-      }
-    }
-    for (int r = wave; r < TZ + 6; r += 4) {
-      const int gz = z0 - 3 + r;
-      const bool zin = gz >= 0 && gz < d.NZ;
-      for (int c = lane; c < TX + 3; c += 64) {
-        const int gx = x0 - 1 + c;
-        sVZ[r][c] = (zin && gx >= 0 && gx < d.NX) ? VZ0[(long long)gz * d.PX + gx] : 0.f;
-      }
-    }
-    __syncthreads();
     const int scell = a.src_cell[s];
     const float supd = a.dt * a.wav[(long long)s * a.wav_stride + a.it];
-    // p' on the tile and its halo
-    for (int r = wave; r < TZ + 3; r += 4) {
-      const int gz = z0 - 1 + r;
-      const bool zin = gz >= 0 && gz < d.NZ;
-      const bool zreg = gz >= d.f + 1 && gz <= d.NZ - 3;
-      for (int c = lane; c < TX + 3; c += 64) {
-        const int gx = x0 - 1 + c;
-        float pv = 0.f;
-        if (zin && gx >= 0 && gx < d.NX) {
-          const long long q = (long long)gz * d.PX + gx;
-          pv = P0[q];
-          if (zreg && gx >= 2 && gx <= d.NX - 3) {
-            float s1 = sVX[r][c + 2] - sVX[r][c + 1];
-            s1 = s1 + sVZ[r + 2][c];
-            s1 = s1 - sVZ[r + 1][c];
-            float s2 = sVX[r][c + 3] - sVX[r][c];
-            s2 = s2 + sVZ[r + 3][c];
-            s2 = s2 - sVZ[r][c];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RV * NG) sVX[it / NG][it % NG + 1] = ((vx_ok >> n) & 1u) ? rvx[n] : f4zero();
+    }
+#pragma unroll
+    for (int n = 0; n < NZZ; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RZ * NG) sVZ[it / NG][it % NG + 1] = ((vz_ok >> n) & 1u) ? rvz[n] : f4zero();
+    }
+    if (s + 1 < s_end) load_v(s + 1);  // next shot's vx/vz loads overlap this shot
+    __syncthreads();
+    // p' on the tile and its halo (acoustic_2d.py:213-237)
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RP * NG) {
+        const int r = it / NG, g = it % NG;
+        const float4 xa = sVX[r][g], xb = sVX[r][g + 1], xc = sVX[r][g + 2];
+        const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
+        const float4 zm2 = sVZ[r][g + 1], zm1 = sVZ[r + 1][g + 1];
+        const float4 z00 = sVZ[r + 2][g + 1], zp1 = sVZ[r + 3][g + 1];
+        const float4 p0 = ((p_ok >> n) & 1u) ? rp[n] : f4zero();
+        float4 out = p0, dv = f4zero();
+        const unsigned base = p_idx[n] * 4u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const unsigned bit = 1u << (4 * n + e);
+          float pv = f4get(p0, e);
+          float br = 0.f;
+          if (p_reg & bit) {
+            float s1 = vxs[4 + e] - vxs[3 + e];
+            s1 = s1 + f4get(z00, e);
+            s1 = s1 - f4get(zm1, e);
+            float s2 = vxs[5 + e] - vxs[2 + e];
+            s2 = s2 + f4get(zp1, e);
+            s2 = s2 - f4get(zm2, e);
             const float t1 = kC1 * s1;
             const float t2 = kC2 * s2;
-            const float br = t1 + t2;
-            const float u = omk1[q] * pv;
-            pv = u - a1[q] * br;
-            if constexpr (RECOMP) {
-              if (r >= 1 && r <= TZ && c >= 1 && c <= TX) a.dbuf[so + q] = br;
-            }
+            br = t1 + t2;
+            const float u = f4get(pc_omk[n], e) * pv;
+            pv = u - f4get(pc_a1[n], e) * br;
           }
-          if ((int)q == scell) pv = pv + supd;
+          if ((p_in & bit) && (int)(base + e) == scell) pv = pv + supd;
+          f4set(out, e, pv);
+          f4set(dv, e, br);
         }
-        sP[r][c] = pv;
+        sP[r][g + 1] = out;
+        if constexpr (RECOMP) {
+          // tile groups only: rows 1..TZ of the rectangle, float4 groups 1..16
+          if (r >= 1 && r <= TZ && g >= 1 && g <= 16 && ((p_ok >> n) & 1u))
+            reinterpret_cast<float4*>(a.dbuf + so)[p_idx[n]] = dv;
+        }
       }
     }
+    if (s + 1 < s_end) load_p(s + 1);
     __syncthreads();
     if (d.fs) {
-      const int rm = d.f - 1 - (z0 - 1);  // rect row of f-1; f+1 is rect row rm+2
+      const int rm = d.f - z0;  // rect row of f-1; f+1 is rect row rm+2
       if (rm >= 0 && rm <= TZ) {
-        for (int c = tid; c < TX + 3; c += NTHREADS) sP[rm][c] = -sP[rm + 2][c];
+        for (int g = tid; g < NG; g += NTHREADS) {
+          const float4 v = sP[rm + 2][g + 1];
+          sP[rm][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+        }
       }
       __syncthreads();
     }
-    // v' on the tile
+    // v' on the tile (acoustic_2d.py:243-274)
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const int tr = wave + 4 * k;
-      const int gz = z0 + tr, gx = x0 + lane;
-      if (gz < d.NZ && gx < d.NX) {
-        const long long q = (long long)gz * d.PX + gx;
-        const float pc = sP[tr + 1][lane + 1];
-        float vx = sVX[tr + 1][lane + 3];
-        float vz = sVZ[tr + 3][lane + 1];
-        if (gz >= d.f && gz <= d.NZ - 2 && gx >= 1 && gx <= d.NX - 3) {
-          const float d1 = sP[tr + 1][lane + 2] - pc;
-          const float d2 = sP[tr + 1][lane + 3] - sP[tr + 1][lane];
-          const float br = kC1 * d1 + kC2 * d2;
-          const float u = omk2[q] * vx;
-          vx = u - a2[q] * br;
-          if constexpr (RECOMP) {
-            if (a.gxbuf) a.gxbuf[so + q] = br;
+    for (int n = 0; n < NT; ++n) {
+      if ((t_ok >> n) & 1u) {
+        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+        const int gz = z0 + tr;
+        const float4 pa = sP[tr + 1][h + 1], pb = sP[tr + 1][h + 2], pcc = sP[tr + 1][h + 3];
+        const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pcc.x, pcc.y, pcc.z, pcc.w};
+        const float4 pu = sP[tr][h + 2], pd1 = sP[tr + 2][h + 2], pd2 = sP[tr + 3][h + 2];
+        float4 vx = sVX[tr + 1][h + 2];
+        float4 vz = sVZ[tr + 3][h + 2];
+        float4 gxv = f4zero(), gzv = f4zero();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const unsigned bit = 1u << (4 * n + e);
+          if (t_x & bit) {
+            const float d1 = ps[5 + e] - ps[4 + e];
+            const float d2 = ps[6 + e] - ps[3 + e];
+            const float br = kC1 * d1 + kC2 * d2;
+            const float u = f4get(tc_omk2[n], e) * f4get(vx, e);
+            f4set(vx, e, u - f4get(tc_a2[n], e) * br);
+            f4set(gxv, e, br);
+          }
+          if (t_z & bit) {
+            const float d1 = f4get(pd1, e) - f4get(pb, e);
+            const float d2 = f4get(pd2, e) - f4get(pu, e);
+            const float br = kC1 * d1 + kC2 * d2;
+            const float u = f4get(tc_omk3[n], e) * f4get(vz, e);
+            f4set(vz, e, u - f4get(tc_a2[n], e) * br);
+            f4set(gzv, e, br);
           }
         }
-        if (gz >= d.f && gz <= d.NZ - 3 && gx >= 1 && gx <= d.NX - 2) {
-          const float d1 = sP[tr + 2][lane + 1] - pc;
-          const float d2 = sP[tr + 3][lane + 1] - sP[tr][lane + 1];
-          const float br = kC1 * d1 + kC2 * d2;
-          const float u = omk3[q] * vz;
-          vz = u - a2[q] * br;
-          if constexpr (RECOMP) {
-            if (a.gzbuf) a.gzbuf[so + q] = br;
+        const unsigned ti = t_idx[n];
+        reinterpret_cast<float4*>(a.P1 + so)[ti] = pb;
+        reinterpret_cast<float4*>(a.VX1 + so)[ti] = vx;
+        if (!(d.fs && gz == d.f - 1)) reinterpret_cast<float4*>(a.VZ1 + so)[ti] = vz;
+        if (d.fs && gz == d.f) reinterpret_cast<float4*>(a.VZ1 + so)[ti - (d.PX >> 2)] = vz;
+        if constexpr (RECOMP) {
+          if (a.gxbuf) {
+            reinterpret_cast<float4*>(a.gxbuf + so)[ti] = gxv;
+            reinterpret_cast<float4*>(a.gzbuf + so)[ti] = gzv;
           }
-        }
-        a.P1[so + q] = pc;
-        a.VX1[so + q] = vx;
-        if (!(d.fs && gz == d.f - 1)) a.VZ1[so + q] = vz;
-        if (d.fs && gz == d.f) a.VZ1[so + q - d.PX] = vz;  // vz[f-1] = vz[f]
-        if constexpr (!RECOMP) {
-          if (a.rec_p && a.rrow[gz]) {
-            for (int h = a.rhead[q]; h >= 0; h = a.rnext[h]) {
-              const long long o = ((long long)s * a.nt + a.it) * a.R + h;
-              a.rec_p[o] = pc;
-              a.rec_vx[o] = vx;
-              a.rec_vz[o] = vz;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            for (int hh = e == 0 ? t_rh[n].x : (e == 1 ? t_rh[n].y : (e == 2 ? t_rh[n].z : t_rh[n].w));
+                 hh >= 0; hh = a.rnext[hh]) {
+              const long long o = ((long long)s * a.nt + a.it) * a.R + hh;
+              a.rec_p[o] = f4get(pb, e);
+              a.rec_vx[o] = f4get(vx, e);
+              a.rec_vz[o] = f4get(vz, e);
             }
           }
-          e_p[k] = e_p[k] + pc * pc;
-          e_vx[k] = e_vx[k] + vx * vx;
-          e_vz[k] = e_vz[k] + vz * vz;
+          e_p[n] = make_float4(e_p[n].x + pb.x * pb.x, e_p[n].y + pb.y * pb.y,
+                               e_p[n].z + pb.z * pb.z, e_p[n].w + pb.w * pb.w);
+          e_vx[n] = make_float4(e_vx[n].x + vx.x * vx.x, e_vx[n].y + vx.y * vx.y,
+                                e_vx[n].z + vx.z * vx.z, e_vx[n].w + vx.w * vx.w);
+          e_vz[n] = make_float4(e_vz[n].x + vz.x * vz.x, e_vz[n].y + vz.y * vz.y,
+                                e_vz[n].z + vz.z * vz.z, e_vz[n].w + vz.w * vz.w);
         }
       }
     }
     __syncthreads();
   }
-  (void)fstride;
-  if (!RECOMP && a.energy && s_begin < s_end) {
-    const long long inner = (long long)d.nz * d.nx;
-    float* eg = a.energy + (long long)blockIdx.y * 3 * inner;
+  if constexpr (!RECOMP) {
+    if (a.energy) {
+      const long long inner = (long long)d.nz * d.nx;
+      float* eg = a.energy + (long long)blockIdx.y * 3 * inner;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const int ia = z0 + wave + 4 * k - d.nabc;
-      const int jb = x0 + lane - d.nabc;
-      if (ia >= 0 && ia < d.nz && jb >= 0 && jb < d.nx) {
-        const long long o = (long long)ia * d.nx + jb;
-        eg[o] = eg[o] + e_p[k];
-        eg[inner + o] = eg[inner + o] + e_vx[k];
-        eg[2 * inner + o] = eg[2 * inner + o] + e_vz[k];
+      for (int n = 0; n < NT; ++n) {
+        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if ((t_e >> (4 * n + e)) & 1u) {
+            const long long o = (long long)(z0 + tr - d.nabc) * d.nx + (x0 + 4 * h + e - d.nabc);
+            eg[o] = eg[o] + f4get(e_p[n], e);
+            eg[inner + o] = eg[inner + o] + f4get(e_vx[n], e);
+            eg[2 * inner + o] = eg[2 * inner + o] + f4get(e_vz[n], e);
+          }
+        }
       }
     }
   }
@@ -351,7 +509,7 @@ __global__ void snapshot_kernel(Dims d, const float* __restrict__ P, float* __re
 // ---------------------------------------------------------------------------
 struct AdjArgs {
   Dims d;
-  int S, spb, tiles_x;
+  int S, spb, tiles_x, ntiles;
   const float* LP0; const float* LX0; const float* LZ0;  // adjoint of state it+1 (records of it injected)
   float* LP1; float* LX1; float* LZ1;                    // adjoint of state it (records of it-1 injected)
   const float* coef;
@@ -368,155 +526,333 @@ struct AdjArgs {
 
 template <int TZ>
 __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
-  __shared__ float sQX[TZ + 3][TX + 6];  // a2*lambda_vx on the vx region, rows z0-1.., cols x0-3..
-  __shared__ float sQZ[TZ + 6][TX + 3];  // a2*lambda_vz on the vz region, rows z0-3.., cols x0-1..
-  __shared__ float sLP[TZ + 3][TX + 3];  // lambda_p' rows z0-1.., cols x0-1..
-  __shared__ float sR[TZ + 3][TX + 3];   // a1*lambda_p' on the p region
-  constexpr int RPT = TZ / 4;
+  constexpr int RV = TZ + 3;  // lambda_vx rows z0-1 ..
+  constexpr int RZ = TZ + 6;  // lambda_vz rows z0-3 ..
+  constexpr int RP = TZ + 3;  // lambda_p' rows z0-1 ..
+  constexpr int NV = (RV * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NZZ = (RZ * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NP = (RP * NG + NTHREADS - 1) / NTHREADS;
+  constexpr int NT = TZ * 16 / NTHREADS;
+  static_assert(TZ % 16 == 0, "TZ must be a multiple of 16");
+  __shared__ float4 sLX[RV][LG];  // lambda_vx as read
+  __shared__ float4 sQX[RV][LG];  // a2 * lambda_vx on the vx region, else 0
+  __shared__ float4 sLZ[RZ][LG];  // lambda_vz, the vz[f-1] = vz[f] transpose folded in
+  __shared__ float4 sQZ[RZ][LG];  // a2 * lambda_vz on the vz region, else 0
+  __shared__ float4 sLP[RP][LG];  // lambda_p'
+  __shared__ float4 sR[RP][LG];   // a1 * lambda_p' on the p region, else 0
   const Dims d = a.d;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tile = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile();
+  if (tile >= a.ntiles) return;
   const int x0 = (tile % a.tiles_x) * TX;
   const int z0 = (tile / a.tiles_x) * TZ;
   const int s_begin = blockIdx.y * a.spb;
   const int s_end = min(a.S, s_begin + a.spb);
-  const float* omk1 = a.coef;
-  const float* a1 = a.coef + d.plane;
-  const float* omk2 = a.coef + 2 * d.plane;
-  const float* omk3 = a.coef + 3 * d.plane;
-  const float* a2 = a.coef + 4 * d.plane;
+  if (s_begin >= s_end) return;
+  const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
+  const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
+  const float4* __restrict__ omk3 = reinterpret_cast<const float4*>(a.coef + 3 * d.plane);
+  const float4* __restrict__ a2 = reinterpret_cast<const float4*>(a.coef + 4 * d.plane);
 
-  float gacc[RPT], gacc2[RPT];
+  for (int i = tid; i < RV; i += NTHREADS) sQX[i][0] = sQX[i][LG - 1] = f4zero();
+
+  // --- per-workgroup constants ---
+  unsigned x_idx[NV], z_idx[NZZ], p_idx[NP];
+  unsigned x_ok = 0, z_ok = 0, p_ok = 0, x_reg = 0, z_reg = 0, p_reg = 0;
+  float4 xa2[NV], za2[NZZ], pa1[NP];
 #pragma unroll
-  for (int k = 0; k < RPT; ++k) gacc[k] = gacc2[k] = 0.f;
+  for (int n = 0; n < NV; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    const int gz = z0 - 1 + r, xb = x0 - 4 + 4 * g;
+    bool ok;
+    x_idx[n] = group_index(d, gz, xb, ok);
+    ok = ok && it < RV * NG;
+    if (ok) x_ok |= 1u << n;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      if (ok && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) x_reg |= 1u << (4 * n + e);
+    }
+    xa2[n] = a2[x_idx[n]];
+  }
+#pragma unroll
+  for (int n = 0; n < NZZ; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    const int gz = z0 - 3 + r, xb = x0 - 4 + 4 * g;
+    bool ok;
+    z_idx[n] = group_index(d, gz, xb, ok);
+    ok = ok && it < RZ * NG;
+    if (ok) z_ok |= 1u << n;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      if (ok && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) z_reg |= 1u << (4 * n + e);
+    }
+    za2[n] = a2[z_idx[n]];
+  }
+#pragma unroll
+  for (int n = 0; n < NP; ++n) {
+    const int it = tid + n * NTHREADS, r = it / NG, g = it % NG;
+    const int gz = z0 - 1 + r, xb = x0 - 4 + 4 * g;
+    bool ok;
+    p_idx[n] = group_index(d, gz, xb, ok);
+    ok = ok && it < RP * NG;
+    if (ok) p_ok |= 1u << n;
+    const bool zr = gz >= d.f + 1 && gz <= d.NZ - 3;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      if (ok && zr && x >= 2 && x <= d.NX - 3) p_reg |= 1u << (4 * n + e);
+    }
+    pa1[n] = a1[p_idx[n]];
+  }
+  unsigned t_idx[NT];
+  float4 tc_omk1[NT], tc_omk2[NT], tc_omk3[NT];
+  unsigned t_ok = 0, t_p = 0, t_x = 0, t_z = 0, t_vx = 0;
+  int4 t_rh[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+    const int gz = z0 + tr, xb = x0 + 4 * h;
+    bool ok;
+    t_idx[n] = group_index(d, gz, xb, ok);
+    if (ok) t_ok |= 1u << n;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      const bool in = ok && x < d.NX;
+      const unsigned bit = 1u << (4 * n + e);
+      if (in && gz >= d.f + 1 && gz <= d.NZ - 3 && x >= 2 && x <= d.NX - 3) t_p |= bit;
+      if (in && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) t_x |= bit;
+      if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) t_z |= bit;
+      if (in && x <= d.NX - 2) t_vx |= bit;  // vx has NX-1 columns
+    }
+    int4 rh = make_int4(-1, -1, -1, -1);
+    if (ok && a.it >= 1 && a.rrow[gz]) rh = reinterpret_cast<const int4*>(a.rhead)[t_idx[n]];
+    t_rh[n] = rh;
+    tc_omk1[n] = omk1[t_idx[n]];
+    tc_omk2[n] = omk2[t_idx[n]];
+    tc_omk3[n] = omk3[t_idx[n]];
+  }
+  // the free-surface fold touches this tile's vz rectangle at rect row f - (z0-3)
+  const int rfold = d.f - (z0 - 3);
+  const bool fold = d.fs && rfold >= 0 && rfold < RZ;
+
+  float4 gacc[NT], gacc2[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) gacc[n] = gacc2[n] = f4zero();
+
+  float4 rlx[NV], rlz[NZZ], rlp[NP], rd[NT];
+  auto load_v = [&](int s) {
+    const long long so = (long long)s * d.plane;
+    const float4* __restrict__ LX = reinterpret_cast<const float4*>(a.LX0 + so);
+    const float4* __restrict__ LZ = reinterpret_cast<const float4*>(a.LZ0 + so);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) rlx[n] = LX[x_idx[n]];
+#pragma unroll
+    for (int n = 0; n < NZZ; ++n) rlz[n] = LZ[z_idx[n]];
+  };
+  auto load_p = [&](int s) {
+    const long long so = (long long)s * d.plane;
+    const float4* __restrict__ LP = reinterpret_cast<const float4*>(a.LP0 + so);
+    const float4* __restrict__ DB = reinterpret_cast<const float4*>(a.dbuf + so);
+#pragma unroll
+    for (int n = 0; n < NP; ++n) rlp[n] = LP[p_idx[n]];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) rd[n] = DB[t_idx[n]];
+  };
+  load_v(s_begin);
+  load_p(s_begin);
 
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const float* LP0 = a.LP0 + so;
-    const float* LX0 = a.LX0 + so;
-    const float* LZ0 = a.LZ0 + so;
-    for (int r = wave; r < TZ + 3; r += 4) {
-      const int gz = z0 - 1 + r;
-      const bool zreg = gz >= d.f && gz <= d.NZ - 2;
-      for (int c = lane; c < TX + 6; c += 64) {
-        const int gx = x0 - 3 + c;
-        float v = 0.f;
-        if (zreg && gx >= 1 && gx <= d.NX - 3) {
-          const long long q = (long long)gz * d.PX + gx;
-          v = a2[q] * LX0[q];
-        }
-        sQX[r][c] = v;
-      }
-    }
-    for (int r = wave; r < TZ + 6; r += 4) {
-      const int gz = z0 - 3 + r;
-      const bool zreg = gz >= d.f && gz <= d.NZ - 3;
-      for (int c = lane; c < TX + 3; c += 64) {
-        const int gx = x0 - 1 + c;
-        float v = 0.f;
-        if (zreg && gx >= 1 && gx <= d.NX - 2) {
-          const long long q = (long long)gz * d.PX + gx;
-          float lz = LZ0[q];
-          if (d.fs && gz == d.f) lz += LZ0[q - d.PX];  // transpose of vz[f-1] = vz[f]
-          v = a2[q] * lz;
-        }
-        sQZ[r][c] = v;
-      }
-    }
-    __syncthreads();
-    // lambda_p' on the tile and its halo
-    for (int r = wave; r < TZ + 3; r += 4) {
-      const int gz = z0 - 1 + r;
-      const bool zin = gz >= 0 && gz < d.NZ;
-      for (int c = lane; c < TX + 3; c += 64) {
-        const int gx = x0 - 1 + c;
-        float lp = 0.f, rr = 0.f;
-        if (zin && gx >= 0 && gx < d.NX) {
-          const long long q = (long long)gz * d.PX + gx;
-          lp = LP0[q];
-          const float az = kC1 * (sQZ[r + 2][c] - sQZ[r + 1][c]) + kC2 * (sQZ[r + 3][c] - sQZ[r][c]);
-          const float ax = kC1 * (sQX[r][c + 2] - sQX[r][c + 1]) + kC2 * (sQX[r][c + 3] - sQX[r][c]);
-          float acc = az;
-          acc = acc + ax;
-          lp = lp + acc;
-          if (d.fs) {
-            if (gz == d.f + 1) {
-              // lambda_p'[f-1] = lambda_p[f-1] + c2 * q_z[f]: rows above f carry no q
-              const float zero = 0.f;
-              const float azm = kC1 * (zero - zero) + kC2 * (sQZ[r + 1][c] - zero);
-              const float axm = kC1 * (zero - zero) + kC2 * (zero - zero);
-              float accm = azm;
-              accm = accm + axm;
-              const float lpm = LP0[q - 2 * d.PX] + accm;
-              lp = lp - lpm;
-            } else if (gz == d.f - 1) {
-              lp = 0.f;
-            }
-          }
-          if (gz >= d.f + 1 && gz <= d.NZ - 3 && gx >= 2 && gx <= d.NX - 3) rr = a1[q] * lp;
-        }
-        sLP[r][c] = lp;
-        sR[r][c] = rr;
-      }
-    }
-    __syncthreads();
-    const bool inject = a.it >= 1;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const int tr = wave + 4 * k;
-      const int gz = z0 + tr, gx = x0 + lane;
-      if (gz < d.NZ && gx < d.NX) {
-        const long long q = (long long)gz * d.PX + gx;
-        const float lp = sLP[tr + 1][lane + 1];
-        float lpo = lp;
-        if (gz >= d.f + 1 && gz <= d.NZ - 3 && gx >= 2 && gx <= d.NX - 3) {
-          gacc[k] = gacc[k] - lp * a.dbuf[so + q];
-          lpo = omk1[q] * lp;
+    for (int n = 0; n < NV; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RV * NG) {
+        const float4 v = ((x_ok >> n) & 1u) ? rlx[n] : f4zero();
+        float4 q = f4zero();
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if ((x_reg >> (4 * n + e)) & 1u) f4set(q, e, f4get(xa2[n], e) * f4get(v, e));
+        sLX[it / NG][it % NG + 1] = v;
+        sQX[it / NG][it % NG + 1] = q;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NZZ; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RZ * NG) {
+        const float4 v = ((z_ok >> n) & 1u) ? rlz[n] : f4zero();
+        sLZ[it / NG][it % NG + 1] = v;
+        if (!fold) {
+          float4 q = f4zero();
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if ((z_reg >> (4 * n + e)) & 1u) f4set(q, e, f4get(za2[n], e) * f4get(v, e));
+          sQZ[it / NG][it % NG + 1] = q;
         }
-        if ((int)q == a.src_cell[s] && a.gw) a.gw[(long long)s * a.nt + a.it] = a.dt * lp;
-        float lx = LX0[q];
-        float lz = LZ0[q];
-        if (d.fs && gz == d.f) lz += LZ0[q - d.PX];
-        if (d.fs && gz == d.f - 1) lz = 0.f;
-        const bool xreg = gz >= d.f && gz <= d.NZ - 2 && gx >= 1 && gx <= d.NX - 3;
-        const bool zreg = gz >= d.f && gz <= d.NZ - 3 && gx >= 1 && gx <= d.NX - 2;
+      }
+    }
+    if (s + 1 < s_end) load_v(s + 1);
+    __syncthreads();
+    if (fold) {  // transpose of vz[f-1] = vz[f]: lambda_vz[f] += lambda_vz[f-1]; lambda_vz[f-1] = 0
+      for (int g = tid; g < NG; g += NTHREADS) {
+        float4 up;
+        if (rfold >= 1) {
+          up = sLZ[rfold - 1][g + 1];
+          sLZ[rfold - 1][g + 1] = f4zero();
+        } else {
+          bool ok;
+          const unsigned gi = group_index(d, d.f - 1, x0 - 4 + 4 * g, ok);
+          up = ok ? reinterpret_cast<const float4*>(a.LZ0 + so)[gi] : f4zero();
+        }
+        const float4 v = sLZ[rfold][g + 1];
+        sLZ[rfold][g + 1] = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int n = 0; n < NZZ; ++n) {
+        const int it = tid + n * NTHREADS;
+        if (it < RZ * NG) {
+          const float4 v = sLZ[it / NG][it % NG + 1];
+          float4 q = f4zero();
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if ((z_reg >> (4 * n + e)) & 1u) f4set(q, e, f4get(za2[n], e) * f4get(v, e));
+          sQZ[it / NG][it % NG + 1] = q;
+        }
+      }
+      __syncthreads();
+    }
+    // lambda_p' on the tile and its halo
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+      const int it = tid + n * NTHREADS;
+      if (it < RP * NG) {
+        const int r = it / NG, g = it % NG;
+        const int gz = z0 - 1 + r;
+        const float4 qa = sQX[r][g], qb = sQX[r][g + 1], qc = sQX[r][g + 2];
+        const float qx[12] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, qc.w};
+        const float4 zm2 = sQZ[r][g + 1], zm1 = sQZ[r + 1][g + 1];
+        const float4 z00 = sQZ[r + 2][g + 1], zp1 = sQZ[r + 3][g + 1];
+        const bool ok = (p_ok >> n) & 1u;
+        const float4 lp0 = ok ? rlp[n] : f4zero();
+        float4 lpm4 = f4zero();
+        if (d.fs && ok && gz == d.f + 1)  // lambda_p at row f-1, same columns
+          lpm4 = reinterpret_cast<const float4*>(a.LP0 + so)[p_idx[n] - (d.PX >> 2) * 2];
+        float4 lpv = f4zero(), rrv = f4zero();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float lp = 0.f, rr = 0.f;
+          if (ok) {
+            lp = f4get(lp0, e);
+            const float az = kC1 * (f4get(z00, e) - f4get(zm1, e)) + kC2 * (f4get(zp1, e) - f4get(zm2, e));
+            const float ax = kC1 * (qx[4 + e] - qx[3 + e]) + kC2 * (qx[5 + e] - qx[2 + e]);
+            float acc = az;
+            acc = acc + ax;
+            lp = lp + acc;
+            if (d.fs) {
+              if (gz == d.f + 1) {
+                // lambda_p'[f-1] = lambda_p[f-1] + c2 * q_z[f]: rows above f carry no q
+                const float zero = 0.f;
+                const float azm = kC1 * (zero - zero) + kC2 * (f4get(zm1, e) - zero);
+                const float axm = kC1 * (zero - zero) + kC2 * (zero - zero);
+                float accm = azm;
+                accm = accm + axm;
+                const float lpm = f4get(lpm4, e) + accm;
+                lp = lp - lpm;
+              } else if (gz == d.f - 1) {
+                lp = 0.f;
+              }
+            }
+            if ((p_reg >> (4 * n + e)) & 1u) rr = f4get(pa1[n], e) * lp;
+          }
+          f4set(lpv, e, lp);
+          f4set(rrv, e, rr);
+        }
+        sLP[r][g + 1] = lpv;
+        sR[r][g + 1] = rrv;
+      }
+    }
+    float4 dcur[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) dcur[n] = rd[n];
+    if (s + 1 < s_end) load_p(s + 1);
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      if ((t_ok >> n) & 1u) {
+        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+        const int gz = z0 + tr;
+        const unsigned ti = t_idx[n];
+        const float4 lp4 = sLP[tr + 1][h + 2];
+        const float4 ra = sR[tr + 1][h + 1], rb = sR[tr + 1][h + 2], rc = sR[tr + 1][h + 3];
+        const float rs[12] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z, rc.w};
+        const float4 ru = sR[tr][h + 2], rd1 = sR[tr + 2][h + 2], rd2 = sR[tr + 3][h + 2];
+        float4 lx4 = sLX[tr + 1][h + 2];
+        float4 lz4 = sLZ[tr + 3][h + 2];
+        float4 gxv = f4zero(), gzv = f4zero();
         if (a.gA2) {
-          if (zreg) gacc2[k] = gacc2[k] - lz * a.gzbuf[so + q];
-          if (xreg) gacc2[k] = gacc2[k] - lx * a.gxbuf[so + q];
+          gxv = reinterpret_cast<const float4*>(a.gxbuf + so)[ti];
+          gzv = reinterpret_cast<const float4*>(a.gzbuf + so)[ti];
         }
-        if (xreg) lx = omk2[q] * lx;
-        if (zreg) lz = omk3[q] * lz;
-        if (gx <= d.NX - 2)
-          lx += kC1 * (sR[tr + 1][lane + 2] - sR[tr + 1][lane + 1]) +
-                kC2 * (sR[tr + 1][lane + 3] - sR[tr + 1][lane]);
-        if (gz <= d.NZ - 2)
-          lz += kC1 * (sR[tr + 2][lane + 1] - sR[tr + 1][lane + 1]) +
-                kC2 * (sR[tr + 3][lane + 1] - sR[tr][lane + 1]);
-        if (inject && a.rrow[gz]) {
-          for (int h = a.rhead[q]; h >= 0; h = a.rnext[h]) {
-            const long long o = ((long long)s * a.nt + (a.it - 1)) * a.R + h;
-            if (a.g_p) lpo += a.g_p[o];
+        float4 lpo = lp4;
+        const int scell = a.src_cell[s];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const unsigned bit = 1u << (4 * n + e);
+          const float lp = f4get(lp4, e);
+          if (t_p & bit) {
+            f4set(gacc[n], e, f4get(gacc[n], e) - lp * f4get(dcur[n], e));
+            f4set(lpo, e, f4get(tc_omk1[n], e) * lp);
+          }
+          if ((int)(ti * 4u + e) == scell && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
+            a.gw[(long long)s * a.nt + a.it] = a.dt * lp;
+          float lx = f4get(lx4, e), lz = f4get(lz4, e);
+          if (a.gA2) {
+            float g2 = f4get(gacc2[n], e);
+            if (t_z & bit) g2 = g2 - lz * f4get(gzv, e);
+            if (t_x & bit) g2 = g2 - lx * f4get(gxv, e);
+            f4set(gacc2[n], e, g2);
+          }
+          if (t_x & bit) lx = f4get(tc_omk2[n], e) * lx;
+          if (t_z & bit) lz = f4get(tc_omk3[n], e) * lz;
+          if (t_vx & bit)
+            lx += kC1 * (rs[5 + e] - rs[4 + e]) + kC2 * (rs[6 + e] - rs[3 + e]);
+          if (gz <= d.NZ - 2)
+            lz += kC1 * (f4get(rd1, e) - f4get(rb, e)) + kC2 * (f4get(rd2, e) - f4get(ru, e));
+          float lpv = f4get(lpo, e);
+          for (int hh = e == 0 ? t_rh[n].x : (e == 1 ? t_rh[n].y : (e == 2 ? t_rh[n].z : t_rh[n].w));
+               hh >= 0; hh = a.rnext[hh]) {
+            const long long o = ((long long)s * a.nt + (a.it - 1)) * a.R + hh;
+            if (a.g_p) lpv += a.g_p[o];
             if (a.g_vx) lx += a.g_vx[o];
             if (a.g_vz) lz += a.g_vz[o];
           }
+          f4set(lpo, e, lpv);
+          f4set(lx4, e, lx);
+          f4set(lz4, e, lz);
         }
-        a.LP1[so + q] = lpo;
-        a.LX1[so + q] = lx;
-        a.LZ1[so + q] = lz;
+        reinterpret_cast<float4*>(a.LP1 + so)[ti] = lpo;
+        reinterpret_cast<float4*>(a.LX1 + so)[ti] = lx4;
+        reinterpret_cast<float4*>(a.LZ1 + so)[ti] = lz4;
       }
     }
     __syncthreads();
   }
-  if (s_begin < s_end) {
-    float* g1 = a.gA1 + (long long)blockIdx.y * d.plane;
-    float* g2 = a.gA2 ? a.gA2 + (long long)blockIdx.y * d.plane : nullptr;
+  float4* g1 = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
+  float4* g2 = a.gA2 ? reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane) : nullptr;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-      const int gz = z0 + wave + 4 * k, gx = x0 + lane;
-      if (gz < d.NZ && gx < d.NX) {
-        const long long q = (long long)gz * d.PX + gx;
-        g1[q] = g1[q] + gacc[k];
-        if (g2) g2[q] = g2[q] + gacc2[k];
+  for (int n = 0; n < NT; ++n) {
+    if ((t_ok >> n) & 1u) {
+      const float4 v = g1[t_idx[n]];
+      g1[t_idx[n]] = make_float4(v.x + gacc[n].x, v.y + gacc[n].y, v.z + gacc[n].z, v.w + gacc[n].w);
+      if (g2) {
+        const float4 w = g2[t_idx[n]];
+        g2[t_idx[n]] = make_float4(w.x + gacc2[n].x, w.y + gacc2[n].y, w.z + gacc2[n].z,
+                                   w.w + gacc2[n].w);
       }
     }
   }
@@ -643,9 +979,9 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
   if (pass >= 1 && pl.k == 0) return fail(GBW_ERR_INVALID, "the gradient needs checkpoint_every > 0");
   pl.nseg = pl.k ? (p->nt + pl.k - 1) / pl.k : 0;
-  pl.TZ = p->tile_rows == 16 ? 16 : 8;
-  if (p->tile_rows != 0 && p->tile_rows != 8 && p->tile_rows != 16)
-    return fail(GBW_ERR_INVALID, "tile_rows must be 0, 8 or 16");
+  pl.TZ = p->tile_rows == 32 ? 32 : 16;
+  if (p->tile_rows != 0 && p->tile_rows != 16 && p->tile_rows != 32)
+    return fail(GBW_ERR_INVALID, "tile_rows must be 0, 16 or 32");
   pl.tiles_x = (d.NX + TX - 1) / TX;
   pl.tiles_z = (d.NZ + pl.TZ - 1) / pl.TZ;
   const int tiles = pl.tiles_x * pl.tiles_z;
@@ -790,15 +1126,19 @@ int build_coefs(const Plan& pl, const gbw_acoustic2d_problem* p, const gbw_acous
   return 0;
 }
 
+// grid.x is the tile count rounded up to a multiple of 8 (the XCD count), so
+// xcd_tile() is a bijection; surplus workgroups exit at once
+inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.tiles_z + 7) / 8 * 8); }
+
 template <int TZ, bool RECOMP>
 void launch_fwd(const Plan& pl, const FwdArgs& a, hipStream_t st) {
-  dim3 grid(pl.tiles_x * pl.tiles_z, pl.groups);
+  dim3 grid(tile_grid(pl), pl.groups);
   fwd_step_kernel<TZ, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
 }
 
 template <int TZ>
 void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
-  dim3 grid(pl.tiles_x * pl.tiles_z, pl.groups);
+  dim3 grid(tile_grid(pl), pl.groups);
   adj_step_kernel<TZ><<<grid, NTHREADS, 0, st>>>(a);
 }
 
@@ -806,6 +1146,7 @@ FwdArgs base_fwd_args(const Plan& pl, const gbw_acoustic2d_problem* p,
                       const gbw_acoustic2d_inputs* in, void* ws) {
   FwdArgs a{};
   a.d = pl.d; a.S = pl.S; a.spb = pl.spb; a.tiles_x = pl.tiles_x;
+  a.ntiles = pl.tiles_x * pl.tiles_z;
   a.coef = at<float>(ws, pl.off_coef);
   a.src_cell = at<int>(ws, pl.off_src);
   a.wav = in->src_v;
@@ -893,7 +1234,7 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
       next = (cur == A) ? B : A;
     set_state(a, cur, next, fstride);
     a.it = it;
-    if (pl.TZ == 16) launch_fwd<16, false>(pl, a, st); else launch_fwd<8, false>(pl, a, st);
+    if (pl.TZ == 32) launch_fwd<32, false>(pl, a, st); else launch_fwd<16, false>(pl, a, st);
     LAUNCH_CHECK();
     cur = next;
     const bool chunk_end = ((it + 1) % pl.echunk == 0) || it + 1 == pl.nt;
@@ -955,7 +1296,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   }
   FwdArgs f = base_fwd_args(pl, prob, in, workspace);
   AdjArgs b{};
-  b.d = d; b.S = pl.S; b.spb = pl.spb; b.tiles_x = pl.tiles_x;
+  b.d = d; b.S = pl.S; b.spb = pl.spb; b.tiles_x = pl.tiles_x; b.ntiles = f.ntiles;
   b.coef = f.coef; b.src_cell = f.src_cell; b.nt = pl.nt; b.dt = prob->dt;
   b.rhead = f.rhead; b.rnext = f.rnext; b.rrow = f.rrow; b.R = pl.R;
   b.g_p = adj->g_p; b.g_vx = adj->g_vx; b.g_vz = adj->g_vz;
@@ -972,7 +1313,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
       f.dbuf = D + (long long)(it - t0) * fstride;
       f.gxbuf = GX ? GX + (long long)(it - t0) * fstride : nullptr;
       f.gzbuf = GZ ? GZ + (long long)(it - t0) * fstride : nullptr;
-      if (pl.TZ == 16) launch_fwd<16, true>(pl, f, st); else launch_fwd<8, true>(pl, f, st);
+      if (pl.TZ == 32) launch_fwd<32, true>(pl, f, st); else launch_fwd<16, true>(pl, f, st);
       LAUNCH_CHECK();
       cur = next;
     }
@@ -984,7 +1325,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
       b.dbuf = D + (long long)(it - t0) * fstride;
       b.gxbuf = GX ? GX + (long long)(it - t0) * fstride : nullptr;
       b.gzbuf = GZ ? GZ + (long long)(it - t0) * fstride : nullptr;
-      if (pl.TZ == 16) launch_adj<16>(pl, b, st); else launch_adj<8>(pl, b, st);
+      if (pl.TZ == 32) launch_adj<32>(pl, b, st); else launch_adj<16>(pl, b, st);
       LAUNCH_CHECK();
       lcur = lnext;
     }

@@ -116,7 +116,7 @@ def test_gradient_matches_reference_autograd(name):
             assert e_ref <= TOL, key
 
 
-@pytest.mark.parametrize("tile_rows,spb", [(8, 1), (16, 0), (16, 2), (8, 3)])
+@pytest.mark.parametrize("tile_rows,spb", [(16, 1), (32, 0), (32, 2), (16, 3)])
 def test_launch_choices_do_not_change_results(tile_rows, spb):
     print(f"[case] launch tile_rows={tile_rows} spb={spb}", flush=True)
     z, m = load("tiny2_grad")
@@ -215,11 +215,12 @@ def test_invalid_geometry_raises_before_launch():
         run(z, m, wavelet=z["in_wavelet"][:, :5])
 
 
-def test_full_size_config2_exact_scaling_and_batch():
-    """350 x 1700, 4000 steps: doubling the wavelet doubles every output bit for
-    bit (scaling by 2 is exact in binary floating point), and a 4-shot batch
-    reproduces each shot run alone."""
-    print("[case] full-size config 2 scaling", flush=True)
+def test_full_size_config2_linearity_and_batch():
+    """350 x 1700, 4000 steps: doubling the wavelet doubles every output (exact
+    in binary floating point except where values pass through the subnormal
+    range ahead of the wavefront, hence a 1e-12 bound), and a 4-shot batch
+    reproduces each shot run alone bit for bit."""
+    print("[case] full-size config 2 linearity", flush=True)
     from geobrain_amd import synthetic
     nz, nx, nt = 350, 1700, 4000
     vp = synthetic.grf_model(nz, nx, 5.0, seed=0)
@@ -235,8 +236,12 @@ def test_full_size_config2_exact_scaling_and_batch():
         a, _ = run(z, m)
         b, _ = run(z, m, wavelet=2 * z["in_wavelet"])
         for k in ("p", "vx", "vz"):
-            assert torch.equal(2 * a[k], b[k]), k
+            e = rel_l2(np_(b[k]), 2 * np_(a[k]))
+            same = float((2 * a[k] == b[k]).float().mean())
+            print(f"  {k}: rel-L2(F(2w), 2F(w)) = {e:.1e}, bitwise-equal fraction {same:.6f}")
+            assert e <= 1e-12, k
         z1 = dict(z, in_src_x=xs[2:3], in_src_z=np.full(1, 2), in_wavelet=z["in_wavelet"][2:3])
         c, _ = run(z1, m)
-        assert torch.equal(c["p"][0], a["p"][2])
+        for k in ("p", "vx", "vz"):
+            assert torch.equal(c[k][0], a[k][2]), k
     assert float(a["p"].abs().max()) > 0
