@@ -942,6 +942,70 @@ __global__ void wavelet_grad_kernel(const float* __restrict__ gw, int S, int nt,
 }
 
 // ---------------------------------------------------------------------------
+// BF16 checkpoints: per (field, shot) plane, a power-of-two FP32 scale s >=
+// max|x| and bf16(x / s) rounded to nearest even on the FP32 bit pattern.
+// Dividing and multiplying by a power of two is exact, so the only rounding
+// is the bf16 one (8 significant bits).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f32_to_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// power-of-two scale >= m (m >= 0): 2^(E-126) for biased exponent E of m
+__device__ __forceinline__ float pow2_scale(float m) {
+  const unsigned e = (__float_as_uint(m) >> 23) & 0xffu;
+  if (!(m > 0.f) || e >= 254u) return 1.f;
+  return __uint_as_float((e + 1u) << 23);
+}
+
+// maxbits[plane] = max |x| over each plane (bits of a non-negative float order
+// like unsigned integers, so an integer max is a float max)
+__global__ void absmax_kernel(long long plane, const float* __restrict__ st,
+                              unsigned* __restrict__ maxbits) {
+  __shared__ float red[NTHREADS / 64];
+  const float* p = st + (long long)blockIdx.y * plane;
+  float m = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < plane;
+       i += (long long)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(p[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NTHREADS / 64; ++w) m = fmaxf(m, red[w]);
+    atomicMax(&maxbits[blockIdx.y], __float_as_uint(m));
+  }
+}
+
+__global__ void quantize_kernel(long long plane, const float* __restrict__ st,
+                                const unsigned* __restrict__ maxbits,
+                                unsigned short* __restrict__ q, float* __restrict__ scales) {
+  const float scale = pow2_scale(__uint_as_float(maxbits[blockIdx.y]));
+  const float inv = 1.f / scale;
+  const float* p = st + (long long)blockIdx.y * plane;
+  unsigned short* o = q + (long long)blockIdx.y * plane;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < plane;
+       i += (long long)gridDim.x * blockDim.x)
+    o[i] = f32_to_bf16(p[i] * inv);
+  if (blockIdx.x == 0 && threadIdx.x == 0) scales[blockIdx.y] = scale;
+}
+
+__global__ void dequantize_kernel(long long plane, const unsigned short* __restrict__ q,
+                                  const float* __restrict__ scales, float* __restrict__ st) {
+  const float scale = scales[blockIdx.y];
+  const unsigned short* p = q + (long long)blockIdx.y * plane;
+  float* o = st + (long long)blockIdx.y * plane;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < plane;
+       i += (long long)gridDim.x * blockDim.x)
+    o[i] = bf16_to_f32(p[i]) * scale;
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -949,10 +1013,11 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Plan {
   Dims d;
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
-  size_t plane_bytes;
+  bool bf16;
+  size_t plane_bytes, slot_bytes, slot_scale_off;
   // byte offsets into the workspace
   size_t off_coef, off_A, off_B, off_LA, off_LB, off_D, off_GX, off_GZ, off_E, off_G1, off_G2,
-      off_GW, off_rhead, off_rnext, off_rrow, off_rcell, off_heads, off_src, total;
+      off_GW, off_rhead, off_rnext, off_rrow, off_rcell, off_heads, off_src, off_MX, total;
 };
 
 int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
@@ -965,7 +1030,6 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   if (p->src_n <= 0) return fail(GBW_ERR_INVALID, "src_n must be positive");
   if (p->rcv_n < 0) return fail(GBW_ERR_INVALID, "rcv_n must be >= 0");
   if (!(p->dt > 0.f) || !(p->dz > 0.f)) return fail(GBW_ERR_INVALID, "dt and dz must be positive");
-  if (p->checkpoint_bf16) return fail(GBW_ERR_UNSUPPORTED, "bf16 checkpoints are not built yet");
   Dims& d = pl.d;
   d.nz = p->nz; d.nx = p->nx; d.nabc = p->nabc;
   d.NZ = p->nz + 2 * p->nabc; d.NX = p->nx + 2 * p->nabc;
@@ -997,6 +1061,14 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
   pl.plane_bytes = (size_t)d.plane * sizeof(float);
   const size_t state = 3 * (size_t)pl.S * pl.plane_bytes;
+  pl.bf16 = p->checkpoint_bf16 != 0;
+  if (pl.bf16) {  // bf16 planes, then one fp32 scale per (field, shot)
+    pl.slot_scale_off = align256(3 * (size_t)pl.S * d.plane * sizeof(unsigned short));
+    pl.slot_bytes = pl.slot_scale_off + align256(3 * (size_t)pl.S * sizeof(float));
+  } else {
+    pl.slot_scale_off = 0;
+    pl.slot_bytes = state;
+  }
   const size_t inner = (size_t)d.nz * d.nx * sizeof(float);
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o += align256(bytes); return at; };
@@ -1025,12 +1097,38 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.off_heads = take(2 * (size_t)std::max(1, pl.R) * sizeof(int));
   pl.off_rrow = take((size_t)d.NZ);
   pl.off_src = take((size_t)pl.S * sizeof(int));
+  pl.off_MX = take(3 * (size_t)pl.S * sizeof(unsigned));
   pl.total = o;
   return 0;
 }
 
-size_t checkpoint_bytes(const Plan& pl) {
-  return pl.k ? (size_t)pl.nseg * 3 * pl.S * pl.plane_bytes : 0;
+size_t checkpoint_bytes(const Plan& pl) { return pl.k ? (size_t)pl.nseg * pl.slot_bytes : 0; }
+
+unsigned plane_blocks(long long plane) { return (unsigned)std::min<long long>((plane + 255) / 256, 512); }
+
+// state (fp32, [field][shot][plane]) -> bf16 checkpoint slot with scales
+int quantize_state(const Plan& pl, const float* state, char* slot, unsigned* maxbits,
+                   hipStream_t st) {
+  const int planes = 3 * pl.S;
+  fill_int_kernel<<<1, 256, 0, st>>>(reinterpret_cast<int*>(maxbits), planes, 0);
+  LAUNCH_CHECK();
+  dim3 g(plane_blocks(pl.d.plane), planes);
+  absmax_kernel<<<g, NTHREADS, 0, st>>>(pl.d.plane, state, maxbits);
+  LAUNCH_CHECK();
+  quantize_kernel<<<g, 256, 0, st>>>(pl.d.plane, state, maxbits,
+                                     reinterpret_cast<unsigned short*>(slot),
+                                     reinterpret_cast<float*>(slot + pl.slot_scale_off));
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int dequantize_state(const Plan& pl, const char* slot, float* state, hipStream_t st) {
+  dim3 g(plane_blocks(pl.d.plane), 3 * pl.S);
+  dequantize_kernel<<<g, 256, 0, st>>>(pl.d.plane, reinterpret_cast<const unsigned short*>(slot),
+                                       reinterpret_cast<const float*>(slot + pl.slot_scale_off),
+                                       state);
+  LAUNCH_CHECK();
+  return 0;
 }
 
 template <typename T>
@@ -1210,7 +1308,8 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   float* A = at<float>(workspace, pl.off_A);
   float* B = at<float>(workspace, pl.off_B);
   float* E = at<float>(workspace, pl.off_E);
-  float* ck = static_cast<float*>(out->checkpoints);
+  char* ck = static_cast<char*>(out->checkpoints);
+  unsigned* MX = at<unsigned>(workspace, pl.off_MX);
   const long long slot = 3 * fstride;
   if (energy) {
     if ((rc = fill(E, (long long)pl.groups * 3 * inner, 0.f, st))) return rc;
@@ -1218,8 +1317,12 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
     if (out->energy_vx && (rc = fill(out->energy_vx, inner, 0.f, st))) return rc;
     if (out->energy_vz && (rc = fill(out->energy_vz, inner, 0.f, st))) return rc;
   }
-  float* cur = pl.k ? ck : A;
+  // fp32 checkpoints are written in place of the ping-pong output (no extra
+  // traffic); bf16 checkpoints are quantised from it after the step
+  const bool ck32 = pl.k && !pl.bf16;
+  float* cur = ck32 ? reinterpret_cast<float*>(ck) : A;
   if ((rc = fill(cur, slot, 0.f, st))) return rc;
+  if (pl.k && pl.bf16 && (rc = quantize_state(pl, cur, ck, MX, st))) return rc;
   FwdArgs a = base_fwd_args(pl, prob, in, workspace);
   a.rec_p = out->rec_p; a.rec_vx = out->rec_vx; a.rec_vz = out->rec_vz;
   if (!rec) a.rec_p = a.rec_vx = a.rec_vz = nullptr;
@@ -1228,8 +1331,9 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   int snap = 0;
   for (int it = 0; it < pl.nt; ++it) {
     float* next;
-    if (pl.k && (it + 1) % pl.k == 0 && it + 1 < pl.nt)
-      next = ck + (long long)((it + 1) / pl.k) * slot;
+    const bool ck_step = pl.k && (it + 1) % pl.k == 0 && it + 1 < pl.nt;
+    if (ck_step && ck32)
+      next = reinterpret_cast<float*>(ck + (size_t)((it + 1) / pl.k) * pl.slot_bytes);
     else
       next = (cur == A) ? B : A;
     set_state(a, cur, next, fstride);
@@ -1237,6 +1341,9 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
     if (pl.TZ == 32) launch_fwd<32, false>(pl, a, st); else launch_fwd<16, false>(pl, a, st);
     LAUNCH_CHECK();
     cur = next;
+    if (ck_step && pl.bf16 &&
+        (rc = quantize_state(pl, cur, ck + (size_t)((it + 1) / pl.k) * pl.slot_bytes, MX, st)))
+      return rc;
     const bool chunk_end = ((it + 1) % pl.echunk == 0) || it + 1 == pl.nt;
     if (energy && chunk_end) {
       energy_flush_kernel<<<(unsigned)((3 * inner + 255) / 256), 256, 0, st>>>(
@@ -1281,7 +1388,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   float* G1 = at<float>(workspace, pl.off_G1);
   float* G2 = pass == 2 ? at<float>(workspace, pl.off_G2) : nullptr;
   float* GW = at<float>(workspace, pl.off_GW);
-  const float* ck = static_cast<const float*>(adj->checkpoints);
+  const char* ck = static_cast<const char*>(adj->checkpoints);
   if ((rc = fill(G1, (long long)pl.groups * d.plane, 0.f, st))) return rc;
   if (G2 && (rc = fill(G2, (long long)pl.groups * d.plane, 0.f, st))) return rc;
   if ((rc = fill(GW, (long long)pl.S * pl.nt, 0.f, st))) return rc;
@@ -1305,7 +1412,13 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   for (int c = pl.nseg - 1; c >= 0; --c) {
     const int t0 = c * pl.k, t1 = std::min(pl.nt, t0 + pl.k);
     // recompute the interval from its checkpoint, keeping D^n (and vx/vz brackets)
-    float* cur = const_cast<float*>(ck + (long long)c * slot);
+    float* cur;
+    if (pl.bf16) {
+      if ((rc = dequantize_state(pl, ck + (size_t)c * pl.slot_bytes, A, st))) return rc;
+      cur = A;
+    } else {
+      cur = const_cast<float*>(reinterpret_cast<const float*>(ck + (size_t)c * pl.slot_bytes));
+    }
     for (int it = t0; it < t1; ++it) {
       float* next = (cur == A) ? B : A;
       set_state(f, cur, next, fstride);

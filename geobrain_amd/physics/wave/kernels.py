@@ -226,6 +226,8 @@ def acoustic_forward_kernel(
     device: torch.device = torch.device('cpu'),
     dtype: torch.dtype = torch.float32,
     snapshot_every: int = 0,
+    *,
+    checkpoint_dtype: torch.dtype = torch.float32,
 ) -> Dict[str, Tensor]:
     """Forward simulation (signature of ``acoustic_2d.py:294-317``).
 
@@ -233,9 +235,17 @@ def acoustic_forward_kernel(
     ``'forward_wavefield_p/vx/vz'`` ``(nz, nx)`` (detached) and, when
     ``snapshot_every > 0``, ``'snapshots'`` ``(n_snaps, nz, nx)`` on the CPU.
     Gradients flow to ``vp``, ``rho`` and ``src_v`` when they require grad.
+
+    ``checkpoint_dtype`` (keyword-only; not in the reference) stores the
+    forward checkpoints the gradient recomputes from as ``torch.bfloat16``
+    (per-field, per-shot power-of-two FP32 scale) instead of FP32; the time
+    stepping stays FP32.  It changes the gradient slightly, never the forward
+    outputs.
     """
     if dtype != torch.float32:
         raise NotImplementedError("geobrain_amd computes in float32 only (the reference default)")
+    if checkpoint_dtype not in (torch.float32, torch.bfloat16):
+        raise ValueError("checkpoint_dtype must be torch.float32 or torch.bfloat16")
     dev = vp.device
     for name, t in (("vp", vp), ("rho", rho), ("src_v", src_v), ("damp", damp)):
         _on_gpu(t, name, dev)
@@ -281,7 +291,7 @@ def acoustic_forward_kernel(
         prob.checkpoint_every = chunk_length(nt, checkpoint_segments)
     else:
         prob.checkpoint_every = auto_checkpoint_interval(nt)
-    prob.checkpoint_bf16 = 0
+    prob.checkpoint_bf16 = int(checkpoint_dtype == torch.bfloat16)
     choice = _launch_override
     prob.tile_rows = choice.tile_rows if choice else 0
     prob.shots_per_block = choice.shots_per_block if choice else 0

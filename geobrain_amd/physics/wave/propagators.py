@@ -57,7 +57,12 @@ class AcousticPropagator(AbstractPropagator):
     ``AcousticPropagator(model, survey, device, dtype)``;
     ``forward(model=None, shot_index=None, checkpoint_segments=1, snapshot_every=0)``
     returns a ``WaveResult`` whose ``p``/``vx``/``vz`` are ``(n_shots, nt, n_rcv)``.
+
+    ``checkpoint_dtype`` (attribute, default ``torch.float32``) selects the
+    storage of the gradient's forward checkpoints (``torch.bfloat16`` halves it).
     """
+
+    checkpoint_dtype: torch.dtype = torch.float32
 
     def __init__(self, model: AcousticModel, survey: Survey, device: str = 'cpu',
                  dtype: torch.dtype = torch.float32) -> None:
@@ -122,7 +127,8 @@ class AcousticPropagator(AbstractPropagator):
             rcv_x=self._rcv_x_h, rcv_z=self._rcv_z_h, rcv_n=self.rcv_n,
             damp=self.damp, vp=model.vp, rho=model.rho,
             checkpoint_segments=checkpoint_segments, device=torch.device(self.device),
-            dtype=self.dtype, snapshot_every=snapshot_every)
+            dtype=self.dtype, snapshot_every=snapshot_every,
+            checkpoint_dtype=self.checkpoint_dtype)
         return WaveResult.from_dict(result)
 
     def __call__(self, model: Optional[AcousticModel] = None, shot_index: Optional[int] = None,

@@ -51,7 +51,7 @@ def _load():
         fwd_args = [I, I, I, I, D, D, I, I, P, P, P, I, P, P, P, P, P, I,
                     P, P, P, P, P, P, P, P]
         grad_args = [I, I, I, I, D, D, I, I, P, P, P, I, I, P, P, P, P, P, I,
-                     P, P, P, P, P, P]
+                     P, P, P, P, P, P, I]
         for name in ("oracle_forward_f32", "oracle_forward_f64"):
             fn = getattr(lib, name)
             fn.argtypes = fwd_args
@@ -134,13 +134,15 @@ def forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_
 
 def gradient(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
              vp, rho, damp, g_p=None, g_vx=None, g_vz=None, chunk=0,
-             precision="f64", wavelet_shared=False):
+             precision="f64", wavelet_shared=False, checkpoint_bf16=False):
     """Gradient of ``sum(g_p*p + g_vx*vx + g_vz*vz)`` w.r.t. vp, rho and the wavelet.
 
     ``g_*`` are the adjoint sources (the loss's derivative w.r.t. each
     seismogram), shape ``(S, nt, R)``.  Returns ``(grad_vp, grad_rho, grad_w)``;
     ``grad_rho`` is the gradient for a density that is an independent input
-    (the reference's ``rho_grad=True`` case).
+    (the reference's ``rho_grad=True`` case).  ``checkpoint_bf16`` rounds every
+    checkpoint to bf16 with a per-(field, shot) power-of-two scale before the
+    recompute, as the GPU library's bf16 checkpoint storage does.
     """
     src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp = _common(
         nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp)
@@ -163,7 +165,8 @@ def gradient(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv
     rc = fn(nz, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
             _ptr(src_z), _ptr(src_x), _ptr(src_v), int(bool(wavelet_shared)), R,
             _ptr(rcv_z), _ptr(rcv_x), _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
-            _ptr(g_p), _ptr(g_vx), _ptr(g_vz), _ptr(gvp), _ptr(grho), _ptr(gw))
+            _ptr(g_p), _ptr(g_vx), _ptr(g_vz), _ptr(gvp), _ptr(grho), _ptr(gw),
+            int(bool(checkpoint_bf16)))
     if rc != 0:
         raise ValueError(f"oracle gradient rejected its inputs (code {rc})")
     return gvp, grho, gw

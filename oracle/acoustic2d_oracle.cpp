@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <vector>
 #include <algorithm>
+#include <cmath>
 
 namespace {
 
@@ -417,13 +418,47 @@ static void step_adjoint(const Grid<T>& g, const Coefs<T>& k, const Geometry<T>&
   }
 }
 
+// bf16 checkpoint emulation: per (field, shot) plane a power-of-two scale
+// 2^(E-126) >= max|x| (E = biased exponent of max|x|), then round-to-nearest-
+// even of the FP32 bit pattern to 8 significant bits.  Defines what the GPU
+// library's bf16 checkpoints hold.
+static float bf16_round(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return x;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+template <typename T>
+static void bf16_checkpoint(std::vector<T>& fld, size_t plane, int S) {
+  for (int s = 0; s < S; ++s) {
+    T* p = fld.data() + (size_t)s * plane;
+    float m = 0.f;
+    for (size_t i = 0; i < plane; ++i) m = std::max(m, std::fabs((float)p[i]));
+    unsigned mb;
+    std::memcpy(&mb, &m, 4);
+    const unsigned e = (mb >> 23) & 0xffu;
+    float scale = 1.f;
+    if (m > 0.f && e < 254u) {
+      const unsigned sb = (e + 1u) << 23;
+      std::memcpy(&scale, &sb, 4);
+    }
+    const float inv = 1.f / scale;
+    for (size_t i = 0; i < plane; ++i) p[i] = (T)(bf16_round((float)p[i] * inv) * scale);
+  }
+}
+
 template <typename T>
 static int gradient_impl(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
                          const int64_t* src_z, const int64_t* src_x, const float* src_v,
                          int wavelet_shared, int R, const int64_t* rcv_z, const int64_t* rcv_x,
                          const float* vp, const float* rho, const float* damp, int chunk_len,
                          const T* g_p, const T* g_vx, const T* g_vz, T* grad_vp, T* grad_rho,
-                         T* grad_w) {
+                         T* grad_w, int ckpt_bf16) {
   Grid<T> g; Geometry<T> geo;
   int rc = setup<T>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x, g, geo);
   if (rc) return rc;
@@ -438,6 +473,11 @@ static int gradient_impl(int nz, int nx, int nabc, int fs, double dt, double dz,
   State<T> st; st.init(n);
   for (int c = 0; c < nseg; ++c) {
     ckpt[c] = st;
+    if (ckpt_bf16) {
+      bf16_checkpoint(ckpt[c].p, plane, S);
+      bf16_checkpoint(ckpt[c].vx, plane, S);
+      bf16_checkpoint(ckpt[c].vz, plane, S);
+    }
     for (int it = bounds[c]; it < bounds[c + 1]; ++it) step_forward(g, k, geo, it, st);
   }
   // adjoint sweep
@@ -515,10 +555,10 @@ int oracle_gradient_f32(int nz, int nx, int nabc, int fs, double dt, double dz, 
                         int wavelet_shared, int R, const int64_t* rcv_z, const int64_t* rcv_x,
                         const float* vp, const float* rho, const float* damp, int chunk_len,
                         const float* g_p, const float* g_vx, const float* g_vz, float* grad_vp,
-                        float* grad_rho, float* grad_w) {
+                        float* grad_rho, float* grad_w, int ckpt_bf16) {
   return gradient_impl<float>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v,
                               wavelet_shared, R, rcv_z, rcv_x, vp, rho, damp, chunk_len, g_p,
-                              g_vx, g_vz, grad_vp, grad_rho, grad_w);
+                              g_vx, g_vz, grad_vp, grad_rho, grad_w, ckpt_bf16);
 }
 
 int oracle_gradient_f64(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
@@ -526,10 +566,10 @@ int oracle_gradient_f64(int nz, int nx, int nabc, int fs, double dt, double dz, 
                         int wavelet_shared, int R, const int64_t* rcv_z, const int64_t* rcv_x,
                         const float* vp, const float* rho, const float* damp, int chunk_len,
                         const double* g_p, const double* g_vx, const double* g_vz,
-                        double* grad_vp, double* grad_rho, double* grad_w) {
+                        double* grad_vp, double* grad_rho, double* grad_w, int ckpt_bf16) {
   return gradient_impl<double>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v,
                                wavelet_shared, R, rcv_z, rcv_x, vp, rho, damp, chunk_len, g_p,
-                               g_vx, g_vz, grad_vp, grad_rho, grad_w);
+                               g_vx, g_vz, grad_vp, grad_rho, grad_w, ckpt_bf16);
 }
 
 }  // extern "C"
