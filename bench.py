@@ -21,6 +21,7 @@ host's cores) timed on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -153,6 +154,36 @@ def cpu_baseline(w, steps=400, shots=1):
                       f"{t:.1f} s wall"}
 
 
+def per_shot_gpu_baseline(w, d_obs, steps=200, k=50):
+    """The plain per-shot GPU path: the PyTorch restatement of the reference
+    operator (eager ATen launches, autograd through reentrant checkpoint
+    chunks of k steps) on the GPU for one shot, `steps` time steps, timed with
+    events after one warm-up."""
+    import torch
+    dev = torch.device("cuda:0")
+    T = lambda a, **kw: torch.as_tensor(a, device=dev, **kw)  # noqa: E731
+    args = dict(nz=w["nz"], nx=w["nx"], nabc=w["nabc"], free_surface=False, dt=w["dt"],
+                dz=w["dx"], src_z=T(w["src_z"][:1]), src_x=T(w["src_x"][:1]),
+                src_v=T(w["wavelet"][:1, :steps]), rcv_z=T(w["rcv_z"]), rcv_x=T(w["rcv_x"]),
+                damp=T(w["damp"]), rho=T(w["rho"]), checkpoint_segments=max(1, steps // k))
+    obs = d_obs[:1, :steps]
+    times = []
+    for _ in range(2):
+        vp = T(w["vp"]).requires_grad_(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = eager_forward(vp=vp, **args)
+        ((out["p"] - obs) ** 2).sum().backward()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+    cells = (w["nz"] + 2 * w["nabc"]) * (w["nx"] + 2 * w["nabc"]) * steps
+    return {"value": round(cells / times[-1] / 1e9, 4), "unit": "Gcell-updates/s",
+            "sample": f"PyTorch eager restatement of the reference operator on the GPU, "
+                      f"1 shot x {steps} steps, checkpoint every {k}, forward + autograd "
+                      f"gradient, {times[-1]:.2f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +193,8 @@ def main():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--no-tune", action="store_true", help="library default launch choice")
+    ap.add_argument("--no-eager", action="store_true", help="skip the per-shot PyTorch GPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -187,6 +220,10 @@ def main():
         d_obs = wave.acoustic_forward_kernel(vp=T(w["vp_obs"]), **common)["p"]
     vp = T(w["vp"]).requires_grad_(True)
     stream = torch.cuda.current_stream(dev)
+    choice = None
+    if not args.no_tune:
+        from geobrain_amd import autotune
+        choice = autotune.tune(dict(common, vp=vp, nt=nt), nt_trial=60, verbose=True)
 
     def step(ev=None):
         vp.grad = None
@@ -239,7 +276,8 @@ def main():
         "config": {"workload": f"batched gradient: {S} shots, {w['nz']}x{w['nx']} grid "
                                f"(+{w['nabc']} PML), {nt} steps, checkpoint every {k}",
                    "shots": S, "grid": [w["nz"], w["nx"]], "padded": [NZ, NX], "nt": nt,
-                   "checkpoint_every": k},
+                   "checkpoint_every": k,
+                   "launch": None if choice is None else dataclasses.asdict(choice)},
         "roofline": {"bound": "hbm", "kernel": "fwd_step_kernel (forward pass)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -252,6 +290,10 @@ def main():
                    "job_bytes_per_cell_update": 80.0 + 24.0 / k},
         "cpu_baseline": None,
     }
+    if not args.no_eager:
+        base = per_shot_gpu_baseline(w, d_obs)
+        rec["per_shot_gpu"] = base
+        rec["speedup_vs_per_shot_gpu"] = round(value / base["value"], 1)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, steps=args.cpu_steps)
     print(json.dumps(rec), flush=True)

@@ -17,7 +17,7 @@ c_i32 = ctypes.c_int32
 c_f32 = ctypes.c_float
 c_vp = ctypes.c_void_p
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 EXPORTS = (
     "gbw_abi_version",
@@ -36,7 +36,7 @@ class Problem(ctypes.Structure):
         ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32), ("src_n", c_i32),
         ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
         ("snapshot_every", c_i32), ("checkpoint_every", c_i32), ("checkpoint_bf16", c_i32),
-        ("tile_rows", c_i32), ("shots_per_block", c_i32),
+        ("tile_rows", c_i32), ("shots_per_block", c_i32), ("march", c_i32),
     ]
 
 
@@ -87,6 +87,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
             raise RuntimeError(
                 f"geobrain_amd: HIP library not found at {path}; build it with "
                 "`python -m geobrain_amd.build` (there is no CPU fallback)")
+        # torch bundles its own libamdhip64.so.7; loading torch first makes the
+        # library bind to that same HIP runtime (and so share torch's device
+        # context and streams) instead of pulling in a second copy from /opt/rocm
+        import torch  # noqa: F401
         lib = ctypes.CDLL(path)
         lib.gbw_abi_version.restype = ctypes.c_int
         lib.gbw_abi_version.argtypes = []

@@ -1,0 +1,145 @@
+"""Autotuner for the propagator's launch choices.
+
+The library takes launch parameters per call (``tile_rows``: rows per
+workgroup tile, 16 or 32; ``shots_per_block``: shots one workgroup steps
+through, 0 = library default; ``march``: tiled or z-marching recompute in the
+gradient).  ``tune`` times short forward and gradient calls of the caller's
+problem (same grid, shots and geometry, ``nt_trial`` steps) for each candidate
+through the public API and keeps the fastest for each of the two calls.
+
+Results are cached per problem shape, keyed by (padded grid, shots, receivers,
+free surface, wavelet sharing, device name); the cache is process-local and,
+when ``GBW_TUNE_CACHE`` names a file, also persisted there as JSON.  Access is
+guarded by a lock.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+import threading
+from typing import Dict, Iterable, List, Optional, Tuple
+
+import torch
+
+from .physics.wave import kernels as K
+
+_lock = threading.Lock()
+_cache: Dict[str, Tuple[int, ...]] = {}
+
+
+def _key(nz, nx, nabc, n_shots, n_rcv, free_surface, shared) -> str:
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
+    return json.dumps([nz + 2 * nabc, nx + 2 * nabc, n_shots, n_rcv, bool(free_surface),
+                       bool(shared), dev])
+
+
+def _load_file():
+    path = os.environ.get("GBW_TUNE_CACHE")
+    if path and os.path.exists(path):
+        try:
+            with open(path) as fh:
+                for k, v in json.load(fh).items():
+                    _cache.setdefault(k, tuple(v))
+        except (OSError, ValueError):
+            pass
+
+
+def _save_file():
+    path = os.environ.get("GBW_TUNE_CACHE")
+    if path:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump({k: list(v) for k, v in _cache.items()}, fh)
+        os.replace(tmp, path)
+
+
+def default_candidates(n_shots: int) -> List[Tuple[int, int]]:
+    """(tile_rows, shots_per_block) pairs tried for each pass."""
+    cands = [(16, 0), (32, 0)]
+    for spb in (1, 2, 4, 8, 16):
+        if spb < n_shots:
+            cands.append((16, spb))
+    return cands
+
+
+def _timed(fn, repeats: int) -> float:
+    """Best of ``repeats`` timed calls after one untimed call, in ms."""
+    times = []
+    for _ in range(repeats + 1):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        pre = fn(None)
+        start.record()
+        fn(pre)
+        end.record()
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(end))
+    return min(times[1:])
+
+
+def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int, int]]] = None,
+         repeats: int = 2, verbose: bool = False) -> K.LaunchChoice:
+    """Pick the fastest launch choice for the problem described by ``args``
+    (the keyword arguments of ``acoustic_forward_kernel``; ``nt`` and ``src_v``
+    are cut to ``nt_trial`` steps for the trials).  The forward call (with
+    checkpoint stores) and the gradient call (recompute + adjoint, tiles or
+    z-marching recompute) are timed and chosen separately.  Pins the result
+    with ``kernels.set_launch_choice`` and returns it."""
+    nz, nx, nabc = args["nz"], args["nx"], args["nabc"]
+    src_v = args["src_v"]
+    key = _key(nz, nx, nabc, args["src_n"], args["rcv_n"], args["free_surface"], src_v.dim() == 1)
+    with _lock:
+        if not _cache:
+            _load_file()
+        hit = _cache.get(key)
+    if hit is not None and len(hit) == 5:
+        choice = K.LaunchChoice(*hit)
+        K.set_launch_choice(choice)
+        return choice
+    nt = min(nt_trial, args["nt"])
+    trial = dict(args, nt=nt, src_v=src_v[..., :nt].contiguous(),
+                 checkpoint_segments=max(1, nt // 20))
+    vp = args["vp"].detach().clone().requires_grad_(True)
+    trial["vp"] = vp
+    cands = list(candidates or default_candidates(args["src_n"]))
+
+    def log(msg):
+        if verbose:
+            print("  autotune " + msg, file=sys.stderr, flush=True)
+
+    def fwd(pre):
+        return K.acoustic_forward_kernel(**trial)["p"]
+
+    best_f, best_ft = cands[0], float("inf")
+    for tr, spb in cands:
+        K.set_launch_choice(K.LaunchChoice(tr, spb))
+        t = _timed(fwd, repeats)
+        log(f"forward tile_rows={tr} shots_per_block={spb}: {t:.2f} ms")
+        if t < best_ft:
+            best_f, best_ft = (tr, spb), t
+
+    def bwd(pre):
+        if pre is None:
+            vp.grad = None
+            return K.acoustic_forward_kernel(**trial)["p"].sum()
+        pre.backward()
+
+    best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
+    for march in (1, 2):
+        for tr, spb in cands:
+            if march == 2 and tr != cands[0][0]:
+                continue  # the marching recompute ignores tile_rows
+            K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb))
+            t = _timed(bwd, repeats)
+            log(f"gradient tile_rows={tr} shots_per_block={spb} march={march}: {t:.2f} ms")
+            if t < best_bt:
+                best_b, best_bt = (tr, spb, march), t
+    best = (best_f[0], best_f[1], best_b[2], best_b[0], best_b[1])
+    with _lock:
+        _cache[key] = best
+        _save_file()
+    choice = K.LaunchChoice(*best)
+    K.set_launch_choice(choice)
+    return choice

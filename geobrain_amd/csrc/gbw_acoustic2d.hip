@@ -44,8 +44,13 @@ namespace {
 
 constexpr float kC1 = 9.0f / 8.0f;
 constexpr float kC2 = (float)(-1.0 / 24.0);
-constexpr int TX = 64;        // tile width: one 64-lane wavefront per row
-constexpr int NTHREADS = 256; // four wavefronts per workgroup
+constexpr int TX = 128;       // tile width (columns): 32 float4 groups
+// Occupancy target (waves per SIMD) of the forward step kernel: 4 lets two
+// 512-thread workgroups share a CU (at most 128 VGPRs).  16-row tiles fit that
+// budget; 32-row tiles and the adjoint kernel spill under it and run faster
+// with one workgroup per CU (measured on MI355X, 64 shots, 390 x 1740).
+template <int TZ> constexpr int fwd_waves() { return TZ == 16 ? 4 : 1; }
+constexpr int NTHREADS = 512; // eight 64-lane wavefronts per workgroup
 
 thread_local std::string g_last_error;
 
@@ -147,7 +152,8 @@ __global__ void rmap_scatter_kernel(int* __restrict__ rhead, const int* __restri
 // ---------------------------------------------------------------------------
 struct FwdArgs {
   Dims d;
-  int S, spb, tiles_x, ntiles;   // shots, shots per workgroup, tiles along x, tiles
+  int S, spb, tiles_x, ntiles;   // shots, shots per workgroup, tiles along x, workgroup units
+  int seg_h;                     // z-marching kernels: rows per segment
   const float* P0; const float* VX0; const float* VZ0;  // input state fields (shot-major)
   float* P1; float* VX1; float* VZ1;                    // output state fields
   const float* coef;
@@ -164,15 +170,17 @@ struct FwdArgs {
 // ---------------------------------------------------------------------------
 // Staging geometry shared by the step kernels
 //
-// A workgroup (256 threads = four 64-lane wavefronts) owns a tile of TZ rows
-// x 64 columns.  Every staged row covers columns x0-4 .. x0+67 as 18 float4
-// groups, loaded with 16-byte global loads; in LDS a row holds 20 float4
-// (columns x0-8 .. x0+71) so that the 4-wide stencil reads of the outermost
+// A workgroup (512 threads = eight 64-lane wavefronts) owns a tile of TZ rows
+// x TX = 128 columns.  Every staged row covers columns x0-4 .. x0+TX+3 as
+// GW + 2 float4 groups, loaded with 16-byte global loads; in LDS a row holds
+// GW + 4 float4 (columns x0-8 .. x0+TX+7) so that the 4-wide stencil reads of
+// the outermost
 // groups stay inside the row (guard groups 0 and 19 are zero).  Work items are
 // (row, float4 group) pairs: 4 cells along x per thread and item.
 // ---------------------------------------------------------------------------
-constexpr int NG = 18;  // float4 groups staged per row
-constexpr int LG = 20;  // float4 groups per LDS row
+constexpr int GW = TX / 4;  // float4 groups per tile row
+constexpr int NG = GW + 2;  // float4 groups staged per row (one halo group each side)
+constexpr int LG = GW + 4;  // float4 groups per LDS row (plus a zero guard each side)
 
 __device__ __forceinline__ float4 f4zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 __device__ __forceinline__ float f4get(const float4& v, int e) {
@@ -203,14 +211,14 @@ __device__ __forceinline__ int xcd_tile() {
 // RECOMP = false: forward pass (receivers, energy); true: checkpoint-interval
 // recompute for the adjoint (stores the brackets the adjoint needs instead).
 template <int TZ, bool RECOMP>
-__global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
+__global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_waves<TZ>()))) void fwd_step_kernel(FwdArgs a) {
   constexpr int RV = TZ + 3;  // vx rows z0-1 .. z0+TZ+1
   constexpr int RZ = TZ + 6;  // vz rows z0-3 .. z0+TZ+2
   constexpr int RP = TZ + 3;  // p' rows z0-1 .. z0+TZ+1
   constexpr int NV = (RV * NG + NTHREADS - 1) / NTHREADS;
   constexpr int NZZ = (RZ * NG + NTHREADS - 1) / NTHREADS;
   constexpr int NP = (RP * NG + NTHREADS - 1) / NTHREADS;
-  constexpr int NT = TZ * 16 / NTHREADS;  // tile items (TZ rows x 16 groups)
+  constexpr int NT = TZ * GW / NTHREADS;  // tile items (TZ rows x GW groups)
   static_assert(TZ % 16 == 0, "TZ must be a multiple of 16");
   __shared__ float4 sVX[RV][LG];
   __shared__ float4 sVZ[RZ][LG];
@@ -274,16 +282,14 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
   // tile items
   unsigned t_idx[NT];
   float4 tc_omk2[NT], tc_omk3[NT], tc_a2[NT];
-  unsigned t_ok = 0, t_x = 0, t_z = 0, t_e = 0;
-  int4 t_rh[NT];
+  unsigned t_ok = 0, t_x = 0, t_z = 0, t_e = 0, t_rcv = 0;
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+    const int it = tid + n * NTHREADS, tr = it / GW, h = it % GW;
     const int gz = z0 + tr, xb = x0 + 4 * h;
     bool ok;
     t_idx[n] = group_index(d, gz, xb, ok);
     if (ok) t_ok |= 1u << n;
-    int4 rh = make_int4(-1, -1, -1, -1);
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int x = xb + e;
@@ -293,8 +299,7 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
       if (in && gz >= d.nabc && gz < d.nabc + d.nz && x >= d.nabc && x < d.nabc + d.nx)
         t_e |= 1u << (4 * n + e);
     }
-    if (!RECOMP && a.rec_p && ok && a.rrow[gz]) rh = reinterpret_cast<const int4*>(a.rhead)[t_idx[n]];
-    t_rh[n] = rh;
+    if (!RECOMP && a.rec_p && ok && a.rrow[gz]) t_rcv |= 1u << n;
     tc_omk2[n] = omk2[t_idx[n]];
     tc_omk3[n] = omk3[t_idx[n]];
     tc_a2[n] = a2[t_idx[n]];
@@ -375,8 +380,8 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
         }
         sP[r][g + 1] = out;
         if constexpr (RECOMP) {
-          // tile groups only: rows 1..TZ of the rectangle, float4 groups 1..16
-          if (r >= 1 && r <= TZ && g >= 1 && g <= 16 && ((p_ok >> n) & 1u))
+          // tile groups only: rows 1..TZ of the rectangle, float4 groups 1..GW
+          if (r >= 1 && r <= TZ && g >= 1 && g <= GW && ((p_ok >> n) & 1u))
             reinterpret_cast<float4*>(a.dbuf + so)[p_idx[n]] = dv;
         }
       }
@@ -397,7 +402,7 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       if ((t_ok >> n) & 1u) {
-        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+        const int it = tid + n * NTHREADS, tr = it / GW, h = it % GW;
         const int gz = z0 + tr;
         const float4 pa = sP[tr + 1][h + 1], pb = sP[tr + 1][h + 2], pcc = sP[tr + 1][h + 3];
         const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pcc.x, pcc.y, pcc.z, pcc.w};
@@ -436,10 +441,12 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
             reinterpret_cast<float4*>(a.gzbuf + so)[ti] = gzv;
           }
         } else {
+          const int4 rh = ((t_rcv >> n) & 1u) ? reinterpret_cast<const int4*>(a.rhead)[ti]
+                                             : make_int4(-1, -1, -1, -1);
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            for (int hh = e == 0 ? t_rh[n].x : (e == 1 ? t_rh[n].y : (e == 2 ? t_rh[n].z : t_rh[n].w));
-                 hh >= 0; hh = a.rnext[hh]) {
+            for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
+                 hh = a.rnext[hh]) {
               const long long o = ((long long)s * a.nt + a.it) * a.R + hh;
               a.rec_p[o] = f4get(pb, e);
               a.rec_vx[o] = f4get(vx, e);
@@ -463,7 +470,7 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
       float* eg = a.energy + (long long)blockIdx.y * 3 * inner;
 #pragma unroll
       for (int n = 0; n < NT; ++n) {
-        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+        const int it = tid + n * NTHREADS, tr = it / GW, h = it % GW;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           if ((t_e >> (4 * n + e)) & 1u) {
@@ -475,6 +482,238 @@ __global__ __launch_bounds__(NTHREADS) void fwd_step_kernel(FwdArgs a) {
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z-marching recompute step (checkpoint-interval recompute for the adjoint)
+//
+// A workgroup owns one 64-column strip, one z segment [zs, ze) and one shot
+// and walks down the segment TB rows at a time.  Staged rows live in LDS ring
+// buffers (RING rows, indexed by row mod RING), so a row loaded for one block
+// of rows is reused by the next instead of being re-read as a halo: every
+// field row is read once per segment (plus the 2 float4 x-halo groups), not
+// once per tile as in fwd_step_kernel.  p' runs two rows ahead of v' (v'(i)
+// needs p' up to row i+2).  Workgroups are ordered (segment, strip, shot) with
+// the shot fastest and a contiguous range per XCD, so the shots of a strip
+// run together and share the coefficient rows in that XCD's L2.
+// ---------------------------------------------------------------------------
+constexpr int MB = 16;    // rows per march iteration
+constexpr int RING = 32;  // LDS ring rows (live span is at most MB + 3)
+
+__global__ __launch_bounds__(NTHREADS) void recomp_march_kernel(FwdArgs a) {
+  __shared__ float4 rVX[RING][LG];
+  __shared__ float4 rVZ[RING][LG];
+  __shared__ float4 rP[RING][LG];
+  const Dims d = a.d;
+  const int tid = threadIdx.x;
+  const int logical = xcd_tile();
+  if (logical >= a.ntiles) return;
+  const int s = logical % a.S;
+  const int ts = logical / a.S;
+  const int x0 = (ts % a.tiles_x) * TX;
+  const int zs = (ts / a.tiles_x) * a.seg_h;
+  const int ze = min(d.NZ, zs + a.seg_h);
+  const long long so = (long long)s * d.plane;
+  const float4* __restrict__ VX0 = reinterpret_cast<const float4*>(a.VX0 + so);
+  const float4* __restrict__ VZ0 = reinterpret_cast<const float4*>(a.VZ0 + so);
+  const float4* __restrict__ P0 = reinterpret_cast<const float4*>(a.P0 + so);
+  const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
+  const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
+  const float4* __restrict__ omk3 = reinterpret_cast<const float4*>(a.coef + 3 * d.plane);
+  const float4* __restrict__ a2 = reinterpret_cast<const float4*>(a.coef + 4 * d.plane);
+  const int scell = a.src_cell[s];
+  const float supd = a.dt * a.wav[(long long)s * a.wav_stride + a.it];
+  for (int i = tid; i < RING; i += NTHREADS) rVX[i][0] = rVX[i][LG - 1] = f4zero();
+  auto slot = [&](int row) { return (row - zs + RING) & (RING - 1); };
+
+  // staged rows of one field: items (row, group) = (r0 + it / NG, it % NG)
+  struct Rows {
+    float4 v[2];
+    unsigned idx[2];
+    unsigned ok;
+  };
+  auto issue = [&](const float4* __restrict__ src, int r0, int n, Rows& R) {
+    R.ok = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int it = tid + k * NTHREADS;
+      bool ok;
+      R.idx[k] = group_index(d, r0 + it / NG, x0 - 4 + 4 * (it % NG), ok);
+      if (ok && it < n * NG) R.ok |= 1u << k;
+      R.v[k] = src[R.idx[k]];
+    }
+  };
+  auto commit = [&](float4 (*ring)[LG], int r0, int n, const Rows& R) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int it = tid + k * NTHREADS;
+      if (it < n * NG) ring[slot(r0 + it / NG)][it % NG + 1] = ((R.ok >> k) & 1u) ? R.v[k] : f4zero();
+    }
+  };
+  // coefficients of p' items
+  struct PCoef {
+    float4 omk[2], a1[2];
+  };
+  auto issue_pcoef = [&](const Rows& R, PCoef& C) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      C.omk[k] = omk1[R.idx[k]];
+      C.a1[k] = a1[R.idx[k]];
+    }
+  };
+  // p' for rows [r0, r0+n) from the vx/vz rings and the staged p rows
+  auto p_pass = [&](int r0, int n, const Rows& Pr, const PCoef& C) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int it = tid + k * NTHREADS;
+      if (it < n * NG) {
+        const int row = r0 + it / NG, g = it % NG;
+        const int xb = x0 - 4 + 4 * g;
+        const bool ok = (Pr.ok >> k) & 1u;
+        const int sr = slot(row);
+        const float4 xa = rVX[sr][g], xb4 = rVX[sr][g + 1], xc = rVX[sr][g + 2];
+        const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb4.x, xb4.y, xb4.z, xb4.w,
+                               xc.x, xc.y, xc.z, xc.w};
+        const float4 zm2 = rVZ[slot(row - 2)][g + 1], zm1 = rVZ[slot(row - 1)][g + 1];
+        const float4 z00 = rVZ[sr][g + 1], zp1 = rVZ[slot(row + 1)][g + 1];
+        const float4 p0 = ok ? Pr.v[k] : f4zero();
+        const bool zr = row >= d.f + 1 && row <= d.NZ - 3;
+        float4 out = p0, dv = f4zero();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int x = xb + e;
+          float pv = f4get(p0, e);
+          if (ok && zr && x >= 2 && x <= d.NX - 3) {
+            float s1 = vxs[4 + e] - vxs[3 + e];
+            s1 = s1 + f4get(z00, e);
+            s1 = s1 - f4get(zm1, e);
+            float s2 = vxs[5 + e] - vxs[2 + e];
+            s2 = s2 + f4get(zp1, e);
+            s2 = s2 - f4get(zm2, e);
+            const float t1 = kC1 * s1;
+            const float t2 = kC2 * s2;
+            const float br = t1 + t2;
+            const float u = f4get(C.omk[k], e) * pv;
+            pv = u - f4get(C.a1[k], e) * br;
+            f4set(dv, e, br);
+          }
+          if (ok && x < d.NX && (int)(Pr.idx[k] * 4u + e) == scell) pv = pv + supd;
+          f4set(out, e, pv);
+        }
+        rP[sr][g + 1] = out;
+        if (ok && row >= zs && row < ze && g >= 1 && g <= GW)
+          reinterpret_cast<float4*>(a.dbuf + so)[Pr.idx[k]] = dv;
+      }
+    }
+  };
+  // free surface: p'[f-1] = -p'[f+1] once row f+1 is computed and row f-1 is live
+  auto mirror = [&](int r0, int n) {
+    const int rf = d.f + 1;
+    if (d.fs && rf >= r0 && rf < r0 + n && d.f - 1 >= zs - 1) {
+      __syncthreads();
+      for (int g = tid; g < NG; g += NTHREADS) {
+        const float4 v = rP[slot(rf)][g + 1];
+        rP[slot(d.f - 1)][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+      }
+    }
+  };
+  // v' tile item: row v0 + tid / GW, group h = tid % GW (columns x0 + 4h ..)
+  const int tr = tid / GW, h = tid % GW;
+  struct VCoef {
+    float4 omk2, omk3, a2;
+    unsigned idx;
+    bool ok;
+  };
+  auto issue_vcoef = [&](int v0, VCoef& C) {
+    bool ok;
+    C.idx = group_index(d, v0 + tr, x0 + 4 * h, ok);
+    C.ok = ok && v0 + tr < ze;
+    C.omk2 = omk2[C.idx];
+    C.omk3 = omk3[C.idx];
+    C.a2 = a2[C.idx];
+  };
+  auto v_pass = [&](int v0, const VCoef& C) {
+    if (!C.ok) return;
+    const int gz = v0 + tr;
+    const float4 pa = rP[slot(gz)][h + 1], pb = rP[slot(gz)][h + 2], pcc = rP[slot(gz)][h + 3];
+    const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pcc.x, pcc.y, pcc.z, pcc.w};
+    const float4 pu = rP[slot(gz - 1)][h + 2], pd1 = rP[slot(gz + 1)][h + 2];
+    const float4 pd2 = rP[slot(gz + 2)][h + 2];
+    float4 vx = rVX[slot(gz)][h + 2];
+    float4 vz = rVZ[slot(gz)][h + 2];
+    float4 gxv = f4zero(), gzv = f4zero();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = x0 + 4 * h + e;
+      const bool in = x < d.NX;
+      if (in && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) {
+        const float d1 = ps[5 + e] - ps[4 + e];
+        const float d2 = ps[6 + e] - ps[3 + e];
+        const float br = kC1 * d1 + kC2 * d2;
+        const float u = f4get(C.omk2, e) * f4get(vx, e);
+        f4set(vx, e, u - f4get(C.a2, e) * br);
+        f4set(gxv, e, br);
+      }
+      if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) {
+        const float d1 = f4get(pd1, e) - f4get(pb, e);
+        const float d2 = f4get(pd2, e) - f4get(pu, e);
+        const float br = kC1 * d1 + kC2 * d2;
+        const float u = f4get(C.omk3, e) * f4get(vz, e);
+        f4set(vz, e, u - f4get(C.a2, e) * br);
+        f4set(gzv, e, br);
+      }
+    }
+    reinterpret_cast<float4*>(a.P1 + so)[C.idx] = pb;
+    reinterpret_cast<float4*>(a.VX1 + so)[C.idx] = vx;
+    if (!(d.fs && gz == d.f - 1)) reinterpret_cast<float4*>(a.VZ1 + so)[C.idx] = vz;
+    if (d.fs && gz == d.f) reinterpret_cast<float4*>(a.VZ1 + so)[C.idx - (d.PX >> 2)] = vz;
+    if (a.gxbuf) {
+      reinterpret_cast<float4*>(a.gxbuf + so)[C.idx] = gxv;
+      reinterpret_cast<float4*>(a.gzbuf + so)[C.idx] = gzv;
+    }
+  };
+
+  // prologue: p' rows zs-1 .. zs+1 (vx, p rows zs-1..zs+1; vz rows zs-3..zs+2)
+  Rows Rx, Rz, Rp;
+  PCoef Cp;
+  issue(VX0, zs - 1, 3, Rx);
+  issue(VZ0, zs - 3, 6, Rz);
+  issue(P0, zs - 1, 3, Rp);
+  issue_pcoef(Rp, Cp);
+  commit(rVX, zs - 1, 3, Rx);
+  commit(rVZ, zs - 3, 6, Rz);
+  __syncthreads();
+  p_pass(zs - 1, 3, Rp, Cp);
+  mirror(zs - 1, 3);
+  // first iteration's rows
+  issue(VX0, zs + 2, MB, Rx);
+  issue(VZ0, zs + 3, MB, Rz);
+  issue(P0, zs + 2, MB, Rp);
+  issue_pcoef(Rp, Cp);
+  VCoef Cv;
+  issue_vcoef(zs, Cv);
+  __syncthreads();
+  for (int v0 = zs; v0 < ze; v0 += MB) {
+    const int pr0 = v0 + 2;  // p' rows of this iteration: [v0+2, v0+2+MB)
+    commit(rVX, pr0, MB, Rx);
+    commit(rVZ, v0 + 3, MB, Rz);
+    __syncthreads();
+    p_pass(pr0, MB, Rp, Cp);
+    VCoef Cvn = Cv;
+    if (v0 + MB < ze) {  // prefetch the next iteration's rows
+      issue(VX0, pr0 + MB, MB, Rx);
+      issue(VZ0, v0 + 3 + MB, MB, Rz);
+      issue(P0, pr0 + MB, MB, Rp);
+      issue_pcoef(Rp, Cp);
+      issue_vcoef(v0 + MB, Cvn);
+    }
+    mirror(pr0, MB);
+    __syncthreads();
+    v_pass(v0, Cv);
+    Cv = Cvn;
+    __syncthreads();
   }
 }
 
@@ -524,7 +763,8 @@ struct AdjArgs {
   float* gw;    // [S][nt] or null
 };
 
-template <int TZ>
+// RHO: also accumulate the density gradient (needs the recompute's vx/vz brackets)
+template <int TZ, bool RHO>
 __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
   constexpr int RV = TZ + 3;  // lambda_vx rows z0-1 ..
   constexpr int RZ = TZ + 6;  // lambda_vz rows z0-3 ..
@@ -532,7 +772,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
   constexpr int NV = (RV * NG + NTHREADS - 1) / NTHREADS;
   constexpr int NZZ = (RZ * NG + NTHREADS - 1) / NTHREADS;
   constexpr int NP = (RP * NG + NTHREADS - 1) / NTHREADS;
-  constexpr int NT = TZ * 16 / NTHREADS;
+  constexpr int NT = TZ * GW / NTHREADS;
   static_assert(TZ % 16 == 0, "TZ must be a multiple of 16");
   __shared__ float4 sLX[RV][LG];  // lambda_vx as read
   __shared__ float4 sQX[RV][LG];  // a2 * lambda_vx on the vx region, else 0
@@ -609,11 +849,10 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
   }
   unsigned t_idx[NT];
   float4 tc_omk1[NT], tc_omk2[NT], tc_omk3[NT];
-  unsigned t_ok = 0, t_p = 0, t_x = 0, t_z = 0, t_vx = 0;
-  int4 t_rh[NT];
+  unsigned t_ok = 0, t_p = 0, t_x = 0, t_z = 0, t_vx = 0, t_rcv = 0;
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+    const int it = tid + n * NTHREADS, tr = it / GW, h = it % GW;
     const int gz = z0 + tr, xb = x0 + 4 * h;
     bool ok;
     t_idx[n] = group_index(d, gz, xb, ok);
@@ -628,9 +867,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
       if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) t_z |= bit;
       if (in && x <= d.NX - 2) t_vx |= bit;  // vx has NX-1 columns
     }
-    int4 rh = make_int4(-1, -1, -1, -1);
-    if (ok && a.it >= 1 && a.rrow[gz]) rh = reinterpret_cast<const int4*>(a.rhead)[t_idx[n]];
-    t_rh[n] = rh;
+    if (ok && a.it >= 1 && a.rrow[gz]) t_rcv |= 1u << n;
     tc_omk1[n] = omk1[t_idx[n]];
     tc_omk2[n] = omk2[t_idx[n]];
     tc_omk3[n] = omk3[t_idx[n]];
@@ -639,9 +876,13 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
   const int rfold = d.f - (z0 - 3);
   const bool fold = d.fs && rfold >= 0 && rfold < RZ;
 
-  float4 gacc[NT], gacc2[NT];
+  float4 gacc[NT], gacc2[RHO ? NT : 1];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) gacc[n] = gacc2[n] = f4zero();
+  for (int n = 0; n < NT; ++n) gacc[n] = f4zero();
+  if constexpr (RHO) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n) gacc2[n] = f4zero();
+  }
 
   float4 rlx[NV], rlz[NZZ], rlp[NP], rd[NT];
   auto load_v = [&](int s) {
@@ -784,7 +1025,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
       if ((t_ok >> n) & 1u) {
-        const int it = tid + n * NTHREADS, tr = it >> 4, h = it & 15;
+        const int it = tid + n * NTHREADS, tr = it / GW, h = it % GW;
         const int gz = z0 + tr;
         const unsigned ti = t_idx[n];
         const float4 lp4 = sLP[tr + 1][h + 2];
@@ -794,10 +1035,12 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
         float4 lx4 = sLX[tr + 1][h + 2];
         float4 lz4 = sLZ[tr + 3][h + 2];
         float4 gxv = f4zero(), gzv = f4zero();
-        if (a.gA2) {
+        if constexpr (RHO) {
           gxv = reinterpret_cast<const float4*>(a.gxbuf + so)[ti];
           gzv = reinterpret_cast<const float4*>(a.gzbuf + so)[ti];
         }
+        const int4 rh = ((t_rcv >> n) & 1u) ? reinterpret_cast<const int4*>(a.rhead)[ti]
+                                           : make_int4(-1, -1, -1, -1);
         float4 lpo = lp4;
         const int scell = a.src_cell[s];
 #pragma unroll
@@ -811,7 +1054,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
           if ((int)(ti * 4u + e) == scell && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
             a.gw[(long long)s * a.nt + a.it] = a.dt * lp;
           float lx = f4get(lx4, e), lz = f4get(lz4, e);
-          if (a.gA2) {
+          if constexpr (RHO) {
             float g2 = f4get(gacc2[n], e);
             if (t_z & bit) g2 = g2 - lz * f4get(gzv, e);
             if (t_x & bit) g2 = g2 - lx * f4get(gxv, e);
@@ -824,8 +1067,8 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
           if (gz <= d.NZ - 2)
             lz += kC1 * (f4get(rd1, e) - f4get(rb, e)) + kC2 * (f4get(rd2, e) - f4get(ru, e));
           float lpv = f4get(lpo, e);
-          for (int hh = e == 0 ? t_rh[n].x : (e == 1 ? t_rh[n].y : (e == 2 ? t_rh[n].z : t_rh[n].w));
-               hh >= 0; hh = a.rnext[hh]) {
+          for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
+               hh = a.rnext[hh]) {
             const long long o = ((long long)s * a.nt + (a.it - 1)) * a.R + hh;
             if (a.g_p) lpv += a.g_p[o];
             if (a.g_vx) lx += a.g_vx[o];
@@ -843,13 +1086,13 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
     __syncthreads();
   }
   float4* g1 = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
-  float4* g2 = a.gA2 ? reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane) : nullptr;
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
     if ((t_ok >> n) & 1u) {
       const float4 v = g1[t_idx[n]];
       g1[t_idx[n]] = make_float4(v.x + gacc[n].x, v.y + gacc[n].y, v.z + gacc[n].z, v.w + gacc[n].w);
-      if (g2) {
+      if constexpr (RHO) {
+        float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
         const float4 w = g2[t_idx[n]];
         g2[t_idx[n]] = make_float4(w.x + gacc2[n].x, w.y + gacc2[n].y, w.z + gacc2[n].z,
                                    w.w + gacc2[n].w);
@@ -1014,6 +1257,8 @@ struct Plan {
   Dims d;
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
   bool bf16;
+  bool march;       // z-marching recompute instead of tiles
+  int mseg, mseg_h; // z segments per strip and rows per segment for it
   size_t plane_bytes, slot_bytes, slot_scale_off;
   // byte offsets into the workspace
   size_t off_coef, off_A, off_B, off_LA, off_LB, off_D, off_GX, off_GZ, off_E, off_G1, off_G2,
@@ -1059,6 +1304,16 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.spb = spb;
   pl.groups = (pl.S + spb - 1) / spb;
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
+  if (p->march < 0 || p->march > 2) return fail(GBW_ERR_INVALID, "march must be 0, 1 or 2");
+  // z-marching recompute only on request: with coefficients reloaded per shot
+  // it measured slower than tiles on 128-column strips
+  pl.march = p->march == 2;
+  {
+    const int strips = pl.tiles_x;
+    int ns = std::max(1, std::min((2048 + strips * pl.S - 1) / (strips * pl.S), (d.NZ + 47) / 48));
+    pl.mseg_h = (d.NZ + ns - 1) / ns;
+    pl.mseg = (d.NZ + pl.mseg_h - 1) / pl.mseg_h;
+  }
   pl.plane_bytes = (size_t)d.plane * sizeof(float);
   const size_t state = 3 * (size_t)pl.S * pl.plane_bytes;
   pl.bf16 = p->checkpoint_bf16 != 0;
@@ -1237,7 +1492,10 @@ void launch_fwd(const Plan& pl, const FwdArgs& a, hipStream_t st) {
 template <int TZ>
 void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
-  adj_step_kernel<TZ><<<grid, NTHREADS, 0, st>>>(a);
+  if (a.gA2)
+    adj_step_kernel<TZ, true><<<grid, NTHREADS, 0, st>>>(a);
+  else
+    adj_step_kernel<TZ, false><<<grid, NTHREADS, 0, st>>>(a);
 }
 
 FwdArgs base_fwd_args(const Plan& pl, const gbw_acoustic2d_problem* p,
@@ -1426,7 +1684,16 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
       f.dbuf = D + (long long)(it - t0) * fstride;
       f.gxbuf = GX ? GX + (long long)(it - t0) * fstride : nullptr;
       f.gzbuf = GZ ? GZ + (long long)(it - t0) * fstride : nullptr;
-      if (pl.TZ == 32) launch_fwd<32, true>(pl, f, st); else launch_fwd<16, true>(pl, f, st);
+      if (pl.march) {
+        FwdArgs fm = f;
+        fm.ntiles = pl.tiles_x * pl.mseg * pl.S;
+        fm.seg_h = pl.mseg_h;
+        recomp_march_kernel<<<(unsigned)((fm.ntiles + 7) / 8 * 8), NTHREADS, 0, st>>>(fm);
+      } else if (pl.TZ == 32) {
+        launch_fwd<32, true>(pl, f, st);
+      } else {
+        launch_fwd<16, true>(pl, f, st);
+      }
       LAUNCH_CHECK();
       cur = next;
     }

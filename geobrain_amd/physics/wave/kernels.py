@@ -58,10 +58,17 @@ def auto_checkpoint_interval(nt: int) -> int:
 
 @dataclass
 class LaunchChoice:
-    """Launch parameters the library accepts (``tile_rows``, ``shots_per_block``);
-    zeros select the library's defaults.  The autotuner fills these."""
+    """Launch parameters the library accepts: ``tile_rows`` (16 or 32),
+    ``shots_per_block`` and ``march`` (gradient recompute: 1 tiles, 2 z-marching
+    strips); zeros select the library's defaults.  ``bwd_tile_rows`` and
+    ``bwd_shots_per_block`` apply to the gradient call (0 = same as the
+    forward's): the forward and adjoint kernels have different register
+    budgets, so their best tiles differ.  The autotuner fills these."""
     tile_rows: int = 0
     shots_per_block: int = 0
+    march: int = 0
+    bwd_tile_rows: int = 0
+    bwd_shots_per_block: int = 0
 
 
 _launch_override: Optional[LaunchChoice] = None
@@ -81,6 +88,7 @@ class _Spec:
     rcv_x: np.ndarray
     rcv_z: np.ndarray
     n_snap: int
+    bwd_launch: tuple = (0, 0)  # (tile_rows, shots_per_block) of the gradient call
     keep: Dict[str, object] = field(default_factory=dict)
 
 
@@ -161,6 +169,8 @@ class AcousticFunction(torch.autograd.Function):
             return None if g is None else g.to(torch.float32).contiguous()
 
         g_p, g_vx, g_vz = prep(g_p), prep(g_vx), prep(g_vz)
+        prob = _native.Problem.from_buffer_copy(prob)
+        prob.tile_rows, prob.shots_per_block = spec.bwd_launch
         want_vp, want_rho, want_w = ctx.needs_input_grad[:3]
         grad_vp = torch.empty_like(vp) if want_vp else None
         grad_rho = torch.empty_like(rho) if want_rho else None
@@ -295,7 +305,12 @@ def acoustic_forward_kernel(
     choice = _launch_override
     prob.tile_rows = choice.tile_rows if choice else 0
     prob.shots_per_block = choice.shots_per_block if choice else 0
-    spec = _Spec(prob=prob, src_x=sx, src_z=sz, rcv_x=rx, rcv_z=rz, n_snap=n_snap)
+    prob.march = choice.march if choice else 0
+    bwd_launch = (prob.tile_rows, prob.shots_per_block)
+    if choice and (choice.bwd_tile_rows or choice.bwd_shots_per_block):
+        bwd_launch = (choice.bwd_tile_rows, choice.bwd_shots_per_block)
+    spec = _Spec(prob=prob, src_x=sx, src_z=sz, rcv_x=rx, rcv_z=rz, n_snap=n_snap,
+                 bwd_launch=bwd_launch)
 
     f32 = torch.float32
     p, vx, vz, ep, evx, evz, snaps = AcousticFunction.apply(

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GBW_ABI_VERSION 1
+#define GBW_ABI_VERSION 2
 
 typedef enum {
   GBW_OK = 0,
@@ -64,8 +64,10 @@ typedef struct gbw_acoustic2d_problem {
   int32_t snapshot_every;    /* >0: copy shot-0 pressure after every chunk of this size */
   int32_t checkpoint_every;  /* k>0: keep the state every k steps for the gradient     */
   int32_t checkpoint_bf16;   /* 1: store checkpoints as bf16 with per-field fp32 scale */
-  int32_t tile_rows;         /* launch choice: rows per tile (8 or 16; 0 = default)    */
+  int32_t tile_rows;         /* launch choice: rows per tile (16 or 32; 0 = default)   */
   int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
+  int32_t march;             /* launch choice for the gradient's recompute: 0 auto,     */
+                             /* 1 tiles, 2 z-marching strips                             */
 } gbw_acoustic2d_problem;
 
 /* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model

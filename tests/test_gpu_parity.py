@@ -317,3 +317,95 @@ def test_multitile_multishot_bitwise_vs_oracle(spb):
     for k in ("p", "vx", "vz"):
         np.testing.assert_array_equal(np_(a[k]), np_(b[k]))
         np.testing.assert_array_equal(np_(a[k]), ref[k])
+
+
+@pytest.mark.parametrize("name", ["tiny1_grad", "tiny2_grad", "c2r_grad"])
+def test_marching_recompute_equals_tiled(name):
+    """The z-marching recompute stores the same brackets as the tiled one, so the
+    gradients agree bit for bit (and both match the reference)."""
+    print(f"[case] march recompute {name}", flush=True)
+    z, m = load(name)
+    want = ["vp"] + (["rho"] if "grad_rho" in z else []) + (["w"] if "grad_w" in z else [])
+    g = adjoint_sources(z, m) if "g_seed" in m else None
+    res = {}
+    for march in (1, 2):
+        out, ts = run(z, m, grad=want, launch=K.LaunchChoice(0, 0, march))
+        if g is not None:
+            loss = sum((_t(gk) * out[k]).sum() for gk, k in zip(g, ("p", "vx", "vz")))
+        else:
+            loss = (out["p"] ** 2).sum()
+        loss.backward()
+        res[march] = {k: np_(ts[k].grad) for k in want}
+    for k in want:
+        np.testing.assert_array_equal(res[1][k], res[2][k])
+        if "g_seed" in m:
+            assert rel_l2(res[2][k], z[{"vp": "grad_vp", "rho": "grad_rho", "w": "grad_w"}[k]]) <= TOL
+
+
+def test_marching_recompute_multishot_segments():
+    """7 shots on a 120 x 400 grid with the z range split into several marching
+    segments: gradient identical to the tiled recompute."""
+    print("[case] march recompute multishot", flush=True)
+    from geobrain_amd import synthetic
+    nz, nx, nt, S = 120, 400, 300, 7
+    vp = synthetic.grf_model(nz, nx, 5.0, seed=4)
+    w = synthetic.ricker_padded(15.0, 5e-4, nt)
+    damp = wave.bc_pml(nx, nz, 5.0, 5.0, 20, float(vp.max()), False).astype(np.float32)
+    z = dict(in_vp=vp, in_rho=synthetic.gardner(vp), in_damp=damp,
+             in_wavelet=np.stack([w] * S).astype(np.float32),
+             in_src_x=synthetic.surface_shots(nx, S), in_src_z=np.arange(S) % 4,
+             in_rcv_x=np.arange(nx), in_rcv_z=np.full(nx, 3))
+    res = {}
+    for fs in (False, True):
+        m = dict(nz=nz, nx=nx, nt=nt, dx=5.0, dt=5e-4, nabc=20, free_surface=fs, segments=6,
+                 snapshot_every=0)
+        for march in (1, 2):
+            out, ts = run(z, m, grad=("vp", "rho"), launch=K.LaunchChoice(0, 0, march))
+            (out["p"] ** 2).sum().backward()
+            res[(fs, march)] = (np_(ts["vp"].grad), np_(ts["rho"].grad))
+        for i in range(2):
+            np.testing.assert_array_equal(res[(fs, 1)][i], res[(fs, 2)][i])
+
+
+@pytest.mark.parametrize("march", [1, 2])
+def test_free_surface_receivers_on_mirror_rows(march):
+    """Free surface with receivers on the mirrored rows: unpadded rcv_z = -1 is
+    padded row f-1, which the reference's indexing accepts; its vz sample is
+    VZ[f-1] after the mirror VZ[f-1] = VZ[f] (acoustic_2d.py:267-274).  Source
+    and receivers span tile boundaries.  Seismograms bit-identical to the FP32
+    oracle; gradients (vp, rho, wavelet) from random adjoint sources on all
+    three seismograms within TOL of the FP32 and FP64 oracles."""
+    print(f"[case] free-surface receivers on row f-1, march={march}", flush=True)
+    from geobrain_amd import synthetic
+    nz, nx, nt, S, nabc = 60, 300, 240, 3, 10
+    vp = synthetic.grf_model(nz, nx, 5.0, seed=5)
+    w = synthetic.ricker_padded(15.0, 5e-4, nt)
+    damp = wave.bc_pml(nx, nz, 5.0, 5.0, nabc, float(vp.max()), False).astype(np.float32)
+    rx = np.arange(0, nx, 2)
+    rz = np.fromiter(([-1, 0, 1, 2, 6][i % 5] for i in range(len(rx))), dtype=np.int64)
+    z = dict(in_vp=vp, in_rho=synthetic.gardner(vp), in_damp=damp,
+             in_wavelet=np.stack([w * (1 + 0.2 * s) for s in range(S)]).astype(np.float32),
+             in_src_x=np.array([5, 150, 290]), in_src_z=np.array([-1, 1, 3]),
+             in_rcv_x=rx, in_rcv_z=rz)
+    m = dict(nz=nz, nx=nx, nt=nt, dx=5.0, dt=5e-4, nabc=nabc, free_surface=True, segments=4,
+             snapshot_every=0)
+    args = oracle_args(z, m)
+    ref = oracle.forward(**args, chunk=chunk_len(nt, 4), precision="f32")
+    out, ts = run(z, m, grad=("vp", "rho", "w"), launch=K.LaunchChoice(0, 0, march))
+    for k in ("p", "vx", "vz"):
+        bad = np.argwhere(np_(out[k]) != ref[k])
+        print(f"  {k}: {len(bad)} samples differ from the FP32 oracle"
+              + (f", first at (shot, step, receiver) {tuple(bad[0])}" if len(bad) else ""))
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_array_equal(np_(out[k]), ref[k])
+    g = tuple(np.random.default_rng(11).standard_normal((S, nt, len(rx))).astype(np.float32)
+              for _ in range(3))
+    loss = sum((_t(gk) * out[k]).sum() for gk, k in zip(g, ("p", "vx", "vz")))
+    loss.backward()
+    for prec in ("f32", "f64"):
+        gv, gr, gw = oracle.gradient(**args, g_p=g[0], g_vx=g[1], g_vz=g[2],
+                                     chunk=chunk_len(nt, 4), precision=prec)
+        for key, want in (("vp", gv), ("rho", gr), ("w", gw)):
+            e = rel_l2(np_(ts[key].grad), want)
+            print(f"  grad {key}: rel-L2 vs {prec} oracle {e:.2e}")
+            assert e <= TOL, (key, prec)
