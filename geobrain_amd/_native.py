@@ -17,7 +17,7 @@ c_i32 = ctypes.c_int32
 c_f32 = ctypes.c_float
 c_vp = ctypes.c_void_p
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 EXPORTS = (
     "gbw_abi_version",
@@ -37,6 +37,7 @@ class Problem(ctypes.Structure):
         ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
         ("snapshot_every", c_i32), ("checkpoint_every", c_i32), ("checkpoint_bf16", c_i32),
         ("tile_rows", c_i32), ("shots_per_block", c_i32), ("march", c_i32),
+        ("steps_per_launch", c_i32),
     ]
 
 

@@ -64,6 +64,14 @@ def default_candidates(n_shots: int) -> List[Tuple[int, int]]:
     return cands
 
 
+def forward_candidates(n_shots: int) -> List[Tuple[int, int, int]]:
+    """(tile_rows, shots_per_block, steps_per_launch) triples tried for the
+    forward call (two-step launches always tile 16 rows)."""
+    cands = [(tr, spb, 1) for tr, spb in default_candidates(n_shots)]
+    cands += [(16, spb, 2) for spb in (0, 2, 4, 8, 16, 32) if spb < n_shots or spb == 0]
+    return cands
+
+
 def _timed(fn, repeats: int) -> float:
     """Best of ``repeats`` timed calls after one untimed call, in ms."""
     times = []
@@ -94,7 +102,7 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
         if not _cache:
             _load_file()
         hit = _cache.get(key)
-    if hit is not None and len(hit) == 5:
+    if hit is not None and len(hit) == 6:
         choice = K.LaunchChoice(*hit)
         K.set_launch_choice(choice)
         return choice
@@ -112,13 +120,13 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     def fwd(pre):
         return K.acoustic_forward_kernel(**trial)["p"]
 
-    best_f, best_ft = cands[0], float("inf")
-    for tr, spb in cands:
-        K.set_launch_choice(K.LaunchChoice(tr, spb))
+    best_f, best_ft = (16, 0, 2), float("inf")
+    for tr, spb, spl in forward_candidates(args["src_n"]):
+        K.set_launch_choice(K.LaunchChoice(tr, spb, steps_per_launch=spl))
         t = _timed(fwd, repeats)
-        log(f"forward tile_rows={tr} shots_per_block={spb}: {t:.2f} ms")
+        log(f"forward tile_rows={tr} shots_per_block={spb} steps_per_launch={spl}: {t:.2f} ms")
         if t < best_ft:
-            best_f, best_ft = (tr, spb), t
+            best_f, best_ft = (tr, spb, spl), t
 
     def bwd(pre):
         if pre is None:
@@ -126,17 +134,19 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
             return K.acoustic_forward_kernel(**trial)["p"].sum()
         pre.backward()
 
+    # the recompute inside the gradient call follows the forward's
+    # steps_per_launch; the adjoint kernel takes its own tiles
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
     for march in (1, 2):
         for tr, spb in cands:
             if march == 2 and tr != cands[0][0]:
                 continue  # the marching recompute ignores tile_rows
-            K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb))
+            K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb, best_f[2]))
             t = _timed(bwd, repeats)
             log(f"gradient tile_rows={tr} shots_per_block={spb} march={march}: {t:.2f} ms")
             if t < best_bt:
                 best_b, best_bt = (tr, spb, march), t
-    best = (best_f[0], best_f[1], best_b[2], best_b[0], best_b[1])
+    best = (best_f[0], best_f[1], best_b[2], best_b[0], best_b[1], best_f[2])
     with _lock:
         _cache[key] = best
         _save_file()

@@ -165,6 +165,7 @@ struct FwdArgs {
   float* energy;                 // [group][3][nz][nx] or null
   float* dbuf;                   // [S][plane] bracket D of this step, or null
   float* gxbuf; float* gzbuf;    // [S][plane] vx / vz brackets of this step, or null
+  long long bstride;             // two-step kernel: offset of step it+1's brackets
 };
 
 // ---------------------------------------------------------------------------
@@ -299,7 +300,10 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
       if (in && gz >= d.nabc && gz < d.nabc + d.nz && x >= d.nabc && x < d.nabc + d.nx)
         t_e |= 1u << (4 * n + e);
     }
-    if (!RECOMP && a.rec_p && ok && a.rrow[gz]) t_rcv |= 1u << n;
+    // receivers on this row, or (free surface, row f) vz samples of row f-1,
+    // which the mirror vz[f-1] = vz[f] fills from this row
+    if (!RECOMP && a.rec_p && ok && (a.rrow[gz] || (d.fs && gz == d.f && a.rrow[gz - 1])))
+      t_rcv |= 1u << n;
     tc_omk2[n] = omk2[t_idx[n]];
     tc_omk3[n] = omk3[t_idx[n]];
     tc_a2[n] = a2[t_idx[n]];
@@ -441,8 +445,9 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
             reinterpret_cast<float4*>(a.gzbuf + so)[ti] = gzv;
           }
         } else {
-          const int4 rh = ((t_rcv >> n) & 1u) ? reinterpret_cast<const int4*>(a.rhead)[ti]
-                                             : make_int4(-1, -1, -1, -1);
+          const bool rcv = (t_rcv >> n) & 1u;
+          const int4 rh = rcv ? reinterpret_cast<const int4*>(a.rhead)[ti] : make_int4(-1, -1, -1, -1);
+          const bool mirror_row = d.fs && gz == d.f - 1;  // its vz sample comes from row f
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
@@ -450,8 +455,16 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
               const long long o = ((long long)s * a.nt + a.it) * a.R + hh;
               a.rec_p[o] = f4get(pb, e);
               a.rec_vx[o] = f4get(vx, e);
-              a.rec_vz[o] = f4get(vz, e);
+              if (!mirror_row) a.rec_vz[o] = f4get(vz, e);
             }
+          }
+          if (rcv && d.fs && gz == d.f) {
+            const int4 ru = reinterpret_cast<const int4*>(a.rhead)[ti - (d.PX >> 2)];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              for (int hh = e == 0 ? ru.x : (e == 1 ? ru.y : (e == 2 ? ru.z : ru.w)); hh >= 0;
+                   hh = a.rnext[hh])
+                a.rec_vz[((long long)s * a.nt + a.it) * a.R + hh] = f4get(vz, e);
           }
           e_p[n] = make_float4(e_p[n].x + pb.x * pb.x, e_p[n].y + pb.y * pb.y,
                                e_p[n].z + pb.z * pb.z, e_p[n].w + pb.w * pb.w);
@@ -478,6 +491,436 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
             eg[o] = eg[o] + f4get(e_p[n], e);
             eg[inner + o] = eg[inner + o] + f4get(e_vx[n], e);
             eg[2 * inner + o] = eg[2 * inner + o] + f4get(e_vz[n], e);
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-step forward kernel (temporal blocking)
+//
+// One launch advances every shot by steps it and it+1, so the state is read
+// and written once per two steps.  A workgroup stages its TZ x TX tile with a
+// 6-cell halo (rows z0-6 .. z0+TZ+5; float4 groups -2 .. GW+1, i.e. columns
+// x0-8 .. x0+TX+7) once per shot, then computes in LDS, relative to the tile
+// (rows, float4 groups; 0 = first tile row / group):
+//   p'   rows -4 .. TZ+4, groups -1 .. GW+1   (reads vx groups -2 .. GW+2, vz rows -6 .. TZ+5)
+//   v'   rows -3 .. TZ+2, groups -1 .. GW
+//   p''  rows -1 .. TZ+1, groups -1 .. GW
+//   v''  the tile
+// which are the halos step it+1 of the tile needs (the stencil reaches 2 cells
+// back and 1 forward from p to v, 1 back and 2 forward from v to p).  Cells
+// outside those regions hold stale values that nothing reads.  Each cell
+// belongs to one thread ("slot") in every phase, so fields are updated in
+// place; the free-surface mirrors run between phases.  Per cell, the
+// arithmetic is the one-step kernel's, in the same order.
+// ---------------------------------------------------------------------------
+constexpr int W2 = GW + 4;   // staged float4 groups per row (-2 .. GW+1)
+constexpr int MAX_SPB = 64;  // shots per workgroup of the two-step kernel
+#ifndef FWD2_THREADS
+#define FWD2_THREADS 1024     // one staged group per thread for 16-row tiles
+#endif
+constexpr int LW2 = GW + 6;  // LDS row: staged groups plus a zero guard each side
+
+// per-slot flag word: bits 0-6 slot flags, then 4-bit per-cell masks
+constexpr unsigned F_OK = 1u, F_PV = 2u, F_P1 = 4u, F_V1 = 8u, F_P2 = 16u, F_T = 32u, F_RCV = 64u;
+constexpr unsigned F_PFULL = 128u, F_VFULL = 1u << 28;  // all 4 cells in the P / X and Z regions
+constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
+
+template <int TZ, int NTH, bool RECOMP>
+__global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
+  constexpr int SR = TZ + 12;                    // staged rows (-6 .. TZ+5)
+  constexpr int NS = (SR * W2 + NTH - 1) / NTH;  // slots per thread
+  // LDS row index = tile row + 6; LDS column of staged group g (0 = group -2) is g + 1
+  __shared__ float4 sP[SR][LW2];
+  __shared__ float4 sVX[SR][LW2];
+  __shared__ float4 sVZ[SR][LW2];
+  __shared__ float4 sE[RECOMP ? 1 : 3][RECOMP ? 1 : TZ][RECOMP ? 1 : GW];  // energy sums
+  const Dims d = a.d;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile();
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.tiles_x) * TX;
+  const int z0 = (tile / a.tiles_x) * TZ;
+  const int s_begin = blockIdx.y * a.spb;
+  const int s_end = min(a.S, s_begin + a.spb);
+  if (s_begin >= s_end) return;  // uniform per workgroup
+  const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
+  const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
+  const float4* __restrict__ omk3 = reinterpret_cast<const float4*>(a.coef + 3 * d.plane);
+  const float4* __restrict__ a2 = reinterpret_cast<const float4*>(a.coef + 4 * d.plane);
+
+  for (int i = tid; i < SR; i += NTH) {
+    sP[i][0] = sP[i][LW2 - 1] = f4zero();
+    sVX[i][0] = sVX[i][LW2 - 1] = f4zero();
+    sVZ[i][0] = sVZ[i][LW2 - 1] = f4zero();
+  }
+
+  // --- per-slot constants (registers across the shot loop) ---
+  unsigned idx[NS], fl[NS];
+  float4 c_omk1[NS], c_a1[NS], c_omk2[NS], c_omk3[NS], c_a2[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const int item = tid + n * NTH;
+    const int r = item / W2, g = item % W2;
+    const int gz = z0 - 6 + r, xb = x0 - 8 + 4 * g;
+    bool ok;
+    idx[n] = group_index(d, gz, xb, ok);
+    const bool valid = item < SR * W2;
+    ok = ok && valid;
+    const bool p1 = valid && r >= 2 && r <= TZ + 10 && g >= 1;
+    const bool v1 = valid && r >= 3 && r <= TZ + 8 && g >= 1 && g <= GW + 2;
+    const bool p2 = valid && r >= 5 && r <= TZ + 7 && g >= 1 && g <= GW + 2;
+    const bool tt = valid && r >= 6 && r <= TZ + 5 && g >= 2 && g <= GW + 1;
+    unsigned f = 0;
+    if (ok) f |= F_OK;
+    if (ok && r >= 2 && r <= TZ + 10) f |= F_PV;
+    if (p1) f |= F_P1;
+    if (v1) f |= F_V1;
+    if (p2) f |= F_P2;
+    if (tt) f |= F_T;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      const bool in = ok && x < d.NX;
+      if (in) f |= 1u << (M_PIN + e);
+      if (in && gz >= d.f + 1 && gz <= d.NZ - 3 && x >= 2 && x <= d.NX - 3) f |= 1u << (M_PREG + e);
+      if (in && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) f |= 1u << (M_XREG + e);
+      if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) f |= 1u << (M_ZREG + e);
+      if (tt && in && gz >= d.nabc && gz < d.nabc + d.nz && x >= d.nabc && x < d.nabc + d.nx)
+        f |= 1u << (M_ENRG + e);
+    }
+    if (!RECOMP && a.rec_p && tt && ok && (a.rrow[gz] || (d.fs && gz == d.f && a.rrow[gz - 1])))
+      f |= F_RCV;
+    if (((f >> M_PREG) & 15u) == 15u) f |= F_PFULL;
+    if (((f >> M_XREG) & 15u) == 15u && ((f >> M_ZREG) & 15u) == 15u) f |= F_VFULL;
+    fl[n] = f;
+    c_omk1[n] = p1 ? omk1[idx[n]] : f4zero();
+    c_a1[n] = p1 ? a1[idx[n]] : f4zero();
+    c_omk2[n] = v1 ? omk2[idx[n]] : f4zero();
+    c_omk3[n] = v1 ? omk3[idx[n]] : f4zero();
+    c_a2[n] = v1 ? a2[idx[n]] : f4zero();
+    if constexpr (!RECOMP) {
+      if (tt) sE[0][r - 6][g - 2] = sE[1][r - 6][g - 2] = sE[2][r - 6][g - 2] = f4zero();
+    }
+  }
+
+  // p and vx are needed on the p' rows only; the other slots load the tile's
+  // first group (one shared line per wavefront) instead of branching around
+  // the load, which keeps the three prefetch loads back to back
+  unsigned pv_idx[NS];
+  const unsigned idx0 = (unsigned)((min(z0, d.NZ - 1) * d.PX + x0) >> 2);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) pv_idx[n] = (fl[n] & F_PV) ? idx[n] : idx0;
+  float4 rp[NS], rvx[NS], rvz[NS];
+  auto load = [&](int s) {
+    const long long so = (long long)s * d.plane;
+    const float4* __restrict__ P = reinterpret_cast<const float4*>(a.P0 + so);
+    const float4* __restrict__ VX = reinterpret_cast<const float4*>(a.VX0 + so);
+    const float4* __restrict__ VZ = reinterpret_cast<const float4*>(a.VZ0 + so);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      rp[n] = P[pv_idx[n]];
+      rvx[n] = VX[pv_idx[n]];
+      rvz[n] = VZ[idx[n]];
+    }
+  };
+  load(s_begin);
+
+  // p-update of the slot's group (acoustic_2d.py:213-237); returns the new
+  // group, the bracket D in dv
+  // p-update of the slot's group (acoustic_2d.py:213-237): computed for all
+  // four cells, then cells outside the P-region keep p (a select, not a
+  // branch; whole groups inside the region skip the selects).  The source
+  // term is added when the shot's source lies in the staged region.
+  auto p_update = [&](int n, int r, int c, bool src_here, int scell, float supd,
+                      float4& dv) -> float4 {
+    const float4 xa = sVX[r][c - 1], xb = sVX[r][c], xc = sVX[r][c + 1];
+    const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
+    const float4 zm2 = sVZ[r - 2][c], zm1 = sVZ[r - 1][c];
+    const float4 z00 = sVZ[r][c], zp1 = sVZ[r + 1][c];
+    const float4 p0 = sP[r][c];
+    float4 out, brv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float s1 = vxs[4 + e] - vxs[3 + e];
+      s1 = s1 + f4get(z00, e);
+      s1 = s1 - f4get(zm1, e);
+      float s2 = vxs[5 + e] - vxs[2 + e];
+      s2 = s2 + f4get(zp1, e);
+      s2 = s2 - f4get(zm2, e);
+      const float t1 = kC1 * s1;
+      const float t2 = kC2 * s2;
+      const float br = t1 + t2;
+      const float u = f4get(c_omk1[n], e) * f4get(p0, e);
+      f4set(out, e, u - f4get(c_a1[n], e) * br);
+      f4set(brv, e, br);
+    }
+    if (fl[n] & F_PFULL) {
+      dv = brv;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool m = (fl[n] >> (M_PREG + e)) & 1u;
+        f4set(out, e, m ? f4get(out, e) : f4get(p0, e));
+        f4set(dv, e, m ? f4get(brv, e) : 0.f);
+      }
+    }
+    if (src_here) {
+      const unsigned base = idx[n] * 4u;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (((fl[n] >> (M_PIN + e)) & 1u) && (int)(base + e) == scell)
+          f4set(out, e, f4get(out, e) + supd);
+    }
+    return out;
+  };
+  // v-update of the slot's group (acoustic_2d.py:243-265) from the p in sP;
+  // cells outside the X / Z regions keep their value
+  auto v_update = [&](int n, int r, int c, float4& vx, float4& vz, float4& gxv, float4& gzv) {
+    const float4 pa = sP[r][c - 1], pb = sP[r][c], pcc = sP[r][c + 1];
+    const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pcc.x, pcc.y, pcc.z, pcc.w};
+    const float4 pu = sP[r - 1][c], pd1 = sP[r + 1][c], pd2 = sP[r + 2][c];
+    const float4 vx0 = sVX[r][c], vz0 = sVZ[r][c];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      {
+        const float d1 = ps[5 + e] - ps[4 + e];
+        const float d2 = ps[6 + e] - ps[3 + e];
+        const float br = kC1 * d1 + kC2 * d2;
+        const float u = f4get(c_omk2[n], e) * f4get(vx0, e);
+        f4set(vx, e, u - f4get(c_a2[n], e) * br);
+        f4set(gxv, e, br);
+      }
+      {
+        const float d1 = f4get(pd1, e) - f4get(pb, e);
+        const float d2 = f4get(pd2, e) - f4get(pu, e);
+        const float br = kC1 * d1 + kC2 * d2;
+        const float u = f4get(c_omk3[n], e) * f4get(vz0, e);
+        f4set(vz, e, u - f4get(c_a2[n], e) * br);
+        f4set(gzv, e, br);
+      }
+    }
+    if (!(fl[n] & F_VFULL)) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool mx = (fl[n] >> (M_XREG + e)) & 1u;
+        const bool mz = (fl[n] >> (M_ZREG + e)) & 1u;
+        f4set(vx, e, mx ? f4get(vx, e) : f4get(vx0, e));
+        f4set(gxv, e, mx ? f4get(gxv, e) : 0.f);
+        f4set(vz, e, mz ? f4get(vz, e) : f4get(vz0, e));
+        f4set(gzv, e, mz ? f4get(gzv, e) : 0.f);
+      }
+    }
+  };
+  auto add_sq = [](float4& acc, const float4& v) {
+    acc = make_float4(acc.x + v.x * v.x, acc.y + v.y * v.y, acc.z + v.z * v.z, acc.w + v.w * v.w);
+  };
+  // receivers of a tile slot: p, vx, vz of step `step`; for the free-surface
+  // row f the slot also writes the vz samples of row f-1 (the mirror's source)
+  auto record = [&](int n, int s, int step, int gz, const float4& pv, const float4& vx, const float4& vz) {
+    const unsigned ti = idx[n];
+    const int4 rh = reinterpret_cast<const int4*>(a.rhead)[ti];
+    const bool mirror_row = d.fs && gz == d.f - 1;
+    const long long ob = ((long long)s * a.nt + step) * a.R;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
+           hh = a.rnext[hh]) {
+        a.rec_p[ob + hh] = f4get(pv, e);
+        a.rec_vx[ob + hh] = f4get(vx, e);
+        if (!mirror_row) a.rec_vz[ob + hh] = f4get(vz, e);
+      }
+    }
+    if (d.fs && gz == d.f) {
+      const int4 ru = reinterpret_cast<const int4*>(a.rhead)[ti - (d.PX >> 2)];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        for (int hh = e == 0 ? ru.x : (e == 1 ? ru.y : (e == 2 ? ru.z : ru.w)); hh >= 0;
+             hh = a.rnext[hh])
+          a.rec_vz[ob + hh] = f4get(vz, e);
+    }
+  };
+
+  const int rf = d.f - 1 - (z0 - 6);  // LDS row of the free-surface mirror row f-1
+  // per-shot source cell and source terms of this workgroup's shots, staged in
+  // LDS up front: reading them in the shot loop then never waits on the
+  // in-flight state prefetch
+  __shared__ int sSrc[MAX_SPB];
+  __shared__ float sW[2][MAX_SPB];
+  for (int i = tid; i < s_end - s_begin; i += NTH) {
+    const float* wv = a.wav + (long long)(s_begin + i) * a.wav_stride + a.it;
+    sSrc[i] = a.src_cell[s_begin + i];
+    sW[0][i] = a.dt * wv[0];
+    sW[1][i] = a.dt * wv[1];
+  }
+  __syncthreads();
+  for (int s = s_begin; s < s_end; ++s) {
+    const long long so = (long long)s * d.plane;
+    const int scell = sSrc[s - s_begin];
+    const float supd0 = sW[0][s - s_begin];
+    const float supd1 = sW[1][s - s_begin];
+    const int sz = scell / d.PX, sx = scell - sz * d.PX;
+    const bool src_here = sz >= z0 - 6 && sz < z0 + TZ + 6 && sx >= x0 - 8 && sx < x0 + TX + 8;
+    // stage this shot
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const int item = tid + n * NTH;
+      if (item < SR * W2) {
+        const int r = item / W2, c = item % W2 + 1;
+        const bool ok = fl[n] & F_OK;
+        if (fl[n] & F_PV) {
+          sP[r][c] = rp[n];
+          sVX[r][c] = rvx[n];
+        } else {
+          sP[r][c] = f4zero();
+          sVX[r][c] = f4zero();
+        }
+        sVZ[r][c] = ok ? rvz[n] : f4zero();
+      }
+    }
+    if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
+    __syncthreads();
+    // step it: p'
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if (fl[n] & F_P1) {
+        const int item = tid + n * NTH;
+        const int r = item / W2, c = item % W2 + 1;
+        float4 dv;
+        const float4 pn = p_update(n, r, c, src_here, scell, supd0, dv);
+        sP[r][c] = pn;  // only this thread reads this group in this phase
+        if constexpr (RECOMP) {
+          if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK))
+            reinterpret_cast<float4*>(a.dbuf + so)[idx[n]] = dv;
+        }
+      }
+    }
+    __syncthreads();
+    if (d.fs && rf >= 2 && rf + 2 <= TZ + 10) {  // p'[f-1] = -p'[f+1]
+      for (int g = tid; g < W2; g += NTH) {
+        const float4 v = sP[rf + 2][g + 1];
+        sP[rf][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+      }
+      __syncthreads();
+    }
+    // step it: v'
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if (fl[n] & F_V1) {
+        const int item = tid + n * NTH;
+        const int r = item / W2, c = item % W2 + 1;
+        float4 vx, vz, gxv, gzv;
+        v_update(n, r, c, vx, vz, gxv, gzv);
+        // in place: the v-update of a group reads no other group's v
+        sVX[r][c] = vx;
+        sVZ[r][c] = vz;
+        if constexpr (RECOMP) {
+          if (a.gxbuf && (fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
+            reinterpret_cast<float4*>(a.gxbuf + so)[idx[n]] = gxv;
+            reinterpret_cast<float4*>(a.gzbuf + so)[idx[n]] = gzv;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (d.fs && rf >= 3 && rf + 1 <= TZ + 8) {  // vz'[f-1] = vz'[f]
+      for (int g = tid; g < W2; g += NTH) sVZ[rf][g + 1] = sVZ[rf + 1][g + 1];
+      __syncthreads();
+    }
+    // step it: receivers and energy of the tile
+    if constexpr (!RECOMP) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
+          const int item = tid + n * NTH;
+          const int r = item / W2, g = item % W2;
+          const float4 pv = sP[r][g + 1], vx = sVX[r][g + 1], vz = sVZ[r][g + 1];
+          if (fl[n] & F_RCV) record(n, s, a.it, z0 - 6 + r, pv, vx, vz);
+          if (a.energy) {
+            add_sq(sE[0][r - 6][g - 2], pv);
+            add_sq(sE[1][r - 6][g - 2], vx);
+            add_sq(sE[2][r - 6][g - 2], vz);
+          }
+        }
+      }
+    }
+    // step it+1: p'' (reads only v' and this slot's own p')
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if (fl[n] & F_P2) {
+        const int item = tid + n * NTH;
+        const int r = item / W2, c = item % W2 + 1;
+        float4 dv;
+        const float4 pn = p_update(n, r, c, src_here, scell, supd1, dv);
+        sP[r][c] = pn;
+        if constexpr (RECOMP) {
+          if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK))
+            reinterpret_cast<float4*>(a.dbuf + a.bstride + so)[idx[n]] = dv;
+        }
+      }
+    }
+    __syncthreads();
+    if (d.fs && rf >= 5 && rf + 2 <= TZ + 7) {  // p''[f-1] = -p''[f+1]
+      for (int g = tid; g < W2; g += NTH) {
+        const float4 v = sP[rf + 2][g + 1];
+        sP[rf][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+      }
+      __syncthreads();
+    }
+    // step it+1: v'' on the tile, store the state
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
+        const int item = tid + n * NTH;
+        const int r = item / W2, c = item % W2 + 1;
+        const int gz = z0 - 6 + r;
+        float4 vx, vz, gxv, gzv;
+        v_update(n, r, c, vx, vz, gxv, gzv);
+        const float4 pv = sP[r][c];
+        const unsigned ti = idx[n];
+        reinterpret_cast<float4*>(a.P1 + so)[ti] = pv;
+        reinterpret_cast<float4*>(a.VX1 + so)[ti] = vx;
+        if (!(d.fs && gz == d.f - 1)) reinterpret_cast<float4*>(a.VZ1 + so)[ti] = vz;
+        if (d.fs && gz == d.f) reinterpret_cast<float4*>(a.VZ1 + so)[ti - (d.PX >> 2)] = vz;
+        if constexpr (RECOMP) {
+          if (a.gxbuf) {
+            reinterpret_cast<float4*>(a.gxbuf + a.bstride + so)[ti] = gxv;
+            reinterpret_cast<float4*>(a.gzbuf + a.bstride + so)[ti] = gzv;
+          }
+        } else {
+          if (fl[n] & F_RCV) record(n, s, a.it + 1, gz, pv, vx, vz);
+          if (a.energy) {
+            add_sq(sE[0][r - 6][c - 3], pv);
+            add_sq(sE[1][r - 6][c - 3], vx);
+            add_sq(sE[2][r - 6][c - 3], vz);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (!RECOMP) {
+    if (a.energy) {
+      const long long inner = (long long)d.nz * d.nx;
+      float* eg = a.energy + (long long)blockIdx.y * 3 * inner;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if (fl[n] & F_T) {
+          const int item = tid + n * NTH;
+          const int r = item / W2, g = item % W2;
+          const int gz = z0 - 6 + r, xb = x0 - 8 + 4 * g;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if ((fl[n] >> (M_ENRG + e)) & 1u) {
+              const long long o = (long long)(gz - d.nabc) * d.nx + (xb + e - d.nabc);
+              eg[o] = eg[o] + f4get(sE[0][r - 6][g - 2], e);
+              eg[inner + o] = eg[inner + o] + f4get(sE[1][r - 6][g - 2], e);
+              eg[2 * inner + o] = eg[2 * inner + o] + f4get(sE[2][r - 6][g - 2], e);
+            }
           }
         }
       }
@@ -1258,6 +1701,7 @@ struct Plan {
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
+  bool two;         // two time steps per forward / recompute launch
   int mseg, mseg_h; // z segments per strip and rows per segment for it
   size_t plane_bytes, slot_bytes, slot_scale_off;
   // byte offsets into the workspace
@@ -1301,10 +1745,14 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
     spb = (pl.S + want_groups - 1) / want_groups;
   }
   spb = std::min(spb, pl.S);
+  if (p->steps_per_launch != 1) spb = std::min(spb, MAX_SPB);
   pl.spb = spb;
   pl.groups = (pl.S + spb - 1) / spb;
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
   if (p->march < 0 || p->march > 2) return fail(GBW_ERR_INVALID, "march must be 0, 1 or 2");
+  if (p->steps_per_launch < 0 || p->steps_per_launch > 2)
+    return fail(GBW_ERR_INVALID, "steps_per_launch must be 0, 1 or 2");
+  pl.two = p->steps_per_launch != 1;
   // z-marching recompute only on request: with coefficients reloaded per shot
   // it measured slower than tiles on 128-column strips
   pl.march = p->march == 2;
@@ -1483,10 +1931,21 @@ int build_coefs(const Plan& pl, const gbw_acoustic2d_problem* p, const gbw_acous
 // xcd_tile() is a bijection; surplus workgroups exit at once
 inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.tiles_z + 7) / 8 * 8); }
 
-template <int TZ, bool RECOMP>
-void launch_fwd(const Plan& pl, const FwdArgs& a, hipStream_t st) {
+// one forward / recompute launch of `steps` (1 or 2) time steps from a.it
+// (the two-step kernel always tiles 16 rows, whatever tile_rows the one-step
+// and adjoint kernels use)
+template <bool RECOMP>
+void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
-  fwd_step_kernel<TZ, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
+  if (steps == 2) {
+    FwdArgs a2 = a;
+    a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+    fwd2_step_kernel<16, FWD2_THREADS, RECOMP>
+        <<<dim3((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups), FWD2_THREADS, 0, st>>>(a2);
+  } else if (pl.TZ == 32)
+    fwd_step_kernel<32, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
+  else
+    fwd_step_kernel<16, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
 }
 
 template <int TZ>
@@ -1587,7 +2046,15 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   a.energy = energy ? E : nullptr;
   const int se = prob->snapshot_every;
   int snap = 0;
-  for (int it = 0; it < pl.nt; ++it) {
+  // the state after step `t` must be materialised (checkpoint, energy chunk
+  // end, snapshot): a two-step launch never ends its first step there
+  auto boundary = [&](int t) {
+    return (pl.k && (t + 1) % pl.k == 0) || (t + 1) % pl.echunk == 0 ||
+           (se > 0 && (t + 1) % se == 0);
+  };
+  for (int it0 = 0; it0 < pl.nt;) {
+    const int steps = (pl.two && it0 + 1 < pl.nt && !boundary(it0)) ? 2 : 1;
+    const int it = it0 + steps - 1;  // last step of this launch
     float* next;
     const bool ck_step = pl.k && (it + 1) % pl.k == 0 && it + 1 < pl.nt;
     if (ck_step && ck32)
@@ -1595,9 +2062,10 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
     else
       next = (cur == A) ? B : A;
     set_state(a, cur, next, fstride);
-    a.it = it;
-    if (pl.TZ == 32) launch_fwd<32, false>(pl, a, st); else launch_fwd<16, false>(pl, a, st);
+    a.it = it0;
+    launch_steps<false>(pl, a, steps, st);
     LAUNCH_CHECK();
+    it0 += steps;
     cur = next;
     if (ck_step && pl.bf16 &&
         (rc = quantize_state(pl, cur, ck + (size_t)((it + 1) / pl.k) * pl.slot_bytes, MX, st)))
@@ -1677,22 +2145,23 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
     } else {
       cur = const_cast<float*>(reinterpret_cast<const float*>(ck + (size_t)c * pl.slot_bytes));
     }
-    for (int it = t0; it < t1; ++it) {
+    for (int it = t0; it < t1;) {
+      const int steps = (pl.two && !pl.march && it + 1 < t1) ? 2 : 1;
       float* next = (cur == A) ? B : A;
       set_state(f, cur, next, fstride);
       f.it = it;
       f.dbuf = D + (long long)(it - t0) * fstride;
       f.gxbuf = GX ? GX + (long long)(it - t0) * fstride : nullptr;
       f.gzbuf = GZ ? GZ + (long long)(it - t0) * fstride : nullptr;
+      f.bstride = fstride;
+      it += steps;
       if (pl.march) {
         FwdArgs fm = f;
         fm.ntiles = pl.tiles_x * pl.mseg * pl.S;
         fm.seg_h = pl.mseg_h;
         recomp_march_kernel<<<(unsigned)((fm.ntiles + 7) / 8 * 8), NTHREADS, 0, st>>>(fm);
-      } else if (pl.TZ == 32) {
-        launch_fwd<32, true>(pl, f, st);
       } else {
-        launch_fwd<16, true>(pl, f, st);
+        launch_steps<true>(pl, f, steps, st);
       }
       LAUNCH_CHECK();
       cur = next;

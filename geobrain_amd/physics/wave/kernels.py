@@ -60,7 +60,8 @@ def auto_checkpoint_interval(nt: int) -> int:
 class LaunchChoice:
     """Launch parameters the library accepts: ``tile_rows`` (16 or 32),
     ``shots_per_block`` and ``march`` (gradient recompute: 1 tiles, 2 z-marching
-    strips); zeros select the library's defaults.  ``bwd_tile_rows`` and
+    strips), ``steps_per_launch`` (1 or 2 time steps per forward / recompute
+    launch); zeros select the library's defaults.  ``bwd_tile_rows`` and
     ``bwd_shots_per_block`` apply to the gradient call (0 = same as the
     forward's): the forward and adjoint kernels have different register
     budgets, so their best tiles differ.  The autotuner fills these."""
@@ -69,6 +70,7 @@ class LaunchChoice:
     march: int = 0
     bwd_tile_rows: int = 0
     bwd_shots_per_block: int = 0
+    steps_per_launch: int = 0
 
 
 _launch_override: Optional[LaunchChoice] = None
@@ -306,6 +308,7 @@ def acoustic_forward_kernel(
     prob.tile_rows = choice.tile_rows if choice else 0
     prob.shots_per_block = choice.shots_per_block if choice else 0
     prob.march = choice.march if choice else 0
+    prob.steps_per_launch = choice.steps_per_launch if choice else 0
     bwd_launch = (prob.tile_rows, prob.shots_per_block)
     if choice and (choice.bwd_tile_rows or choice.bwd_shots_per_block):
         bwd_launch = (choice.bwd_tile_rows, choice.bwd_shots_per_block)

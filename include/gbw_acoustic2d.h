@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GBW_ABI_VERSION 2
+#define GBW_ABI_VERSION 3
 
 typedef enum {
   GBW_OK = 0,
@@ -68,6 +68,8 @@ typedef struct gbw_acoustic2d_problem {
   int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
   int32_t march;             /* launch choice for the gradient's recompute: 0 auto,     */
                              /* 1 tiles, 2 z-marching strips                             */
+  int32_t steps_per_launch;  /* launch choice: time steps per forward/recompute launch */
+                             /* (1 or 2; 0 = default 2; two-step launches tile 16 rows) */
 } gbw_acoustic2d_problem;
 
 /* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model

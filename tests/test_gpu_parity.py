@@ -409,3 +409,43 @@ def test_free_surface_receivers_on_mirror_rows(march):
             e = rel_l2(np_(ts[key].grad), want)
             print(f"  grad {key}: rel-L2 vs {prec} oracle {e:.2e}")
             assert e <= TOL, (key, prec)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_two_step_launches_equal_one_step(fs):
+    """Two time steps per launch (temporal blocking) against one step per
+    launch on a multi-tile grid with an odd step count and odd checkpoint /
+    energy-chunk lengths, so launches of both kinds mix: seismograms and
+    gradients bit-identical, energy maps within TOL (summation order differs),
+    and seismograms bit-identical to the FP32 oracle."""
+    print(f"[case] two-step vs one-step launches, free_surface={fs}", flush=True)
+    from geobrain_amd import synthetic
+    nz, nx, nt, S = 70, 300, 241, 5
+    vp = synthetic.grf_model(nz, nx, 5.0, seed=6)
+    w = synthetic.ricker_padded(15.0, 5e-4, nt)
+    damp = wave.bc_pml(nx, nz, 5.0, 5.0, 20, float(vp.max()), False).astype(np.float32)
+    rx = np.arange(0, nx, 3)
+    z = dict(in_vp=vp, in_rho=synthetic.gardner(vp), in_damp=damp,
+             in_wavelet=np.stack([w * (1 + 0.1 * s) for s in range(S)]).astype(np.float32),
+             in_src_x=synthetic.surface_shots(nx, S), in_src_z=np.arange(S) % 3,
+             in_rcv_x=rx, in_rcv_z=np.full(len(rx), 2))
+    m = dict(nz=nz, nx=nx, nt=nt, dx=5.0, dt=5e-4, nabc=20, free_surface=fs, segments=4,
+             snapshot_every=0)
+    res = {}
+    for spl in (1, 2):
+        out, ts = run(z, m, grad=("vp", "rho", "w"),
+                      launch=K.LaunchChoice(0, 2, steps_per_launch=spl))
+        loss = (out["p"] ** 2).sum() + (out["vz"] ** 2).sum()
+        loss.backward()
+        res[spl] = ({k: np_(out[k]) for k in out},
+                    {k: np_(ts[k].grad) for k in ("vp", "rho", "w")})
+    ref = oracle.forward(**oracle_args(z, m), chunk=chunk_len(nt, 4), precision="f32")
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_array_equal(res[2][0][k], res[1][0][k])
+        np.testing.assert_array_equal(res[2][0][k], ref[k])
+    for k in ("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"):
+        e = rel_l2(res[2][0][k], res[1][0][k])
+        print(f"  {k}: rel-L2 two-step vs one-step {e:.2e}")
+        assert e <= TOL, k
+    for k in ("vp", "rho", "w"):
+        np.testing.assert_array_equal(res[2][1][k], res[1][1][k])
