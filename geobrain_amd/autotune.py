@@ -134,11 +134,15 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
             return K.acoustic_forward_kernel(**trial)["p"].sum()
         pre.backward()
 
-    # the recompute inside the gradient call follows the forward's
-    # steps_per_launch; the adjoint kernel takes its own tiles
+    # the recompute and adjoint inside the gradient call follow the forward's
+    # steps_per_launch and take their own tiles; the z-marching recompute
+    # exists for one-step launches only
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
-    for march in (1, 2):
-        for tr, spb in cands:
+    spl = best_f[2]
+    bwd_cands = cands if spl == 1 else [(16, spb) for spb in (0, 4, 8, 16, 32)
+                                        if spb == 0 or spb < args["src_n"]]
+    for march in ((1, 2) if spl == 1 else (1,)):
+        for tr, spb in bwd_cands:
             if march == 2 and tr != cands[0][0]:
                 continue  # the marching recompute ignores tile_rows
             K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb, best_f[2]))

@@ -1197,6 +1197,8 @@ struct AdjArgs {
   const float* coef;
   const float* dbuf;                       // [S][plane] bracket D^it
   const float* gxbuf; const float* gzbuf;  // [S][plane] vx/vz brackets of step it (rho gradient) or null
+  const float* dbuf1;                      // two-step kernel: D^(it-1)
+  const float* gxbuf1; const float* gzbuf1;  // two-step kernel: brackets of step it-1 or null
   const int* src_cell;
   int it, nt; float dt;
   const int* rhead; const int* rnext; const unsigned char* rrow; int R;
@@ -1539,6 +1541,360 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
         const float4 w = g2[t_idx[n]];
         g2[t_idx[n]] = make_float4(w.x + gacc2[n].x, w.y + gacc2[n].y, w.z + gacc2[n].z,
                                    w.w + gacc2[n].w);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-step adjoint kernel: adjoint steps it and it-1 in one launch, on the
+// staging geometry of the two-step forward kernel (the adjoint's stencils
+// reach as far as the forward's).  Regions, relative to the tile:
+//   step it:   lambda_p' rows -4 .. TZ+4, groups -1 .. GW+1;
+//              lambda_v  rows -4 .. TZ+2, groups -1 .. GW (row -4 only feeds
+//              the free-surface fold of step it-1)
+//   step it-1: lambda_p' rows -1 .. TZ+1, groups -1 .. GW; lambda_v the tile.
+// Each slot keeps its own adjoint state in registers across the phases; LDS
+// holds what neighbours read: lambda_p and lambda_vz at the start of a step
+// (the transposes of the free-surface mirrors read row f-1), q = a2*lambda_v
+// and r = a1*lambda_p'.  The records of steps it-1 and it-2 are injected where
+// the next step reads them.  Per cell the arithmetic is the one-step kernel's,
+// in the same order, and the two steps' gradient sums are added in step order,
+// so results equal two one-step launches bit for bit.
+// ---------------------------------------------------------------------------
+template <int TZ, int NTH, bool RHO>
+__global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
+  constexpr int SR = TZ + 12;
+  constexpr int NS = (SR * W2 + NTH - 1) / NTH;
+  __shared__ float4 sLP[SR][LW2];  // lambda_p at the start of the step
+  __shared__ float4 sLZ[SR][LW2];  // lambda_vz at the start of the step
+  __shared__ float4 sQX[SR][LW2];  // a2 * lambda_vx on the X-region, else 0
+  __shared__ float4 sQZ[SR][LW2];  // a2 * lambda_vz (folded) on the Z-region, else 0
+  __shared__ float4 sR[SR][LW2];   // a1 * lambda_p' on the P-region, else 0
+  __shared__ int sSrc[MAX_SPB];
+  const Dims d = a.d;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile();
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.tiles_x) * TX;
+  const int z0 = (tile / a.tiles_x) * TZ;
+  const int s_begin = blockIdx.y * a.spb;
+  const int s_end = min(a.S, s_begin + a.spb);
+  if (s_begin >= s_end) return;
+  const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
+  const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
+  const float4* __restrict__ omk3 = reinterpret_cast<const float4*>(a.coef + 3 * d.plane);
+  const float4* __restrict__ a2 = reinterpret_cast<const float4*>(a.coef + 4 * d.plane);
+
+  for (int i = tid; i < SR; i += NTH) {
+    sQX[i][0] = sQX[i][LW2 - 1] = f4zero();
+    sQZ[i][0] = sQZ[i][LW2 - 1] = f4zero();
+    sR[i][0] = sR[i][LW2 - 1] = f4zero();
+  }
+  for (int i = tid; i < s_end - s_begin; i += NTH) sSrc[i] = a.src_cell[s_begin + i];
+
+  // --- per-slot constants ---
+  unsigned idx[NS], fl[NS];
+  float4 c_omk1[NS], c_a1[NS], c_omk2[NS], c_omk3[NS], c_a2[NS];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const int item = tid + n * NTH;
+    const int r = item / W2, g = item % W2;
+    const int gz = z0 - 6 + r, xb = x0 - 8 + 4 * g;
+    bool ok;
+    idx[n] = group_index(d, gz, xb, ok);
+    const bool valid = item < SR * W2;
+    ok = ok && valid;
+    const bool p1 = valid && r >= 2 && r <= TZ + 10 && g >= 1;
+    const bool v1 = valid && r >= 2 && r <= TZ + 8 && g >= 1 && g <= GW + 2;
+    const bool p2 = valid && r >= 5 && r <= TZ + 7 && g >= 1 && g <= GW + 2;
+    const bool tt = valid && r >= 6 && r <= TZ + 5 && g >= 2 && g <= GW + 1;
+    unsigned f = 0;
+    if (ok) f |= F_OK;
+    if (p1) f |= F_P1;
+    if (v1) f |= F_V1;
+    if (p2) f |= F_P2;
+    if (tt) f |= F_T;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = xb + e;
+      const bool in = ok && x < d.NX;
+      if (in) f |= 1u << (M_PIN + e);
+      if (in && gz >= d.f + 1 && gz <= d.NZ - 3 && x >= 2 && x <= d.NX - 3) f |= 1u << (M_PREG + e);
+      if (in && gz >= d.f && gz <= d.NZ - 2 && x >= 1 && x <= d.NX - 3) f |= 1u << (M_XREG + e);
+      if (in && gz >= d.f && gz <= d.NZ - 3 && x >= 1 && x <= d.NX - 2) f |= 1u << (M_ZREG + e);
+      // vx has NX-1 columns, vz NZ-1 rows: the 1^T stencil updates those
+      if (in && x <= d.NX - 2) f |= 1u << (M_ENRG + e);
+    }
+    if (ok && (v1 || tt) && a.rrow[gz]) f |= F_RCV;
+    fl[n] = f;
+    c_omk1[n] = p1 ? omk1[idx[n]] : f4zero();
+    c_a1[n] = p1 ? a1[idx[n]] : f4zero();
+    c_omk2[n] = v1 ? omk2[idx[n]] : f4zero();
+    c_omk3[n] = v1 ? omk3[idx[n]] : f4zero();
+    c_a2[n] = a2[idx[n]];
+  }
+
+  float4 rlp[NS], rlx[NS], rlz[NS];
+  auto load = [&](int s) {
+    const long long so = (long long)s * d.plane;
+    const float4* __restrict__ LP = reinterpret_cast<const float4*>(a.LP0 + so);
+    const float4* __restrict__ LX = reinterpret_cast<const float4*>(a.LX0 + so);
+    const float4* __restrict__ LZ = reinterpret_cast<const float4*>(a.LZ0 + so);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      rlp[n] = LP[idx[n]];
+      rlx[n] = LX[idx[n]];
+      rlz[n] = LZ[idx[n]];
+    }
+  };
+  __syncthreads();  // sSrc
+  load(s_begin);
+
+  float4 gacc0[NS], gacc1[NS], gacc20[RHO ? NS : 1], gacc21[RHO ? NS : 1];
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    gacc0[n] = gacc1[n] = f4zero();
+    if constexpr (RHO) gacc20[n] = gacc21[n] = f4zero();
+  }
+
+  const int rfl = d.f - (z0 - 6);  // LDS row of the free-surface row f
+  for (int s = s_begin; s < s_end; ++s) {
+    const long long so = (long long)s * d.plane;
+    const int scell = sSrc[s - s_begin];
+    // adjoint state of this slot (registers); D and brackets of both steps
+    float4 lp[NS], lx[NS], lz[NS];
+    // D (and brackets) of step it for the tile slots (the others read a valid
+    // group), issued before the next shot's prefetch so that waiting for them
+    // does not wait for it; step it-1's are issued when that step starts
+    float4 dA[NS], gxA[RHO ? NS : 1], gzA[RHO ? NS : 1];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const bool ok = fl[n] & F_OK;
+      lp[n] = ok ? rlp[n] : f4zero();
+      lx[n] = ok ? rlx[n] : f4zero();
+      lz[n] = ok ? rlz[n] : f4zero();
+      dA[n] = reinterpret_cast<const float4*>(a.dbuf + so)[idx[n]];
+      if constexpr (RHO) {
+        gxA[n] = reinterpret_cast<const float4*>(a.gxbuf + so)[idx[n]];
+        gzA[n] = reinterpret_cast<const float4*>(a.gzbuf + so)[idx[n]];
+      }
+      const int item = tid + n * NTH;
+      if (item < SR * W2) {
+        const int r = item / W2, c = item % W2 + 1;
+        sLP[r][c] = lp[n];
+        sLZ[r][c] = lz[n];
+      }
+    }
+    if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
+    __syncthreads();
+
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int step = a.it - k;
+      const unsigned qreg = k == 0 ? ~0u : F_V1;   // slots whose q this step needs
+      const unsigned preg_s = k == 0 ? F_P1 : F_P2;
+      const unsigned vreg_s = k == 0 ? F_V1 : F_T;
+      float4 dcur[NS], gxc[RHO ? NS : 1], gzc[RHO ? NS : 1];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if (k == 0) {
+          dcur[n] = dA[n];
+          if constexpr (RHO) {
+            gxc[n] = gxA[n];
+            gzc[n] = gzA[n];
+          }
+        } else {
+          dcur[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
+          if constexpr (RHO) {
+            gxc[n] = reinterpret_cast<const float4*>(a.gxbuf1 + so)[idx[n]];
+            gzc[n] = reinterpret_cast<const float4*>(a.gzbuf1 + so)[idx[n]];
+          }
+        }
+      }
+      // q: transpose of vz[f-1] = vz[f] folded into lambda_vz first
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const int item = tid + n * NTH;
+        if (item < SR * W2 && (fl[n] & qreg)) {
+          const int r = item / W2, c = item % W2 + 1;
+          const int gz = z0 - 6 + r;
+          float4 v = lz[n];
+          if (d.fs && gz == d.f) {
+            float4 up;
+            if (r >= 1) {
+              up = sLZ[r - 1][c];
+            } else {  // row f-1 lies above the staged rows (first step only)
+              up = reinterpret_cast<const float4*>(a.LZ0 + so)[idx[n] - (d.PX >> 2)];
+            }
+            v = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
+          } else if (d.fs && gz == d.f - 1) {
+            v = f4zero();
+          }
+          lz[n] = v;
+          float4 qx = f4zero(), qz = f4zero();
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if ((fl[n] >> (M_XREG + e)) & 1u) f4set(qx, e, f4get(c_a2[n], e) * f4get(lx[n], e));
+            if ((fl[n] >> (M_ZREG + e)) & 1u) f4set(qz, e, f4get(c_a2[n], e) * f4get(v, e));
+          }
+          sQX[r][c] = qx;
+          sQZ[r][c] = qz;
+        }
+      }
+      __syncthreads();
+      // lambda_p' (gather of q; transpose of p[f-1] = -p[f+1]) and r
+      float4 rr[NS];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        rr[n] = f4zero();
+        if (fl[n] & preg_s) {
+          const int item = tid + n * NTH;
+          const int r = item / W2, c = item % W2 + 1;
+          const int gz = z0 - 6 + r;
+          const float4 qa = sQX[r][c - 1], qb = sQX[r][c], qc = sQX[r][c + 1];
+          const float qx[12] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, qc.w};
+          const float4 zm2 = sQZ[r - 2][c], zm1 = sQZ[r - 1][c];
+          const float4 z00 = sQZ[r][c], zp1 = sQZ[r + 1][c];
+          const bool ok = fl[n] & F_OK;
+          float4 lpm4 = f4zero();
+          if (d.fs && ok && gz == d.f + 1) lpm4 = sLP[r - 2][c];  // lambda_p at row f-1
+          float4 lpv = lp[n];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float lpe = 0.f, re = 0.f;
+            if (ok) {
+              lpe = f4get(lpv, e);
+              const float az = kC1 * (f4get(z00, e) - f4get(zm1, e)) + kC2 * (f4get(zp1, e) - f4get(zm2, e));
+              const float ax = kC1 * (qx[4 + e] - qx[3 + e]) + kC2 * (qx[5 + e] - qx[2 + e]);
+              float acc = az;
+              acc = acc + ax;
+              lpe = lpe + acc;
+              if (d.fs) {
+                if (gz == d.f + 1) {
+                  // lambda_p'[f-1] = lambda_p[f-1] + c2 * q_z[f]: rows above f carry no q
+                  const float zero = 0.f;
+                  const float azm = kC1 * (zero - zero) + kC2 * (f4get(zm1, e) - zero);
+                  const float axm = kC1 * (zero - zero) + kC2 * (zero - zero);
+                  float accm = azm;
+                  accm = accm + axm;
+                  const float lpm = f4get(lpm4, e) + accm;
+                  lpe = lpe - lpm;
+                } else if (gz == d.f - 1) {
+                  lpe = 0.f;
+                }
+              }
+              if ((fl[n] >> (M_PREG + e)) & 1u) re = f4get(c_a1[n], e) * lpe;
+            }
+            f4set(lpv, e, lpe);
+            f4set(rr[n], e, re);
+          }
+          lp[n] = lpv;  // lambda_p'
+        }
+      }
+      __syncthreads();  // every read of q and of lambda_p at row f-1 is done
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const int item = tid + n * NTH;
+        if (item < SR * W2) sR[item / W2][item % W2 + 1] = rr[n];
+      }
+      __syncthreads();
+      // lambda_v, lambda_p, imaging condition, records of step-1
+      const bool inject = step >= 1;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if (fl[n] & vreg_s) {
+          const int item = tid + n * NTH;
+          const int r = item / W2, c = item % W2 + 1;
+          const int gz = z0 - 6 + r;
+          const bool own = (fl[n] & (F_T | F_OK)) == (F_T | F_OK);
+          const unsigned ti = idx[n];
+          const float4 ra = sR[r][c - 1], rb = sR[r][c], rc = sR[r][c + 1];
+          const float rs[12] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z, rc.w};
+          const float4 ru = sR[r - 1][c], rd1v = sR[r + 1][c], rd2v = sR[r + 2][c];
+          float4 lpo = lp[n], lx4 = lx[n], lz4 = lz[n];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float lpe = f4get(lp[n], e);
+            if ((fl[n] >> (M_PREG + e)) & 1u) {
+              if (own) {
+                float4& g = k == 0 ? gacc0[n] : gacc1[n];
+                f4set(g, e, f4get(g, e) - lpe * f4get(dcur[n], e));
+              }
+              f4set(lpo, e, f4get(c_omk1[n], e) * lpe);
+            }
+            if (own && a.gw && (int)(ti * 4u + e) == scell && ((fl[n] >> (M_PIN + e)) & 1u))
+              a.gw[(long long)s * a.nt + step] = a.dt * lpe;
+            float lxe = f4get(lx4, e), lze = f4get(lz4, e);
+            if constexpr (RHO) {
+              if (own) {
+                float4& g2 = k == 0 ? gacc20[n] : gacc21[n];
+                const float4 gzv = gzc[n];
+                const float4 gxv = gxc[n];
+                float g2e = f4get(g2, e);
+                if ((fl[n] >> (M_ZREG + e)) & 1u) g2e = g2e - lze * f4get(gzv, e);
+                if ((fl[n] >> (M_XREG + e)) & 1u) g2e = g2e - lxe * f4get(gxv, e);
+                f4set(g2, e, g2e);
+              }
+            }
+            if ((fl[n] >> (M_XREG + e)) & 1u) lxe = f4get(c_omk2[n], e) * lxe;
+            if ((fl[n] >> (M_ZREG + e)) & 1u) lze = f4get(c_omk3[n], e) * lze;
+            if ((fl[n] >> (M_ENRG + e)) & 1u)
+              lxe += kC1 * (rs[5 + e] - rs[4 + e]) + kC2 * (rs[6 + e] - rs[3 + e]);
+            if (((fl[n] >> (M_PIN + e)) & 1u) && gz <= d.NZ - 2)
+              lze += kC1 * (f4get(rd1v, e) - f4get(rb, e)) + kC2 * (f4get(rd2v, e) - f4get(ru, e));
+            f4set(lx4, e, lxe);
+            f4set(lz4, e, lze);
+          }
+          if (inject && (fl[n] & F_RCV)) {
+            const int4 rh = reinterpret_cast<const int4*>(a.rhead)[ti];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              float lpv = f4get(lpo, e), lxe = f4get(lx4, e), lze = f4get(lz4, e);
+              for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
+                   hh = a.rnext[hh]) {
+                const long long o = ((long long)s * a.nt + (step - 1)) * a.R + hh;
+                if (a.g_p) lpv += a.g_p[o];
+                if (a.g_vx) lxe += a.g_vx[o];
+                if (a.g_vz) lze += a.g_vz[o];
+              }
+              f4set(lpo, e, lpv);
+              f4set(lx4, e, lxe);
+              f4set(lz4, e, lze);
+            }
+          }
+          lp[n] = lpo;
+          lx[n] = lx4;
+          lz[n] = lz4;
+          if (k == 0) {
+            sLP[r][c] = lpo;  // read by the next step's transposes at row f-1
+            sLZ[r][c] = lz4;
+          } else if (own) {
+            reinterpret_cast<float4*>(a.LP1 + so)[ti] = lpo;
+            reinterpret_cast<float4*>(a.LX1 + so)[ti] = lx4;
+            reinterpret_cast<float4*>(a.LZ1 + so)[ti] = lz4;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // the two steps' sums in step order: as two one-step launches add them
+  float4* g1 = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
+      const float4 v = g1[idx[n]];
+      float4 w = make_float4(v.x + gacc0[n].x, v.y + gacc0[n].y, v.z + gacc0[n].z, v.w + gacc0[n].w);
+      w = make_float4(w.x + gacc1[n].x, w.y + gacc1[n].y, w.z + gacc1[n].z, w.w + gacc1[n].w);
+      g1[idx[n]] = w;
+      if constexpr (RHO) {
+        float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
+        const float4 u = g2[idx[n]];
+        float4 t = make_float4(u.x + gacc20[n].x, u.y + gacc20[n].y, u.z + gacc20[n].z, u.w + gacc20[n].w);
+        t = make_float4(t.x + gacc21[n].x, t.y + gacc21[n].y, t.z + gacc21[n].z, t.w + gacc21[n].w);
+        g2[idx[n]] = t;
       }
     }
   }
@@ -1948,6 +2304,17 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
     fwd_step_kernel<16, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
 }
 
+// two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles
+void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
+  AdjArgs a2 = a;
+  a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+  dim3 grid((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups);
+  if (a.gA2)
+    adj2_step_kernel<16, FWD2_THREADS, true><<<grid, FWD2_THREADS, 0, st>>>(a2);
+  else
+    adj2_step_kernel<16, FWD2_THREADS, false><<<grid, FWD2_THREADS, 0, st>>>(a2);
+}
+
 template <int TZ>
 void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
@@ -2166,7 +2533,8 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
       LAUNCH_CHECK();
       cur = next;
     }
-    for (int it = t1 - 1; it >= t0; --it) {
+    for (int it = t1 - 1; it >= t0;) {
+      const int steps = (pl.two && it - 1 >= t0) ? 2 : 1;
       float* lnext = (lcur == LA) ? LB : LA;
       b.LP0 = lcur; b.LX0 = lcur + fstride; b.LZ0 = lcur + 2 * fstride;
       b.LP1 = lnext; b.LX1 = lnext + fstride; b.LZ1 = lnext + 2 * fstride;
@@ -2174,9 +2542,19 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
       b.dbuf = D + (long long)(it - t0) * fstride;
       b.gxbuf = GX ? GX + (long long)(it - t0) * fstride : nullptr;
       b.gzbuf = GZ ? GZ + (long long)(it - t0) * fstride : nullptr;
-      if (pl.TZ == 32) launch_adj<32>(pl, b, st); else launch_adj<16>(pl, b, st);
+      if (steps == 2) {
+        b.dbuf1 = b.dbuf - fstride;
+        b.gxbuf1 = GX ? b.gxbuf - fstride : nullptr;
+        b.gzbuf1 = GZ ? b.gzbuf - fstride : nullptr;
+        launch_adj2(pl, b, st);
+      } else if (pl.TZ == 32) {
+        launch_adj<32>(pl, b, st);
+      } else {
+        launch_adj<16>(pl, b, st);
+      }
       LAUNCH_CHECK();
       lcur = lnext;
+      it -= steps;
     }
   }
   if (adj->grad_vp || adj->grad_rho) {
