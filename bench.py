@@ -195,7 +195,23 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-tune", action="store_true", help="library default launch choice")
     ap.add_argument("--no-eager", action="store_true", help="skip the per-shot PyTorch GPU baseline")
+    ap.add_argument("--launch", type=str, default=None,
+                    help="pin a launch choice instead of autotuning: comma-separated "
+                         "tile_rows,shots_per_block,march,bwd_tile_rows,bwd_shots_per_block,"
+                         "steps_per_launch")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy three times (the reference "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 << 28}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(shots shard across GPUs with one gradient all-reduce per iteration, "
@@ -221,7 +237,11 @@ def main():
     vp = T(w["vp"]).requires_grad_(True)
     stream = torch.cuda.current_stream(dev)
     choice = None
-    if not args.no_tune:
+    if args.launch:
+        from geobrain_amd.physics.wave import kernels as K
+        choice = K.LaunchChoice(*[int(v) for v in args.launch.split(",")])
+        K.set_launch_choice(choice)
+    elif not args.no_tune:
         from geobrain_amd import autotune
         choice = autotune.tune(dict(common, vp=vp, nt=nt), nt_trial=60, verbose=True)
 
@@ -245,20 +265,48 @@ def main():
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     torch.cuda.synchronize()
+    # the library brackets each time-step kernel launch with HIP events on the
+    # stream it launches on (torch's current stream) during the timed steps
+    from geobrain_amd import _native
+    _native.set_kernel_timing(True)
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(evs[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    ktimes = _native.kernel_times()
+    _native.set_kernel_timing(False)
     t_fwd = np.mean([e[0].elapsed_time(e[1]) for e in evs]) / 1e3
     t_bwd = np.mean([e[2].elapsed_time(e[3]) for e in evs]) / 1e3
     cells = S * NZ * NX * nt
     step_s = wall / args.steps
     value = cells / step_s / 1e9
-    # forward kernel: one launch per time step, 24 algorithmic bytes per cell-update
-    fwd_launch_s = t_fwd / nt
-    fwd_bytes = 24.0 * S * NZ * NX
-    achieved = fwd_bytes / fwd_launch_s / 1e9
+    # algorithmic bytes per cell and time step of each kernel: state read +
+    # written (p, vx, vz or their adjoints, 24 B), plus the bracket D written by
+    # the recompute / read by the adjoint (4 B); two-step kernels do two steps
+    per_cell_step = {"fwd_step_kernel<forward>": 24.0, "fwd_step_kernel<recompute>": 28.0,
+                     "adj_step_kernel": 28.0, "recomp_march_kernel": 28.0}
+    kernels = {}
+    for name, (launches, ms) in ktimes.items():
+        base = name.replace("fwd2_", "fwd_").replace("adj2_", "adj_")
+        steps_per = 2 if ("fwd2_" in name or "adj2_" in name) else 1
+        b = per_cell_step[base] * steps_per * S * NZ * NX
+        avg_s = ms / launches / 1e3
+        kernels[name] = {"launches": launches, "total_s": round(ms / 1e3, 4),
+                         "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": b,
+                         "achieved_gbs": round(b / avg_s / 1e9, 1)}
+    dom = max(kernels, key=lambda n: kernels[n]["total_s"])
+    achieved = kernels[dom]["achieved_gbs"]
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+    # a copy kernel) from the committed profile of this workload, when it has one
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         f"hbm_traffic_{S}shot_{w['nz']}x{w['nx']}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tdoc = json.load(fh)
+        if tdoc.get("padded") == [NZ, NX] and dom in tdoc.get("kernels", {}):
+            traffic = tdoc["kernels"][dom]["traffic"]
     job_bytes = (80.0 + 24.0 / k) * cells
     rec = {
         "metric": METRIC,
@@ -278,11 +326,12 @@ def main():
                    "shots": S, "grid": [w["nz"], w["nx"]], "padded": [NZ, NX], "nt": nt,
                    "checkpoint_every": k,
                    "launch": None if choice is None else dataclasses.asdict(choice)},
-        "roofline": {"bound": "hbm", "kernel": "fwd_step_kernel (forward pass)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_us": round(fwd_launch_s * 1e6, 2),
-                     "bytes_per_launch": fwd_bytes},
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_launch_us": kernels[dom]["avg_launch_us"],
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
+        "kernels": kernels,
         "phases": {"forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4),
                    "forward_gcells": round(cells / t_fwd / 1e9, 3),
                    "backward_gcells": round(cells / t_bwd / 1e9, 3),

@@ -26,6 +26,8 @@ EXPORTS = (
     "gbw_acoustic2d_checkpoint_bytes",
     "gbw_acoustic2d_forward",
     "gbw_acoustic2d_backward",
+    "gbw_set_kernel_timing",
+    "gbw_kernel_times",
 )
 
 
@@ -109,6 +111,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_acoustic2d_backward.argtypes = [
             ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Adjoint),
             c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_set_kernel_timing.restype = None
+        lib.gbw_set_kernel_timing.argtypes = [ctypes.c_int]
+        lib.gbw_kernel_times.restype = ctypes.c_size_t
+        lib.gbw_kernel_times.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         ver = lib.gbw_abi_version()
         if ver != ABI_VERSION:
             raise RuntimeError(f"geobrain_amd: ABI version {ver} != expected {ABI_VERSION}")
@@ -126,3 +132,21 @@ def check(rc: int) -> None:
 
 def last_error() -> str:
     return load().gbw_last_error().decode(errors="replace")
+
+
+def set_kernel_timing(on: bool) -> None:
+    """Enable (and reset) or disable the library's per-kernel event timing."""
+    load().gbw_set_kernel_timing(int(bool(on)))
+
+
+def kernel_times() -> dict:
+    """Per-kernel ``{name: (launches, total_ms)}`` since timing was enabled."""
+    lib = load()
+    n = lib.gbw_kernel_times(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    lib.gbw_kernel_times(buf, n)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, launches, ms = line.rsplit(" ", 2)
+        out[name] = (int(launches), float(ms))
+    return out

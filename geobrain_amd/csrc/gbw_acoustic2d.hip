@@ -34,6 +34,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -2283,6 +2285,74 @@ int build_coefs(const Plan& pl, const gbw_acoustic2d_problem* p, const gbw_acous
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing (off by default; gbw_set_kernel_timing).  Each
+// step-kernel launch is bracketed by two events on the caller's stream; the
+// API call that made the launches waits for its stream at its end and adds
+// the elapsed times to per-kernel totals.
+// ---------------------------------------------------------------------------
+struct TimedLaunch {
+  const char* name;
+  hipEvent_t a, b;
+};
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::map<std::string, std::pair<long long, double>> g_times;  // launches, ms
+std::vector<hipEvent_t> g_event_pool;
+thread_local std::vector<TimedLaunch> t_pending;
+
+hipEvent_t take_event() {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct LaunchTimer {
+  const char* name;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  LaunchTimer(const char* n, hipStream_t s) : name(n), st(s) {
+    bool on;
+    {
+      std::lock_guard<std::mutex> lk(g_timing_mu);
+      on = g_timing;
+    }
+    if (on && (a = take_event())) (void)hipEventRecord(a, st);
+  }
+  ~LaunchTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event();
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    t_pending.push_back({name, a, b});
+  }
+};
+
+// called at the end of an API call: collects the timed launches
+int flush_timing(hipStream_t st) {
+  if (t_pending.empty()) return 0;
+  HIP_TRY(hipStreamSynchronize(st));
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  for (const TimedLaunch& t : t_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto& v = g_times[t.name];
+      v.first += 1;
+      v.second += ms;
+    }
+    g_event_pool.push_back(t.a);
+    g_event_pool.push_back(t.b);
+  }
+  t_pending.clear();
+  return 0;
+}
+
 // grid.x is the tile count rounded up to a multiple of 8 (the XCD count), so
 // xcd_tile() is a bijection; surplus workgroups exit at once
 inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.tiles_z + 7) / 8 * 8); }
@@ -2293,6 +2363,9 @@ inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.t
 template <bool RECOMP>
 void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
+  LaunchTimer timer(steps == 2 ? (RECOMP ? "fwd2_step_kernel<recompute>" : "fwd2_step_kernel<forward>")
+                               : (RECOMP ? "fwd_step_kernel<recompute>" : "fwd_step_kernel<forward>"),
+                    st);
   if (steps == 2) {
     FwdArgs a2 = a;
     a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
@@ -2309,6 +2382,7 @@ void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   AdjArgs a2 = a;
   a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
   dim3 grid((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups);
+  LaunchTimer timer("adj2_step_kernel", st);
   if (a.gA2)
     adj2_step_kernel<16, FWD2_THREADS, true><<<grid, FWD2_THREADS, 0, st>>>(a2);
   else
@@ -2318,6 +2392,7 @@ void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
 template <int TZ>
 void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
+  LaunchTimer timer("adj_step_kernel", st);
   if (a.gA2)
     adj_step_kernel<TZ, true><<<grid, NTHREADS, 0, st>>>(a);
   else
@@ -2450,7 +2525,7 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
       ++snap;
     }
   }
-  return 0;
+  return flush_timing(st);
 }
 
 int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoustic2d_inputs* in,
@@ -2526,6 +2601,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
         FwdArgs fm = f;
         fm.ntiles = pl.tiles_x * pl.mseg * pl.S;
         fm.seg_h = pl.mseg_h;
+        LaunchTimer timer("recomp_march_kernel", st);
         recomp_march_kernel<<<(unsigned)((fm.ntiles + 7) / 8 * 8), NTHREADS, 0, st>>>(fm);
       } else {
         launch_steps<true>(pl, f, steps, st);
@@ -2575,7 +2651,29 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
                                                          adj->grad_src_v);
     LAUNCH_CHECK();
   }
-  return 0;
+  return flush_timing(st);
+}
+
+void gbw_set_kernel_timing(int on) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_timing = on != 0;
+  g_times.clear();
+}
+
+size_t gbw_kernel_times(char* buf, size_t len) {
+  std::string out;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    for (const auto& kv : g_times)
+      out += kv.first + " " + std::to_string(kv.second.first) + " " +
+             std::to_string(kv.second.second) + "\n";
+  }
+  if (buf && len > 0) {
+    const size_t n = std::min(len - 1, out.size());
+    std::memcpy(buf, out.data(), n);
+    buf[n] = '\0';
+  }
+  return out.size() + 1;
 }
 
 }  // extern "C"

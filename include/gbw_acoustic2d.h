@@ -139,6 +139,16 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob,
                             void* workspace, size_t workspace_bytes,
                             void* stream /* hipStream_t */);
 
+/* Measurement hook, no counterpart in the reference (for benchmarks and
+ * profilers that need per-kernel times from a live run).  While enabled, each
+ * time-step kernel launch is bracketed by two events on the caller's stream
+ * and every forward / backward call waits for its stream before returning.
+ * gbw_set_kernel_timing(1) enables it and clears the totals, (0) disables it.
+ * gbw_kernel_times writes one line per kernel, "name launches total_ms", into
+ * buf (NUL-terminated, truncated to len) and returns the bytes it needs. */
+void gbw_set_kernel_timing(int on);
+size_t gbw_kernel_times(char* buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif
