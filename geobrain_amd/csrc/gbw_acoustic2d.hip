@@ -529,6 +529,7 @@ constexpr int LW2 = GW + 6;  // LDS row: staged groups plus a zero guard each si
 // per-slot flag word: bits 0-6 slot flags, then 4-bit per-cell masks
 constexpr unsigned F_OK = 1u, F_PV = 2u, F_P1 = 4u, F_V1 = 8u, F_P2 = 16u, F_T = 32u, F_RCV = 64u;
 constexpr unsigned F_PFULL = 128u, F_VFULL = 1u << 28;  // all 4 cells in the P / X and Z regions
+constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surface exception
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
 template <int TZ, int NTH, bool RECOMP>
@@ -1574,6 +1575,9 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
   __shared__ float4 sQZ[SR][LW2];  // a2 * lambda_vz (folded) on the Z-region, else 0
   __shared__ float4 sR[SR][LW2];   // a1 * lambda_p' on the P-region, else 0
   __shared__ int sSrc[MAX_SPB];
+  // imaging-condition sums of the tile, per step (each cell owned by one thread)
+  __shared__ float4 sG[2][TZ][GW];
+  __shared__ float4 sG2[RHO ? 2 : 1][RHO ? TZ : 1][RHO ? GW : 1];
   const Dims d = a.d;
   const int tid = threadIdx.x;
   const int tile = xcd_tile();
@@ -1630,6 +1634,13 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
       if (in && x <= d.NX - 2) f |= 1u << (M_ENRG + e);
     }
     if (ok && (v1 || tt) && a.rrow[gz]) f |= F_RCV;
+    if (((f >> M_PREG) & 15u) == 15u) f |= F_PFULL;
+    // every cell inside the grid and in the P, X and Z regions, with the vx
+    // column and vz row the 1^T stencil updates, off the free-surface rows
+    if (ok && ((f >> M_PREG) & 15u) == 15u && ((f >> M_XREG) & 15u) == 15u &&
+        ((f >> M_ZREG) & 15u) == 15u && ((f >> M_ENRG) & 15u) == 15u && gz <= d.NZ - 2 &&
+        !(d.fs && gz >= d.f - 1 && gz <= d.f + 1))
+      f |= F_AINT;
     fl[n] = f;
     c_omk1[n] = p1 ? omk1[idx[n]] : f4zero();
     c_a1[n] = p1 ? a1[idx[n]] : f4zero();
@@ -1654,18 +1665,38 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
   __syncthreads();  // sSrc
   load(s_begin);
 
-  float4 gacc0[NS], gacc1[NS], gacc20[RHO ? NS : 1], gacc21[RHO ? NS : 1];
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
-    gacc0[n] = gacc1[n] = f4zero();
-    if constexpr (RHO) gacc20[n] = gacc21[n] = f4zero();
+    if (fl[n] & F_T) {
+      const int item = tid + n * NTH;
+      const int tr = item / W2 - 6, h = item % W2 - 2;
+      sG[0][tr][h] = sG[1][tr][h] = f4zero();
+      if constexpr (RHO) sG2[0][tr][h] = sG2[1][tr][h] = f4zero();
+    }
   }
 
   const int rfl = d.f - (z0 - 6);  // LDS row of the free-surface row f
+  // q = a2 * lambda_v of one slot (X / Z regions only)
+  auto make_q = [&](int n, const float4& lxv, const float4& lzv, float4& qx, float4& qz) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      f4set(qx, e, f4get(c_a2[n], e) * f4get(lxv, e));
+      f4set(qz, e, f4get(c_a2[n], e) * f4get(lzv, e));
+    }
+    if (!(fl[n] & F_AINT)) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (!((fl[n] >> (M_XREG + e)) & 1u)) f4set(qx, e, 0.f);
+        if (!((fl[n] >> (M_ZREG + e)) & 1u)) f4set(qz, e, 0.f);
+      }
+    }
+  };
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
     const int scell = sSrc[s - s_begin];
-    // adjoint state of this slot (registers); D and brackets of both steps
+    const int ssz = scell / d.PX, ssx = scell - ssz * d.PX;
+    const bool src_tile = ssz >= z0 && ssz < z0 + TZ && ssx >= x0 && ssx < x0 + TX;
+    // adjoint state of this slot (registers)
     float4 lp[NS], lx[NS], lz[NS];
     // D (and brackets) of step it for the tile slots (the others read a valid
     // group), issued before the next shot's prefetch so that waiting for them
@@ -1691,11 +1722,39 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     }
     if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
     __syncthreads();
+    // step it: q, with the transpose of vz[f-1] = vz[f] folded into lambda_vz
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const int item = tid + n * NTH;
+      if (item < SR * W2) {
+        const int r = item / W2, c = item % W2 + 1;
+        const int gz = z0 - 6 + r;
+        float4 v = lz[n];
+        if (d.fs && (gz == d.f || gz == d.f - 1)) {
+          if (gz == d.f) {
+            float4 up;
+            if (r >= 1) {
+              up = sLZ[r - 1][c];
+            } else {  // row f-1 lies above the staged rows
+              up = reinterpret_cast<const float4*>(a.LZ0 + so)[idx[n] - (d.PX >> 2)];
+            }
+            v = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
+          } else {
+            v = f4zero();
+          }
+        }
+        lz[n] = v;
+        float4 qx, qz;
+        make_q(n, lx[n], v, qx, qz);
+        sQX[r][c] = qx;
+        sQZ[r][c] = qz;
+      }
+    }
+    __syncthreads();
 
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int step = a.it - k;
-      const unsigned qreg = k == 0 ? ~0u : F_V1;   // slots whose q this step needs
       const unsigned preg_s = k == 0 ? F_P1 : F_P2;
       const unsigned vreg_s = k == 0 ? F_V1 : F_T;
       float4 dcur[NS], gxc[RHO ? NS : 1], gzc[RHO ? NS : 1];
@@ -1715,94 +1774,64 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           }
         }
       }
-      // q: transpose of vz[f-1] = vz[f] folded into lambda_vz first
+      // lambda_p' (gather of q; transpose of p[f-1] = -p[f+1]) and r = a1 lambda_p'
+      // (r goes to its own array: nothing reads it until after the barrier)
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
         const int item = tid + n * NTH;
-        if (item < SR * W2 && (fl[n] & qreg)) {
-          const int r = item / W2, c = item % W2 + 1;
-          const int gz = z0 - 6 + r;
-          float4 v = lz[n];
-          if (d.fs && gz == d.f) {
-            float4 up;
-            if (r >= 1) {
-              up = sLZ[r - 1][c];
-            } else {  // row f-1 lies above the staged rows (first step only)
-              up = reinterpret_cast<const float4*>(a.LZ0 + so)[idx[n] - (d.PX >> 2)];
-            }
-            v = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
-          } else if (d.fs && gz == d.f - 1) {
-            v = f4zero();
-          }
-          lz[n] = v;
-          float4 qx = f4zero(), qz = f4zero();
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            if ((fl[n] >> (M_XREG + e)) & 1u) f4set(qx, e, f4get(c_a2[n], e) * f4get(lx[n], e));
-            if ((fl[n] >> (M_ZREG + e)) & 1u) f4set(qz, e, f4get(c_a2[n], e) * f4get(v, e));
-          }
-          sQX[r][c] = qx;
-          sQZ[r][c] = qz;
-        }
-      }
-      __syncthreads();
-      // lambda_p' (gather of q; transpose of p[f-1] = -p[f+1]) and r
-      float4 rr[NS];
-#pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        rr[n] = f4zero();
+        if (item >= SR * W2) continue;
+        const int r = item / W2, c = item % W2 + 1;
+        float4 rr = f4zero();
         if (fl[n] & preg_s) {
-          const int item = tid + n * NTH;
-          const int r = item / W2, c = item % W2 + 1;
           const int gz = z0 - 6 + r;
           const float4 qa = sQX[r][c - 1], qb = sQX[r][c], qc = sQX[r][c + 1];
           const float qx[12] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, qc.w};
           const float4 zm2 = sQZ[r - 2][c], zm1 = sQZ[r - 1][c];
           const float4 z00 = sQZ[r][c], zp1 = sQZ[r + 1][c];
-          const bool ok = fl[n] & F_OK;
-          float4 lpm4 = f4zero();
-          if (d.fs && ok && gz == d.f + 1) lpm4 = sLP[r - 2][c];  // lambda_p at row f-1
-          float4 lpv = lp[n];
+          float4 lpv;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            float lpe = 0.f, re = 0.f;
-            if (ok) {
-              lpe = f4get(lpv, e);
-              const float az = kC1 * (f4get(z00, e) - f4get(zm1, e)) + kC2 * (f4get(zp1, e) - f4get(zm2, e));
-              const float ax = kC1 * (qx[4 + e] - qx[3 + e]) + kC2 * (qx[5 + e] - qx[2 + e]);
-              float acc = az;
-              acc = acc + ax;
-              lpe = lpe + acc;
-              if (d.fs) {
-                if (gz == d.f + 1) {
-                  // lambda_p'[f-1] = lambda_p[f-1] + c2 * q_z[f]: rows above f carry no q
-                  const float zero = 0.f;
-                  const float azm = kC1 * (zero - zero) + kC2 * (f4get(zm1, e) - zero);
-                  const float axm = kC1 * (zero - zero) + kC2 * (zero - zero);
-                  float accm = azm;
-                  accm = accm + axm;
-                  const float lpm = f4get(lpm4, e) + accm;
-                  lpe = lpe - lpm;
-                } else if (gz == d.f - 1) {
-                  lpe = 0.f;
-                }
+            const float az = kC1 * (f4get(z00, e) - f4get(zm1, e)) + kC2 * (f4get(zp1, e) - f4get(zm2, e));
+            const float ax = kC1 * (qx[4 + e] - qx[3 + e]) + kC2 * (qx[5 + e] - qx[2 + e]);
+            float acc = az;
+            acc = acc + ax;
+            f4set(lpv, e, f4get(lp[n], e) + acc);
+          }
+          if (!(fl[n] & F_AINT)) {
+            if (!(fl[n] & F_OK)) {
+              lpv = f4zero();
+            } else if (d.fs && gz == d.f + 1) {
+              const float4 lpm4 = sLP[r - 2][c];  // lambda_p at row f-1
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                // lambda_p'[f-1] = lambda_p[f-1] + c2 * q_z[f]: rows above f carry no q
+                const float zero = 0.f;
+                const float azm = kC1 * (zero - zero) + kC2 * (f4get(zm1, e) - zero);
+                const float axm = kC1 * (zero - zero) + kC2 * (zero - zero);
+                float accm = azm;
+                accm = accm + axm;
+                const float lpm = f4get(lpm4, e) + accm;
+                f4set(lpv, e, f4get(lpv, e) - lpm);
               }
-              if ((fl[n] >> (M_PREG + e)) & 1u) re = f4get(c_a1[n], e) * lpe;
+            } else if (d.fs && gz == d.f - 1) {
+              lpv = f4zero();
             }
-            f4set(lpv, e, lpe);
-            f4set(rr[n], e, re);
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) f4set(rr, e, f4get(c_a1[n], e) * f4get(lpv, e));
+          if (!(fl[n] & F_PFULL)) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (!((fl[n] >> (M_PREG + e)) & 1u)) f4set(rr, e, 0.f);
           }
           lp[n] = lpv;  // lambda_p'
         }
-      }
-      __syncthreads();  // every read of q and of lambda_p at row f-1 is done
-#pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        const int item = tid + n * NTH;
-        if (item < SR * W2) sR[item / W2][item % W2 + 1] = rr[n];
+        sR[r][c] = rr;
       }
       __syncthreads();
-      // lambda_v, lambda_p, imaging condition, records of step-1
+      // lambda_v, lambda_p, imaging condition, records of step-1; after the
+      // first step also q of the second (the next step's inputs are this
+      // slot's own outputs, except the free-surface fold below)
       const bool inject = step >= 1;
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
@@ -1815,39 +1844,82 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           const float4 ra = sR[r][c - 1], rb = sR[r][c], rc = sR[r][c + 1];
           const float rs[12] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z, rc.w};
           const float4 ru = sR[r - 1][c], rd1v = sR[r + 1][c], rd2v = sR[r + 2][c];
-          float4 lpo = lp[n], lx4 = lx[n], lz4 = lz[n];
+          float4 lpo, lx4, lz4;
+          if (fl[n] & F_AINT) {  // every cell in every region: no masks
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float lpe = f4get(lp[n], e);
-            if ((fl[n] >> (M_PREG + e)) & 1u) {
-              if (own) {
-                float4& g = k == 0 ? gacc0[n] : gacc1[n];
-                f4set(g, e, f4get(g, e) - lpe * f4get(dcur[n], e));
-              }
+            for (int e = 0; e < 4; ++e) {
+              const float lpe = f4get(lp[n], e);
               f4set(lpo, e, f4get(c_omk1[n], e) * lpe);
+              float lxe = f4get(lx[n], e), lze = f4get(lz[n], e);
+              lxe = f4get(c_omk2[n], e) * lxe;
+              lze = f4get(c_omk3[n], e) * lze;
+              lxe += kC1 * (rs[5 + e] - rs[4 + e]) + kC2 * (rs[6 + e] - rs[3 + e]);
+              lze += kC1 * (f4get(rd1v, e) - f4get(rb, e)) + kC2 * (f4get(rd2v, e) - f4get(ru, e));
+              f4set(lx4, e, lxe);
+              f4set(lz4, e, lze);
             }
-            if (own && a.gw && (int)(ti * 4u + e) == scell && ((fl[n] >> (M_PIN + e)) & 1u))
-              a.gw[(long long)s * a.nt + step] = a.dt * lpe;
-            float lxe = f4get(lx4, e), lze = f4get(lz4, e);
-            if constexpr (RHO) {
-              if (own) {
-                float4& g2 = k == 0 ? gacc20[n] : gacc21[n];
-                const float4 gzv = gzc[n];
-                const float4 gxv = gxc[n];
-                float g2e = f4get(g2, e);
-                if ((fl[n] >> (M_ZREG + e)) & 1u) g2e = g2e - lze * f4get(gzv, e);
-                if ((fl[n] >> (M_XREG + e)) & 1u) g2e = g2e - lxe * f4get(gxv, e);
-                f4set(g2, e, g2e);
+            if (own) {
+              float4 g = sG[k][r - 6][c - 3];
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                f4set(g, e, f4get(g, e) - f4get(lp[n], e) * f4get(dcur[n], e));
+              sG[k][r - 6][c - 3] = g;
+              if constexpr (RHO) {
+                float4 g2 = sG2[k][r - 6][c - 3];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  float g2e = f4get(g2, e);
+                  g2e = g2e - f4get(lz[n], e) * f4get(gzc[n], e);
+                  g2e = g2e - f4get(lx[n], e) * f4get(gxc[n], e);
+                  f4set(g2, e, g2e);
+                }
+                sG2[k][r - 6][c - 3] = g2;
               }
             }
-            if ((fl[n] >> (M_XREG + e)) & 1u) lxe = f4get(c_omk2[n], e) * lxe;
-            if ((fl[n] >> (M_ZREG + e)) & 1u) lze = f4get(c_omk3[n], e) * lze;
-            if ((fl[n] >> (M_ENRG + e)) & 1u)
-              lxe += kC1 * (rs[5 + e] - rs[4 + e]) + kC2 * (rs[6 + e] - rs[3 + e]);
-            if (((fl[n] >> (M_PIN + e)) & 1u) && gz <= d.NZ - 2)
-              lze += kC1 * (f4get(rd1v, e) - f4get(rb, e)) + kC2 * (f4get(rd2v, e) - f4get(ru, e));
-            f4set(lx4, e, lxe);
-            f4set(lz4, e, lze);
+          } else {
+            lpo = lp[n];
+            lx4 = lx[n];
+            lz4 = lz[n];
+            float4 g = f4zero(), g2 = f4zero();
+            if (own) {
+              g = sG[k][r - 6][c - 3];
+              if constexpr (RHO) g2 = sG2[k][r - 6][c - 3];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float lpe = f4get(lp[n], e);
+              if ((fl[n] >> (M_PREG + e)) & 1u) {
+                if (own) f4set(g, e, f4get(g, e) - lpe * f4get(dcur[n], e));
+                f4set(lpo, e, f4get(c_omk1[n], e) * lpe);
+              }
+              float lxe = f4get(lx4, e), lze = f4get(lz4, e);
+              if constexpr (RHO) {
+                if (own) {
+                  float g2e = f4get(g2, e);
+                  if ((fl[n] >> (M_ZREG + e)) & 1u) g2e = g2e - lze * f4get(gzc[n], e);
+                  if ((fl[n] >> (M_XREG + e)) & 1u) g2e = g2e - lxe * f4get(gxc[n], e);
+                  f4set(g2, e, g2e);
+                }
+              }
+              if ((fl[n] >> (M_XREG + e)) & 1u) lxe = f4get(c_omk2[n], e) * lxe;
+              if ((fl[n] >> (M_ZREG + e)) & 1u) lze = f4get(c_omk3[n], e) * lze;
+              if ((fl[n] >> (M_ENRG + e)) & 1u)
+                lxe += kC1 * (rs[5 + e] - rs[4 + e]) + kC2 * (rs[6 + e] - rs[3 + e]);
+              if (((fl[n] >> (M_PIN + e)) & 1u) && gz <= d.NZ - 2)
+                lze += kC1 * (f4get(rd1v, e) - f4get(rb, e)) + kC2 * (f4get(rd2v, e) - f4get(ru, e));
+              f4set(lx4, e, lxe);
+              f4set(lz4, e, lze);
+            }
+            if (own) {
+              sG[k][r - 6][c - 3] = g;
+              if constexpr (RHO) sG2[k][r - 6][c - 3] = g2;
+            }
+          }
+          if (src_tile && own && a.gw) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if ((int)(ti * 4u + e) == scell && ((fl[n] >> (M_PIN + e)) & 1u))
+                a.gw[(long long)s * a.nt + step] = a.dt * f4get(lp[n], e);
           }
           if (inject && (fl[n] & F_RCV)) {
             const int4 rh = reinterpret_cast<const int4*>(a.rhead)[ti];
@@ -1868,10 +1940,17 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           }
           lp[n] = lpo;
           lx[n] = lx4;
-          lz[n] = lz4;
           if (k == 0) {
-            sLP[r][c] = lpo;  // read by the next step's transposes at row f-1
-            sLZ[r][c] = lz4;
+            sLP[r][c] = lpo;  // read by the next step's transpose at row f+1
+            sLZ[r][c] = lz4;  // read by the next step's fold at row f
+            // transpose of vz[f-1] = vz[f] for the next step: row f-1 gives its
+            // value to row f (added below, after the barrier) and becomes 0
+            if (d.fs && gz == d.f - 1) lz4 = f4zero();
+            lz[n] = lz4;
+            float4 qx, qz;
+            make_q(n, lx4, lz4, qx, qz);
+            sQX[r][c] = qx;
+            sQZ[r][c] = qz;
           } else if (own) {
             reinterpret_cast<float4*>(a.LP1 + so)[ti] = lpo;
             reinterpret_cast<float4*>(a.LX1 + so)[ti] = lx4;
@@ -1880,6 +1959,23 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
         }
       }
       __syncthreads();
+      if (k == 0 && d.fs && rfl >= 3 && rfl <= TZ + 8) {
+        // the next step's fold at row f: lambda_vz[f] += lambda_vz[f-1], then its q
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+          const int item = tid + n * NTH;
+          if ((fl[n] & F_V1) && item / W2 == rfl) {
+            const int r = rfl, c = item % W2 + 1;
+            const float4 up = sLZ[r - 1][c];
+            const float4 v = lz[n];
+            lz[n] = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
+            float4 qx, qz;
+            make_q(n, lx[n], lz[n], qx, qz);
+            sQZ[r][c] = qz;
+          }
+        }
+        __syncthreads();
+      }
     }
   }
   // the two steps' sums in step order: as two one-step launches add them
@@ -1887,15 +1983,17 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
     if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
-      const float4 v = g1[idx[n]];
-      float4 w = make_float4(v.x + gacc0[n].x, v.y + gacc0[n].y, v.z + gacc0[n].z, v.w + gacc0[n].w);
-      w = make_float4(w.x + gacc1[n].x, w.y + gacc1[n].y, w.z + gacc1[n].z, w.w + gacc1[n].w);
+      const int item = tid + n * NTH;
+      const int tr = item / W2 - 6, h = item % W2 - 2;
+      const float4 v = g1[idx[n]], a0 = sG[0][tr][h], a1v = sG[1][tr][h];
+      float4 w = make_float4(v.x + a0.x, v.y + a0.y, v.z + a0.z, v.w + a0.w);
+      w = make_float4(w.x + a1v.x, w.y + a1v.y, w.z + a1v.z, w.w + a1v.w);
       g1[idx[n]] = w;
       if constexpr (RHO) {
         float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
-        const float4 u = g2[idx[n]];
-        float4 t = make_float4(u.x + gacc20[n].x, u.y + gacc20[n].y, u.z + gacc20[n].z, u.w + gacc20[n].w);
-        t = make_float4(t.x + gacc21[n].x, t.y + gacc21[n].y, t.z + gacc21[n].z, t.w + gacc21[n].w);
+        const float4 u = g2[idx[n]], b0 = sG2[0][tr][h], b1 = sG2[1][tr][h];
+        float4 t = make_float4(u.x + b0.x, u.y + b0.y, u.z + b0.z, u.w + b0.w);
+        t = make_float4(t.x + b1.x, t.y + b1.y, t.z + b1.z, t.w + b1.w);
         g2[idx[n]] = t;
       }
     }
