@@ -184,6 +184,61 @@ def per_shot_gpu_baseline(w, d_obs, steps=200, k=50):
                       f"gradient, {times[-1]:.2f} s"}
 
 
+def uneven_batch(steps: int, warmup: int, tune: bool) -> dict:
+    """The uneven-batch workload: 96 independent problems from seed 7 (nz in
+    [200, 600], nx in [200, 1200], nt in [1000, 4000], 1-8 shots, f0 in
+    [8, 20] Hz, layered Gaussian-random velocity, receivers on every column),
+    forward modelling of all of them as one HIP-graph launch; launch choice
+    from the autotuner.  Returns throughput, per-problem latency and the batch
+    kernel's live per-launch time."""
+    import torch
+    from geobrain_amd import _native, autotune, synthetic
+    from geobrain_amd import batch as B
+    dev = torch.device("cuda:0")
+    specs = synthetic.uneven_batch(96, seed=7)
+    problems = [B.problem_inputs(sp, dev) for sp in specs]
+    launch = autotune.tune_batch(problems, verbose=True) if tune else B.BatchLaunch()
+    bf = B.BatchedForward(problems, launch)
+    # live per-launch time of the batch kernel (an eager run; the graph's
+    # launches are the same)
+    _native.set_kernel_timing(True)
+    bf.run()
+    torch.cuda.synchronize()
+    kt = _native.kernel_times().get("fwd2_batch_kernel", (0, 0.0))
+    _native.set_kernel_timing(False)
+    bf.capture()
+    for _ in range(warmup):
+        bf.replay()
+    torch.cuda.synchronize()
+    times, lats = [], []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bf.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+        lats.append(bf.latencies_ms())
+    t = float(np.median(times))
+    lat = np.median(np.stack(lats), axis=0)
+    nts = np.array([sp["nt"] for sp in specs])
+    return {"metric": "Gcell-updates/s, forward, uneven batch in one HIP graph",
+            "value": round(bf.cells / t / 1e9, 3), "unit": "Gcell-updates/s",
+            "graph_replay_s": round(t, 4), "replays": steps,
+            "workload": "96 problems from seed 7: nz 200-600, nx 200-1200, nt 1000-4000, "
+                        "1-8 shots, receivers on every column; forward (seismograms)",
+            "cell_updates": bf.cells, "shots": int(sum(sp["n_shots"] for sp in specs)),
+            "launch": {"shots_per_block": launch.shots_per_block,
+                       "steps_per_launch": launch.steps_per_launch},
+            "latency_ms": {"min": round(float(lat.min()), 2),
+                           "median": round(float(np.median(lat)), 2),
+                           "p90": round(float(np.percentile(lat, 90)), 2),
+                           "max": round(float(lat.max()), 2),
+                           "per_step_us_median": round(float(np.median(lat / nts * 1e3)), 2)},
+            "batch_kernel": {"launches": kt[0],
+                             "avg_launch_us": round(kt[1] / max(kt[0], 1) * 1e3, 2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +254,8 @@ def main():
                     help="pin a launch choice instead of autotuning: comma-separated "
                          "tile_rows,shots_per_block,march,bwd_tile_rows,bwd_shots_per_block,"
                          "steps_per_launch")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -345,6 +402,10 @@ def main():
         rec["speedup_vs_per_shot_gpu"] = round(value / base["value"], 1)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, steps=args.cpu_steps)
+    if not args.no_uneven:
+        del d_obs, vp, common
+        torch.cuda.empty_cache()
+        rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune)
     print(json.dumps(rec), flush=True)
 
 

@@ -28,6 +28,10 @@ EXPORTS = (
     "gbw_acoustic2d_backward",
     "gbw_set_kernel_timing",
     "gbw_kernel_times",
+    "gbw_acoustic2d_batch_workspace_bytes",
+    "gbw_acoustic2d_batch_setup",
+    "gbw_acoustic2d_batch_forward",
+    "gbw_device_wall_clock_khz",
 )
 
 
@@ -115,6 +119,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_set_kernel_timing.argtypes = [ctypes.c_int]
         lib.gbw_kernel_times.restype = ctypes.c_size_t
         lib.gbw_kernel_times.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        lib.gbw_acoustic2d_batch_workspace_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic2d_batch_workspace_bytes.argtypes = [ctypes.POINTER(Problem), ctypes.c_int]
+        lib.gbw_acoustic2d_batch_setup.restype = ctypes.c_int
+        lib.gbw_acoustic2d_batch_setup.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs), ctypes.c_int,
+            c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_acoustic2d_batch_forward.restype = ctypes.c_int
+        lib.gbw_acoustic2d_batch_forward.argtypes = [
+            ctypes.POINTER(Problem), ctypes.c_int, c_vp, ctypes.c_size_t, c_vp, c_vp]
+        lib.gbw_device_wall_clock_khz.restype = ctypes.c_int
+        lib.gbw_device_wall_clock_khz.argtypes = []
         ver = lib.gbw_abi_version()
         if ver != ABI_VERSION:
             raise RuntimeError(f"geobrain_amd: ABI version {ver} != expected {ABI_VERSION}")

@@ -157,3 +157,54 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     choice = K.LaunchChoice(*best)
     K.set_launch_choice(choice)
     return choice
+
+
+def batch_candidates() -> List[Tuple[int, int]]:
+    """(shots_per_block, steps_per_launch) pairs tried for a batched forward."""
+    return [(0, 2), (1, 2), (2, 2), (4, 2), (0, 1)]
+
+
+def tune_batch(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
+               verbose: bool = False):
+    """Pick the batched-forward launch choice (shots per workgroup, steps per
+    launch) for ``problems`` by timing each candidate on the same problems cut
+    to ``nt_trial`` steps.  Returns a ``batch.BatchLaunch``; results are cached
+    per batch shape like ``tune``."""
+    from . import batch as B
+    shapes = [(a["nz"] + 2 * a["nabc"], a["nx"] + 2 * a["nabc"], int(a["src_n"]), int(a["rcv_n"]),
+               bool(a["free_surface"]), a["src_v"].dim() == 1) for a in problems]
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
+    key = json.dumps(["batch", shapes, dev])
+    with _lock:
+        if not _cache:
+            _load_file()
+        hit = _cache.get(key)
+    if hit is not None and len(hit) == 2:
+        return B.BatchLaunch(*hit)
+    trial = []
+    for a in problems:
+        nt = min(nt_trial, int(a["nt"]))
+        trial.append(dict(a, nt=nt, src_v=a["src_v"][..., :nt].contiguous()))
+    best, best_t = None, float("inf")
+    for spb, spl in (candidates or batch_candidates()):
+        bf = B.BatchedForward(trial, B.BatchLaunch(spb, spl))
+        times = []
+        for _ in range(repeats + 1):
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record()
+            bf.run()
+            end.record()
+            torch.cuda.synchronize()
+            times.append(start.elapsed_time(end))
+        t = min(times[1:])
+        if verbose:
+            print(f"  autotune batch shots_per_block={spb} steps_per_launch={spl}: {t:.2f} ms",
+                  file=sys.stderr, flush=True)
+        if t < best_t:
+            best, best_t = (spb, spl), t
+        del bf
+    with _lock:
+        _cache[key] = best
+        _save_file()
+    return B.BatchLaunch(*best)

@@ -168,6 +168,7 @@ struct FwdArgs {
   float* dbuf;                   // [S][plane] bracket D of this step, or null
   float* gxbuf; float* gzbuf;    // [S][plane] vx / vz brackets of this step, or null
   long long bstride;             // two-step kernel: offset of step it+1's brackets
+  int one;                       // two-step kernel: do step it only (the last of a problem)
 };
 
 // ---------------------------------------------------------------------------
@@ -532,8 +533,9 @@ constexpr unsigned F_PFULL = 128u, F_VFULL = 1u << 28;  // all 4 cells in the P 
 constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surface exception
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
+// one workgroup's tile `tile` and shot group `group` of a problem
 template <int TZ, int NTH, bool RECOMP>
-__global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
+__device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group) {
   constexpr int SR = TZ + 12;                    // staged rows (-6 .. TZ+5)
   constexpr int NS = (SR * W2 + NTH - 1) / NTH;  // slots per thread
   // LDS row index = tile row + 6; LDS column of staged group g (0 = group -2) is g + 1
@@ -543,11 +545,9 @@ __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
   __shared__ float4 sE[RECOMP ? 1 : 3][RECOMP ? 1 : TZ][RECOMP ? 1 : GW];  // energy sums
   const Dims d = a.d;
   const int tid = threadIdx.x;
-  const int tile = xcd_tile();
-  if (tile >= a.ntiles) return;
   const int x0 = (tile % a.tiles_x) * TX;
   const int z0 = (tile / a.tiles_x) * TZ;
-  const int s_begin = blockIdx.y * a.spb;
+  const int s_begin = group * a.spb;
   const int s_end = min(a.S, s_begin + a.spb);
   if (s_begin >= s_end) return;  // uniform per workgroup
   const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
     const float* wv = a.wav + (long long)(s_begin + i) * a.wav_stride + a.it;
     sSrc[i] = a.src_cell[s_begin + i];
     sW[0][i] = a.dt * wv[0];
-    sW[1][i] = a.dt * wv[1];
+    sW[1][i] = a.one ? 0.f : a.dt * wv[1];  // step it+1 exists only without `one`
   }
   __syncthreads();
   for (int s = s_begin; s < s_end; ++s) {
@@ -851,6 +851,23 @@ __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
         }
       }
     }
+    if (a.one) {
+      // step it only: store p', v' of the tile (LDS holds both mirrors'
+      // results for every tile row, row f-1 included)
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
+          const int item = tid + n * NTH;
+          const int r = item / W2, c = item % W2 + 1;
+          const unsigned ti = idx[n];
+          reinterpret_cast<float4*>(a.P1 + so)[ti] = sP[r][c];
+          reinterpret_cast<float4*>(a.VX1 + so)[ti] = sVX[r][c];
+          reinterpret_cast<float4*>(a.VZ1 + so)[ti] = sVZ[r][c];
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     // step it+1: p'' (reads only v' and this slot's own p')
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
@@ -927,6 +944,72 @@ __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
           }
         }
       }
+    }
+  }
+}
+
+template <int TZ, int NTH, bool RECOMP>
+__global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
+  const int tile = xcd_tile();
+  if (tile >= a.ntiles) return;
+  fwd2_tile<TZ, NTH, RECOMP>(a, tile, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched forward: many independent problems (uneven grids, step counts and
+// shot counts) advanced by one launch per two time steps.  Problems are sorted
+// by step count, longest first, so the problems still running at step `it`
+// are a prefix; work units (tile, shot group) of those problems are numbered
+// consecutively (unit_start) and handed out XCD-aware like single-problem tiles.
+// ---------------------------------------------------------------------------
+struct BatchDesc {
+  FwdArgs a;               // geometry, coefficients, receivers, outputs; it / state set per launch
+  float* A; float* B;      // the problem's two state buffers ([field][shot][plane])
+  long long fstride;       // shots * plane
+  int groups;              // shot groups (units = ntiles * groups)
+  int pad;
+};
+
+__global__ void batch_start_kernel(unsigned long long* done) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) done[0] = wall_clock64();
+}
+
+// zero the initial state of every problem (blockIdx.y = problem)
+__global__ void batch_zero_kernel(const BatchDesc* __restrict__ descs) {
+  const BatchDesc& b = descs[blockIdx.y];
+  const long long n = 3 * b.fstride;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b.A[i] = 0.f;
+}
+
+__global__ __launch_bounds__(FWD2_THREADS) void fwd2_batch_kernel(
+    const BatchDesc* __restrict__ descs, const int* __restrict__ unit_start, int n_active, int it,
+    int parity, int single, unsigned* counters, unsigned long long* done) {
+  const int u = xcd_tile();
+  if (u >= unit_start[n_active]) return;
+  int lo = 0, hi = n_active - 1;  // last problem whose first unit is <= u
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (unit_start[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  const int p = lo;
+  const BatchDesc& bd = descs[p];
+  FwdArgs a = bd.a;
+  float* in = parity ? bd.B : bd.A;
+  float* out = parity ? bd.A : bd.B;
+  a.P0 = in; a.VX0 = in + bd.fstride; a.VZ0 = in + 2 * bd.fstride;
+  a.P1 = out; a.VX1 = out + bd.fstride; a.VZ1 = out + 2 * bd.fstride;
+  a.it = it;
+  a.one = single || a.nt - it == 1;
+  const int local = u - unit_start[p];
+  fwd2_tile<16, FWD2_THREADS, false>(a, local % a.ntiles, local / a.ntiles);
+  if (done && it + (a.one ? 1 : 2) >= a.nt) {  // the problem's last launch
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&counters[p], 1u) == (unsigned)(a.ntiles * bd.groups) - 1u)
+        done[1 + p] = wall_clock64();
     }
   }
 }
@@ -2421,6 +2504,10 @@ struct LaunchTimer {
       std::lock_guard<std::mutex> lk(g_timing_mu);
       on = g_timing;
     }
+    if (on) {  // never inside a graph capture: the call could not wait for its events
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) on = false;
+    }
     if (on && (a = take_event())) (void)hipEventRecord(a, st);
   }
   ~LaunchTimer() {
@@ -2517,6 +2604,60 @@ FwdArgs base_fwd_args(const Plan& pl, const gbw_acoustic2d_problem* p,
 void set_state(FwdArgs& a, float* in, float* out, long long fstride) {
   a.P0 = in; a.VX0 = in + fstride; a.VZ0 = in + 2 * fstride;
   a.P1 = out; a.VX1 = out + fstride; a.VZ1 = out + 2 * fstride;
+}
+
+// ---------------------------------------------------------------------------
+// Batched forward: layout and schedule (host)
+// ---------------------------------------------------------------------------
+struct BatchLayout {
+  int n = 0, max_nt = 0, single = 0;
+  std::vector<int> order;        // sorted position -> caller's problem index
+  std::vector<Plan> plans;       // per sorted position
+  std::vector<size_t> off_prob;  // per sorted position: its sub-workspace
+  std::vector<int> unit_start;   // n + 1 prefix sums of (two-step tiles x shot groups)
+  size_t off_desc = 0, off_units = 0, off_counters = 0, total = 0;
+};
+
+int make_batch(const gbw_acoustic2d_problem* probs, int n, BatchLayout& L) {
+  if (!probs || n <= 0) return fail(GBW_ERR_INVALID, "batch needs n > 0 problems");
+  L = BatchLayout();
+  L.n = n;
+  L.order.resize(n);
+  for (int i = 0; i < n; ++i) L.order[i] = i;
+  std::stable_sort(L.order.begin(), L.order.end(),
+                   [&](int x, int y) { return probs[x].nt > probs[y].nt; });
+  L.single = probs[0].steps_per_launch == 1;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o += align256(bytes); return at; };
+  L.off_desc = take((size_t)n * sizeof(BatchDesc));
+  L.off_units = take((size_t)(n + 1) * sizeof(int));
+  L.off_counters = take((size_t)n * sizeof(unsigned));
+  L.unit_start.assign(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const gbw_acoustic2d_problem& p = probs[L.order[k]];
+    if (p.checkpoint_every != 0 || p.snapshot_every != 0)
+      return fail(GBW_ERR_UNSUPPORTED,
+                  "batched forward: checkpoints and snapshots are not supported (problem " +
+                      std::to_string(L.order[k]) + ")");
+    if ((p.steps_per_launch == 1) != (L.single != 0))
+      return fail(GBW_ERR_INVALID, "batched forward: steps_per_launch must agree across problems");
+    Plan pl;
+    int rc = make_plan(&p, 0, pl);
+    if (rc) return rc;
+    // default: all shots of a problem in one workgroup (the batch supplies the parallelism)
+    int spb = p.shots_per_block > 0 ? p.shots_per_block : pl.S;
+    spb = std::min(std::min(spb, pl.S), MAX_SPB);
+    pl.spb = spb;
+    pl.groups = (pl.S + spb - 1) / spb;
+    const int tiles2 = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+    L.unit_start[k + 1] = L.unit_start[k] + tiles2 * pl.groups;
+    L.plans.push_back(pl);
+    L.off_prob.push_back(take(pl.total));
+    L.max_nt = std::max(L.max_nt, pl.nt);
+  }
+  if (L.unit_start[n] <= 0) return fail(GBW_ERR_INVALID, "batch has no work");
+  L.total = o;
+  return 0;
 }
 
 }  // namespace
@@ -2750,6 +2891,104 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
     LAUNCH_CHECK();
   }
   return flush_timing(st);
+}
+
+size_t gbw_acoustic2d_batch_workspace_bytes(const gbw_acoustic2d_problem* probs, int n) {
+  BatchLayout L;
+  if (make_batch(probs, n, L)) return 0;
+  return L.total;
+}
+
+int gbw_acoustic2d_batch_setup(const gbw_acoustic2d_problem* probs,
+                               const gbw_acoustic2d_inputs* ins,
+                               const gbw_acoustic2d_outputs* outs, int n, void* workspace,
+                               size_t workspace_bytes, void* stream) {
+  BatchLayout L;
+  int rc = make_batch(probs, n, L);
+  if (rc) return rc;
+  if (!ins || !outs) return fail(GBW_ERR_INVALID, "inputs / outputs are NULL");
+  if (!workspace || workspace_bytes < L.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<BatchDesc> descs(n);
+  for (int k = 0; k < n; ++k) {
+    const int o = L.order[k];
+    const Plan& pl = L.plans[k];
+    void* wsk = static_cast<char*>(workspace) + L.off_prob[k];
+    const gbw_acoustic2d_outputs& out = outs[o];
+    if (out.energy_p || out.energy_vx || out.energy_vz || out.snapshots || out.checkpoints)
+      return fail(GBW_ERR_UNSUPPORTED, "batched forward returns seismograms only (problem " +
+                                           std::to_string(o) + ")");
+    const bool rec = out.rec_p || out.rec_vx || out.rec_vz;
+    if (rec && !(out.rec_p && out.rec_vx && out.rec_vz))
+      return fail(GBW_ERR_INVALID, "rec_p, rec_vx and rec_vz must be given together");
+    std::vector<int> rcell;
+    if ((rc = setup_geometry(pl, &ins[o], wsk, st, rcell))) {
+      g_last_error = "problem " + std::to_string(o) + ": " + g_last_error;
+      return rc;
+    }
+    if ((rc = build_coefs(pl, &probs[o], &ins[o], wsk, st))) return rc;
+    BatchDesc& b = descs[k];
+    b = BatchDesc{};
+    b.a = base_fwd_args(pl, &probs[o], &ins[o], wsk);
+    b.a.spb = pl.spb;
+    b.a.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+    b.a.rec_p = rec ? out.rec_p : nullptr;
+    b.a.rec_vx = rec ? out.rec_vx : nullptr;
+    b.a.rec_vz = rec ? out.rec_vz : nullptr;
+    b.a.energy = nullptr;
+    b.A = at<float>(wsk, pl.off_A);
+    b.B = at<float>(wsk, pl.off_B);
+    b.fstride = (long long)pl.S * pl.d.plane;
+    b.groups = pl.groups;
+  }
+  HIP_TRY(hipMemcpyAsync(at<BatchDesc>(workspace, L.off_desc), descs.data(),
+                         descs.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at<int>(workspace, L.off_units), L.unit_start.data(),
+                         L.unit_start.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host vectors go out of scope
+  return 0;
+}
+
+int gbw_acoustic2d_batch_forward(const gbw_acoustic2d_problem* probs, int n, void* workspace,
+                                 size_t workspace_bytes, unsigned long long* done_ticks,
+                                 void* stream) {
+  BatchLayout L;
+  int rc = make_batch(probs, n, L);
+  if (rc) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const BatchDesc* descs = at<BatchDesc>(workspace, L.off_desc);
+  const int* units = at<int>(workspace, L.off_units);
+  unsigned* counters = at<unsigned>(workspace, L.off_counters);
+  if (done_ticks) {
+    batch_start_kernel<<<1, 64, 0, st>>>(done_ticks);
+    LAUNCH_CHECK();
+    fill_int_kernel<<<1, 256, 0, st>>>(reinterpret_cast<int*>(counters), n, 0);
+    LAUNCH_CHECK();
+  }
+  batch_zero_kernel<<<dim3(256, n), 256, 0, st>>>(descs);
+  LAUNCH_CHECK();
+  const int step = L.single ? 1 : 2;
+  int n_active = n, parity = 0;
+  for (int it = 0; it < L.max_nt; it += step) {
+    while (n_active > 0 && L.plans[n_active - 1].nt <= it) --n_active;
+    const int nunits = L.unit_start[n_active];
+    LaunchTimer timer("fwd2_batch_kernel", st);
+    fwd2_batch_kernel<<<(unsigned)((nunits + 7) / 8 * 8), FWD2_THREADS, 0, st>>>(
+        descs, units, n_active, it, parity, L.single, counters, done_ticks);
+    LAUNCH_CHECK();
+    parity ^= 1;
+  }
+  return flush_timing(st);
+}
+
+int gbw_device_wall_clock_khz(void) {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return 0;
+  return khz;
 }
 
 void gbw_set_kernel_timing(int on) {

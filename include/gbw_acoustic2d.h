@@ -139,6 +139,33 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob,
                             void* workspace, size_t workspace_bytes,
                             void* stream /* hipStream_t */);
 
+/* Batched forward modelling of n independent problems (uneven grids, step
+ * counts and shot counts) as one launch sequence: one launch per two time steps
+ * covers every problem still running.  No counterpart in the reference, which
+ * models one problem per acoustic_forward_kernel call (acoustic_2d.py:294-501);
+ * this runs n such calls' forward passes together.  Each problem is described
+ * exactly as for gbw_acoustic2d_forward (seismograms only: energy maps,
+ * snapshots and checkpoints must be NULL / 0; steps_per_launch must agree).
+ *  - gbw_acoustic2d_batch_setup validates the geometry (IndexError semantics),
+ *    builds coefficients and receiver maps from the inputs and stores the
+ *    output pointers in the workspace; it synchronises `stream`.  Call it
+ *    again after changing any input.
+ *  - gbw_acoustic2d_batch_forward only enqueues kernels (no allocation, copy
+ *    or synchronisation), so it can be captured in a HIP graph and replayed.
+ *    With done_ticks (DEVICE, n + 1 entries) it writes the device wall clock
+ *    (gbw_device_wall_clock_khz) at its start into done_ticks[0] and when
+ *    problem i finishes into done_ticks[1 + i], where i is its position in the
+ *    problem list sorted by nt, longest first (ties keep the caller's order). */
+size_t gbw_acoustic2d_batch_workspace_bytes(const gbw_acoustic2d_problem* probs, int n);
+int gbw_acoustic2d_batch_setup(const gbw_acoustic2d_problem* probs,
+                               const gbw_acoustic2d_inputs* ins,
+                               const gbw_acoustic2d_outputs* outs, int n, void* workspace,
+                               size_t workspace_bytes, void* stream /* hipStream_t */);
+int gbw_acoustic2d_batch_forward(const gbw_acoustic2d_problem* probs, int n, void* workspace,
+                                 size_t workspace_bytes, unsigned long long* done_ticks,
+                                 void* stream /* hipStream_t */);
+int gbw_device_wall_clock_khz(void);
+
 /* Measurement hook, no counterpart in the reference (for benchmarks and
  * profilers that need per-kernel times from a live run).  While enabled, each
  * time-step kernel launch is bracketed by two events on the caller's stream

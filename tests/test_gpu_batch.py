@@ -1,0 +1,97 @@
+"""Batched forward of uneven problems (one launch sequence, HIP graph): every
+problem's seismograms equal its own acoustic_forward_kernel call bit for bit,
+for every launch choice, eagerly and on every graph replay."""
+
+import numpy as np
+import pytest
+import torch
+
+from geobrain_amd import batch as B
+from geobrain_amd import synthetic
+from geobrain_amd.physics import wave
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _problems():
+    """Five small problems: odd and even step counts, 1-3 shots, free surface on
+    some, a shared 1-D wavelet on one, receivers at several depths (row f-1 of
+    a free surface included) and a duplicate receiver."""
+    specs = [dict(nz=40, nx=300, nt=61, S=2, fs=False), dict(nz=70, nx=150, nt=100, S=1, fs=True),
+             dict(nz=33, nx=260, nt=47, S=3, fs=True), dict(nz=55, nx=129, nt=120, S=2, fs=False),
+             dict(nz=64, nx=200, nt=80, S=1, fs=False)]
+    probs = []
+    for i, sp in enumerate(specs):
+        nz, nx, nt, S = sp["nz"], sp["nx"], sp["nt"], sp["S"]
+        vp = synthetic.grf_model(nz, nx, 5.0, seed=20 + i)
+        w = synthetic.ricker_padded(12.0 + 2 * i, 5e-4, nt)
+        wav = (w if i == 3 else np.stack([w * (1 + 0.1 * s) for s in range(S)])).astype(np.float32)
+        damp = wave.bc_pml(nx, nz, 5.0, 5.0, 20, float(vp.max()), False).astype(np.float32)
+        rx = np.concatenate([np.arange(0, nx, 2), [7]]).astype(np.int64)
+        rz = np.array([[-1, 0, 2, 5][j % 4] if sp["fs"] else [0, 2, 5][j % 3]
+                       for j in range(len(rx))], np.int64)
+        T = lambda a: torch.as_tensor(a, device=DEV)  # noqa: E731
+        probs.append(dict(nx=nx, nz=nz, dx=5.0, dz=5.0, nt=nt, dt=5e-4, nabc=20,
+                          free_surface=sp["fs"], src_x=synthetic.surface_shots(nx, S),
+                          src_z=np.arange(S, dtype=np.int64) % 3, src_n=S, src_v=T(wav),
+                          rcv_x=rx, rcv_z=rz, rcv_n=len(rx), damp=T(damp), vp=T(vp),
+                          rho=T(synthetic.gardner(vp))))
+    return probs
+
+
+def _single(probs):
+    with torch.no_grad():
+        return [{k: v.cpu().numpy() for k, v in wave.acoustic_forward_kernel(**a).items()
+                 if k in ("p", "vx", "vz")} for a in probs]
+
+
+@pytest.mark.parametrize("spb,spl", [(0, 0), (1, 2), (2, 1)])
+def test_batch_equals_single_calls(spb, spl):
+    print(f"[case] batched forward, shots_per_block={spb} steps_per_launch={spl}", flush=True)
+    probs = _problems()
+    want = _single(probs)
+    bf = B.BatchedForward(probs, B.BatchLaunch(spb, spl))
+    out = bf.run()
+    torch.cuda.synchronize()
+    for i, (o, w) in enumerate(zip(out, want)):
+        for k in ("p", "vx", "vz"):
+            got = o[k].cpu().numpy()
+            print(f"  problem {i} {k}: {int((got != w[k]).sum())} samples differ")
+            np.testing.assert_array_equal(got, w[k])
+    lat = bf.latencies_ms()
+    print("  latencies (ms):", np.round(lat, 3).tolist())
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+
+
+def test_batch_graph_replays_equal_eager():
+    """Capture once, replay three times with eager work and synchronisation in
+    between (outputs cleared each time): every replay equals the eager run."""
+    print("[case] batched forward graph replays", flush=True)
+    probs = _problems()
+    bf = B.BatchedForward(probs)
+    eager = [{k: v.cpu().numpy() for k, v in o.items()} for o in bf.run()]
+    torch.cuda.synchronize()
+    bf.capture()
+    for rep in range(3):
+        for o in bf.outputs:
+            for v in o.values():
+                v.zero_()
+        junk = torch.randn(1 << 20, device=DEV).sum()  # unrelated eager work
+        torch.cuda.synchronize()
+        out = bf.replay()
+        torch.cuda.synchronize()
+        for i, (o, e) in enumerate(zip(out, eager)):
+            for k in ("p", "vx", "vz"):
+                np.testing.assert_array_equal(o[k].cpu().numpy(), e[k])
+        lat = bf.latencies_ms()
+        print(f"  replay {rep}: equal; latencies (ms) {np.round(lat, 3).tolist()} ({float(junk):.1f})")
+        assert np.all(lat > 0)
+
+
+def test_batch_invalid_geometry_raises():
+    probs = _problems()
+    probs[2] = dict(probs[2], rcv_x=np.array([0, 10_000], np.int64),
+                    rcv_z=np.array([0, 0], np.int64), rcv_n=2)
+    with pytest.raises(IndexError):
+        B.BatchedForward(probs)
