@@ -229,7 +229,8 @@ def uneven_batch(steps: int, warmup: int, tune: bool) -> dict:
                         "1-8 shots, receivers on every column; forward (seismograms)",
             "cell_updates": bf.cells, "shots": int(sum(sp["n_shots"] for sp in specs)),
             "launch": {"shots_per_block": launch.shots_per_block,
-                       "steps_per_launch": launch.steps_per_launch},
+                       "steps_per_launch": launch.steps_per_launch,
+                       "tile_rows": launch.tile_rows},
             "latency_ms": {"min": round(float(lat.min()), 2),
                            "median": round(float(np.median(lat)), 2),
                            "p90": round(float(np.percentile(lat, 90)), 2),
@@ -237,6 +238,92 @@ def uneven_batch(steps: int, warmup: int, tune: bool) -> dict:
                            "per_step_us_median": round(float(np.median(lat / nts * 1e3)), 2)},
             "batch_kernel": {"launches": kt[0],
                              "avg_launch_us": round(kt[1] / max(kt[0], 1) * 1e3, 2)}}
+
+
+def fwi_precision(n_iter: int = 20, shots: int = 128, nz: int = 400, nx: int = 2000,
+                  nt: int = 5000) -> dict:
+    """The precision workload: FWI of a 400 x 2000 layered Gaussian-random model
+    (dx 5 m, dt 0.5 ms, 5000 steps, 128 surface shots, 15 Hz Ricker, receivers on
+    every column) from its Gaussian-smoothed (sigma 20 cells) version, 20 Adam
+    iterations at lr 10 m/s with the example-09 driver (trace-normalised misfit,
+    bound clip and Gardner refresh), run twice: checkpoints stored in FP32 and
+    in BF16 (power-of-two FP32 scale per field and shot; compute FP32).  Reports
+    the BF16 run's first gradient and final model against the FP32-storage run."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from geobrain_amd import synthetic
+    from geobrain_amd.fwi import FWI, trace_l2_misfit
+    from geobrain_amd.physics import wave
+    dx, dt, f0 = 5.0, 5e-4, 15.0
+    vp_true = synthetic.grf_model(nz, nx, dx, seed=5)
+    vp_init = gaussian_filter(vp_true.astype(np.float64), sigma=20, mode="nearest").astype(np.float32)
+    grid = wave.GridConfig(nx=nx, nz=nz, dx=dx, dz=dx)
+    boundary = wave.BoundaryConfig(type="pml", n_layers=20, free_surface=False)
+    w = synthetic.ricker_padded(f0, dt, nt)
+    source = wave.Source(nt=nt, dt=dt, f0=f0)
+    for x in synthetic.surface_shots(nx, shots):
+        source.add_source(x=int(x), z=2, wavelet=w)
+    receiver = wave.Receiver(nt=nt, dt=dt)
+    receiver.add_receivers(x=np.arange(nx), z=np.full(nx, 2), rcv_type="pr")
+    survey = wave.Survey(source=source, receiver=receiver)
+    bounds = (float(vp_true.min()), float(vp_true.max()))
+    model_true = wave.AcousticModel(grid=grid, boundary=boundary, vp=vp_true,
+                                    rho=synthetic.gardner(vp_true), vp_grad=False, device="cuda")
+    prop = wave.AcousticPropagator(model=model_true, survey=survey, device="cuda")
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        d_obs = prop.forward()["p"].detach().clone()
+    torch.cuda.synchronize()
+    t_obs = time.perf_counter() - t0
+    del prop, model_true
+    runs = {}
+    for name, cdtype in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        model = wave.AcousticModel(grid=grid, boundary=boundary, vp=vp_init.copy(),
+                                   rho=synthetic.gardner(vp_init), vp_grad=True, rho_grad=False,
+                                   vp_bound=bounds, auto_update_rho=True, device="cuda")
+        prop = wave.AcousticPropagator(model=model, survey=survey, device="cuda")
+        prop.checkpoint_dtype = cdtype
+        fwi = FWI(prop, model, d_obs, lr=10.0, checkpoint_segments=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        first_grad = None
+        for it in range(n_iter):  # FWI.step, keeping the first gradient
+            fwi.optimizer.zero_grad()
+            result = prop.forward(checkpoint_segments=1)
+            loss = trace_l2_misfit(result["p"], d_obs)
+            loss.backward()
+            if first_grad is None:
+                first_grad = model.vp.grad.detach().clone()
+            fwi.optimizer.step()
+            fwi.scheduler.step()
+            model.forward()
+            fwi.history.append(loss.item())
+            print(f"  fwi {name} iteration {it}: loss {fwi.history[-1]:.6e} "
+                  f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        runs[name] = dict(grad=first_grad, vp=model.vp.detach().clone(), losses=list(fwi.history),
+                          seconds=time.perf_counter() - t0)
+        del fwi, prop, model
+        torch.cuda.empty_cache()
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()))
+
+    vt = torch.as_tensor(vp_true, device="cuda")
+    cells = shots * (nz + 40) * (nx + 40) * nt
+    return {"metric": "BF16-checkpoint FWI: gradient and final-model rel-L2 vs FP32 storage",
+            "workload": f"FWI {n_iter} Adam iterations (lr 10): {shots} shots, {nz}x{nx} grid, "
+                        f"{nt} steps, dx 5 m; start = true model smoothed with sigma 20 cells",
+            "grad_rel_l2_bf16_vs_fp32": rel(runs["bf16"]["grad"], runs["fp32"]["grad"]),
+            "model_rel_l2_bf16_vs_fp32": rel(runs["bf16"]["vp"], runs["fp32"]["vp"]),
+            "model_rel_l2_vs_true": {k: rel(runs[k]["vp"], vt) for k in runs},
+            "initial_model_rel_l2_vs_true": rel(torch.as_tensor(vp_init, device="cuda"), vt),
+            "loss_first_last": {k: [runs[k]["losses"][0], runs[k]["losses"][-1]] for k in runs},
+            "seconds_per_iteration": {k: round(runs[k]["seconds"] / n_iter, 3) for k in runs},
+            "gradient_gcells": {k: round(cells / (runs[k]["seconds"] / n_iter) / 1e9, 2)
+                                for k in runs},
+            "observed_data_s": round(t_obs, 2),
+            "fp64_oracle": "reduced-size comparison in tests/test_gpu_precision.py"}
 
 
 def main():
@@ -254,6 +341,10 @@ def main():
                     help="pin a launch choice instead of autotuning: comma-separated "
                          "tile_rows,shots_per_block,march,bwd_tile_rows,bwd_shots_per_block,"
                          "steps_per_launch")
+    ap.add_argument("--workload", choices=["gradient", "fwi-bf16"], default="gradient",
+                    help="gradient: the 64-shot batched gradient (default, plus the uneven "
+                         "batch); fwi-bf16: the BF16-checkpoint FWI precision study")
+    ap.add_argument("--fwi-iters", type=int, default=20)
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -278,6 +369,9 @@ def main():
     import torch
     from geobrain_amd.physics import wave
 
+    if args.workload == "fwi-bf16":
+        print(json.dumps(fwi_precision(n_iter=args.fwi_iters)), flush=True)
+        return
     dev = torch.device("cuda:0")
     w = workload(n_shots=args.shots, nt=args.nt)
     S, nt = args.shots, args.nt

@@ -68,7 +68,8 @@ def forward_candidates(n_shots: int) -> List[Tuple[int, int, int]]:
     """(tile_rows, shots_per_block, steps_per_launch) triples tried for the
     forward call (two-step launches always tile 16 rows)."""
     cands = [(tr, spb, 1) for tr, spb in default_candidates(n_shots)]
-    cands += [(16, spb, 2) for spb in (0, 2, 4, 8, 16, 32) if spb < n_shots or spb == 0]
+    for rows in (16, 24):
+        cands += [(rows, spb, 2) for spb in (0, 4, 8, 16, 32) if spb < n_shots or spb == 0]
     return cands
 
 
@@ -139,7 +140,8 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     # exists for one-step launches only
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
     spl = best_f[2]
-    bwd_cands = cands if spl == 1 else [(16, spb) for spb in (0, 4, 8, 16, 32)
+    bwd_cands = cands if spl == 1 else [(rows, spb) for rows in (16, 24)
+                                        for spb in (0, 4, 8, 16, 32)
                                         if spb == 0 or spb < args["src_n"]]
     for march in ((1, 2) if spl == 1 else (1,)):
         for tr, spb in bwd_cands:
@@ -159,9 +161,9 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     return choice
 
 
-def batch_candidates() -> List[Tuple[int, int]]:
-    """(shots_per_block, steps_per_launch) pairs tried for a batched forward."""
-    return [(0, 2), (1, 2), (2, 2), (4, 2), (0, 1)]
+def batch_candidates() -> List[Tuple[int, int, int]]:
+    """(shots_per_block, steps_per_launch, tile_rows) tried for a batched forward."""
+    return [(0, 2, 0), (2, 2, 0), (4, 2, 0), (0, 1, 0), (0, 2, 24), (2, 2, 24), (4, 2, 24)]
 
 
 def tune_batch(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
@@ -179,15 +181,15 @@ def tune_batch(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
         if not _cache:
             _load_file()
         hit = _cache.get(key)
-    if hit is not None and len(hit) == 2:
+    if hit is not None and len(hit) == 3:
         return B.BatchLaunch(*hit)
     trial = []
     for a in problems:
         nt = min(nt_trial, int(a["nt"]))
         trial.append(dict(a, nt=nt, src_v=a["src_v"][..., :nt].contiguous()))
     best, best_t = None, float("inf")
-    for spb, spl in (candidates or batch_candidates()):
-        bf = B.BatchedForward(trial, B.BatchLaunch(spb, spl))
+    for spb, spl, rows in (candidates or batch_candidates()):
+        bf = B.BatchedForward(trial, B.BatchLaunch(spb, spl, rows))
         times = []
         for _ in range(repeats + 1):
             start = torch.cuda.Event(enable_timing=True)
@@ -199,10 +201,10 @@ def tune_batch(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
             times.append(start.elapsed_time(end))
         t = min(times[1:])
         if verbose:
-            print(f"  autotune batch shots_per_block={spb} steps_per_launch={spl}: {t:.2f} ms",
-                  file=sys.stderr, flush=True)
+            print(f"  autotune batch shots_per_block={spb} steps_per_launch={spl} "
+                  f"tile_rows={rows}: {t:.2f} ms", file=sys.stderr, flush=True)
         if t < best_t:
-            best, best_t = (spb, spl), t
+            best, best_t = (spb, spl, rows), t
         del bf
     with _lock:
         _cache[key] = best

@@ -33,9 +33,11 @@ from .physics.wave import kernels as K
 @dataclass
 class BatchLaunch:
     """``shots_per_block``: shots one workgroup steps through (0 = all shots of
-    its problem); ``steps_per_launch``: 2 (default) or 1."""
+    its problem); ``steps_per_launch``: 2 (default) or 1; ``tile_rows``: 24 for
+    24 x 96 tiles, else 16 x 128."""
     shots_per_block: int = 0
     steps_per_launch: int = 0
+    tile_rows: int = 0
 
 
 class BatchedForward:
@@ -79,6 +81,7 @@ class BatchedForward:
             p.wavelet_shared = int(src_v.dim() == 1)
             p.energy_chunk = nt
             p.shots_per_block = int(self.launch.shots_per_block)
+            p.tile_rows = int(self.launch.tile_rows)
             p.steps_per_launch = int(self.launch.steps_per_launch)
             ins = self._ins[i]
             ins.src_x, ins.src_z = sx.ctypes.data, sz.ctypes.data

@@ -534,8 +534,10 @@ constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surfa
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
 // one workgroup's tile `tile` and shot group `group` of a problem
-template <int TZ, int NTH, bool RECOMP>
+template <int TZ, int TXT, int NTH, bool RECOMP>
 __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group) {
+  // this kernel's tile width (columns) and the staging geometry that follows
+  constexpr int TX = TXT, GW = TXT / 4, W2 = GW + 4, LW2 = GW + 6;
   constexpr int SR = TZ + 12;                    // staged rows (-6 .. TZ+5)
   constexpr int NS = (SR * W2 + NTH - 1) / NTH;  // slots per thread
   // LDS row index = tile row + 6; LDS column of staged group g (0 = group -2) is g + 1
@@ -948,11 +950,11 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   }
 }
 
-template <int TZ, int NTH, bool RECOMP>
+template <int TZ, int TXT, int NTH, bool RECOMP>
 __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
   const int tile = xcd_tile();
   if (tile >= a.ntiles) return;
-  fwd2_tile<TZ, NTH, RECOMP>(a, tile, blockIdx.y);
+  fwd2_tile<TZ, TXT, NTH, RECOMP>(a, tile, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,6 +985,7 @@ __global__ void batch_zero_kernel(const BatchDesc* __restrict__ descs) {
     b.A[i] = 0.f;
 }
 
+template <int TZ, int TXT>
 __global__ __launch_bounds__(FWD2_THREADS) void fwd2_batch_kernel(
     const BatchDesc* __restrict__ descs, const int* __restrict__ unit_start, int n_active, int it,
     int parity, int single, unsigned* counters, unsigned long long* done) {
@@ -1003,7 +1006,7 @@ __global__ __launch_bounds__(FWD2_THREADS) void fwd2_batch_kernel(
   a.it = it;
   a.one = single || a.nt - it == 1;
   const int local = u - unit_start[p];
-  fwd2_tile<16, FWD2_THREADS, false>(a, local % a.ntiles, local / a.ntiles);
+  fwd2_tile<TZ, TXT, FWD2_THREADS, false>(a, local % a.ntiles, local / a.ntiles);
   if (done && it + (a.one ? 1 : 2) >= a.nt) {  // the problem's last launch
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1648,8 +1651,10 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
 // in the same order, and the two steps' gradient sums are added in step order,
 // so results equal two one-step launches bit for bit.
 // ---------------------------------------------------------------------------
-template <int TZ, int NTH, bool RHO>
+template <int TZ, int TXT, int NTH, bool RHO>
 __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
+  // this kernel's tile width (columns) and the staging geometry that follows
+  constexpr int TX = TXT, GW = TXT / 4, W2 = GW + 4, LW2 = GW + 6;
   constexpr int SR = TZ + 12;
   constexpr int NS = (SR * W2 + NTH - 1) / NTH;
   __shared__ float4 sLP[SR][LW2];  // lambda_p at the start of the step
@@ -2241,6 +2246,7 @@ struct Plan {
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
   bool two;         // two time steps per forward / recompute launch
+  int TZ2, TX2, tiles_x2;  // tiles of the two-step kernels (16 x 128 or 24 x 96)
   int mseg, mseg_h; // z segments per strip and rows per segment for it
   size_t plane_bytes, slot_bytes, slot_scale_off;
   // byte offsets into the workspace
@@ -2272,8 +2278,12 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   if (pass >= 1 && pl.k == 0) return fail(GBW_ERR_INVALID, "the gradient needs checkpoint_every > 0");
   pl.nseg = pl.k ? (p->nt + pl.k - 1) / pl.k : 0;
   pl.TZ = p->tile_rows == 32 ? 32 : 16;
-  if (p->tile_rows != 0 && p->tile_rows != 16 && p->tile_rows != 32)
-    return fail(GBW_ERR_INVALID, "tile_rows must be 0, 16 or 32");
+  if (p->tile_rows != 0 && p->tile_rows != 16 && p->tile_rows != 24 && p->tile_rows != 32)
+    return fail(GBW_ERR_INVALID, "tile_rows must be 0, 16, 24 or 32");
+  // two-step kernels: 24 rows x 96 columns on request, else 16 x 128
+  pl.TZ2 = p->tile_rows == 24 ? 24 : 16;
+  pl.TX2 = pl.TZ2 == 24 ? 96 : 128;
+  pl.tiles_x2 = (d.NX + pl.TX2 - 1) / pl.TX2;
   pl.tiles_x = (d.NX + TX - 1) / TX;
   pl.tiles_z = (d.NZ + pl.TZ - 1) / pl.TZ;
   const int tiles = pl.tiles_x * pl.tiles_z;
@@ -2553,9 +2563,13 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
                     st);
   if (steps == 2) {
     FwdArgs a2 = a;
-    a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
-    fwd2_step_kernel<16, FWD2_THREADS, RECOMP>
-        <<<dim3((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups), FWD2_THREADS, 0, st>>>(a2);
+    a2.tiles_x = pl.tiles_x2;
+    a2.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
+    const dim3 g2((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups);
+    if (pl.TZ2 == 24)
+      fwd2_step_kernel<24, 96, FWD2_THREADS, RECOMP><<<g2, FWD2_THREADS, 0, st>>>(a2);
+    else
+      fwd2_step_kernel<16, 128, FWD2_THREADS, RECOMP><<<g2, FWD2_THREADS, 0, st>>>(a2);
   } else if (pl.TZ == 32)
     fwd_step_kernel<32, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
   else
@@ -2565,13 +2579,21 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
 // two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles
 void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   AdjArgs a2 = a;
-  a2.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+  a2.tiles_x = pl.tiles_x2;
+  a2.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
   dim3 grid((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups);
   LaunchTimer timer("adj2_step_kernel", st);
-  if (a.gA2)
-    adj2_step_kernel<16, FWD2_THREADS, true><<<grid, FWD2_THREADS, 0, st>>>(a2);
-  else
-    adj2_step_kernel<16, FWD2_THREADS, false><<<grid, FWD2_THREADS, 0, st>>>(a2);
+  if (pl.TZ2 == 24) {
+    if (a.gA2)
+      adj2_step_kernel<24, 96, FWD2_THREADS, true><<<grid, FWD2_THREADS, 0, st>>>(a2);
+    else
+      adj2_step_kernel<24, 96, FWD2_THREADS, false><<<grid, FWD2_THREADS, 0, st>>>(a2);
+  } else {
+    if (a.gA2)
+      adj2_step_kernel<16, 128, FWD2_THREADS, true><<<grid, FWD2_THREADS, 0, st>>>(a2);
+    else
+      adj2_step_kernel<16, 128, FWD2_THREADS, false><<<grid, FWD2_THREADS, 0, st>>>(a2);
+  }
 }
 
 template <int TZ>
@@ -2649,7 +2671,9 @@ int make_batch(const gbw_acoustic2d_problem* probs, int n, BatchLayout& L) {
     spb = std::min(std::min(spb, pl.S), MAX_SPB);
     pl.spb = spb;
     pl.groups = (pl.S + spb - 1) / spb;
-    const int tiles2 = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+    if (pl.TZ2 != (probs[0].tile_rows == 24 ? 24 : 16))
+      return fail(GBW_ERR_INVALID, "batched forward: tile_rows must agree across problems");
+    const int tiles2 = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
     L.unit_start[k + 1] = L.unit_start[k] + tiles2 * pl.groups;
     L.plans.push_back(pl);
     L.off_prob.push_back(take(pl.total));
@@ -2932,7 +2956,8 @@ int gbw_acoustic2d_batch_setup(const gbw_acoustic2d_problem* probs,
     b = BatchDesc{};
     b.a = base_fwd_args(pl, &probs[o], &ins[o], wsk);
     b.a.spb = pl.spb;
-    b.a.ntiles = pl.tiles_x * ((pl.d.NZ + 15) / 16);
+    b.a.tiles_x = pl.tiles_x2;
+    b.a.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
     b.a.rec_p = rec ? out.rec_p : nullptr;
     b.a.rec_vx = rec ? out.rec_vx : nullptr;
     b.a.rec_vz = rec ? out.rec_vz : nullptr;
@@ -2976,8 +3001,13 @@ int gbw_acoustic2d_batch_forward(const gbw_acoustic2d_problem* probs, int n, voi
     while (n_active > 0 && L.plans[n_active - 1].nt <= it) --n_active;
     const int nunits = L.unit_start[n_active];
     LaunchTimer timer("fwd2_batch_kernel", st);
-    fwd2_batch_kernel<<<(unsigned)((nunits + 7) / 8 * 8), FWD2_THREADS, 0, st>>>(
-        descs, units, n_active, it, parity, L.single, counters, done_ticks);
+    const unsigned gx = (unsigned)((nunits + 7) / 8 * 8);
+    if (L.plans[0].TZ2 == 24)
+      fwd2_batch_kernel<24, 96><<<gx, FWD2_THREADS, 0, st>>>(descs, units, n_active, it, parity,
+                                                            L.single, counters, done_ticks);
+    else
+      fwd2_batch_kernel<16, 128><<<gx, FWD2_THREADS, 0, st>>>(descs, units, n_active, it, parity,
+                                                             L.single, counters, done_ticks);
     LAUNCH_CHECK();
     parity ^= 1;
   }

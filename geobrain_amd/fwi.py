@@ -29,12 +29,28 @@ def normalize_traces(data: torch.Tensor) -> torch.Tensor:
 
 
 def trace_l2_misfit(d_pred: torch.Tensor, d_obs: torch.Tensor) -> torch.Tensor:
-    """Sum over active traces of ``||norm(d_obs) - norm(d_pred)||_2`` (over time)."""
+    """Sum over active traces of ``||norm(d_obs) - norm(d_pred)||_2`` (over time).
+
+    The example takes the square root of every trace's residual energy and then
+    drops the traces silent in both data sets (``examples/09_acoustic_fwi.py:
+    305-318``).  The square root's derivative at a silent trace is 0/0, so the
+    example's gradient is NaN everywhere as soon as one trace is silent (far
+    offsets of long lines), and likewise when a kept trace's residual is exactly
+    zero (e.g. two far traces whose only nonzero sample is the same last one:
+    both normalise to the same spike).  Here the root is taken of the kept
+    traces only, and a zero residual contributes a zero gradient (the
+    subgradient of the norm at 0): the loss is the same sum of the same values
+    in the same order, and wherever the example's gradient is finite this one
+    is identical."""
     residual = normalize_traces(d_obs) - normalize_traces(d_pred)
-    per_trace = torch.sqrt(torch.sum(residual ** 2, dim=-2))
+    energy = torch.sum(residual ** 2, dim=-2)
     active = ~((torch.sum(torch.abs(d_obs), dim=-2) == 0) &
                (torch.sum(torch.abs(d_pred), dim=-2) == 0))
-    return torch.sum(per_trace[active])
+    kept = energy[active]
+    nonzero = kept > 0
+    norms = torch.where(nonzero, torch.sqrt(torch.where(nonzero, kept, torch.ones_like(kept))),
+                        torch.zeros_like(kept))
+    return torch.sum(norms)
 
 
 @dataclass

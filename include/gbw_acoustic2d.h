@@ -64,7 +64,9 @@ typedef struct gbw_acoustic2d_problem {
   int32_t snapshot_every;    /* >0: copy shot-0 pressure after every chunk of this size */
   int32_t checkpoint_every;  /* k>0: keep the state every k steps for the gradient     */
   int32_t checkpoint_bf16;   /* 1: store checkpoints as bf16 with per-field fp32 scale */
-  int32_t tile_rows;         /* launch choice: rows per tile (16 or 32; 0 = default)   */
+  int32_t tile_rows;         /* launch choice: rows per tile: 16 or 32 for one-step     */
+                             /* launches; 24 selects 24 x 96 tiles for two-step ones    */
+                             /* (else 16 x 128); 0 = default                            */
   int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
   int32_t march;             /* launch choice for the gradient's recompute: 0 auto,     */
                              /* 1 tiles, 2 z-marching strips                             */

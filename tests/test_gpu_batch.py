@@ -46,12 +46,13 @@ def _single(probs):
                  if k in ("p", "vx", "vz")} for a in probs]
 
 
-@pytest.mark.parametrize("spb,spl", [(0, 0), (1, 2), (2, 1)])
-def test_batch_equals_single_calls(spb, spl):
-    print(f"[case] batched forward, shots_per_block={spb} steps_per_launch={spl}", flush=True)
+@pytest.mark.parametrize("spb,spl,rows", [(0, 0, 0), (1, 2, 0), (2, 1, 0), (0, 2, 24)])
+def test_batch_equals_single_calls(spb, spl, rows):
+    print(f"[case] batched forward, shots_per_block={spb} steps_per_launch={spl} "
+          f"tile_rows={rows}", flush=True)
     probs = _problems()
     want = _single(probs)
-    bf = B.BatchedForward(probs, B.BatchLaunch(spb, spl))
+    bf = B.BatchedForward(probs, B.BatchLaunch(spb, spl, rows))
     out = bf.run()
     torch.cuda.synchronize()
     for i, (o, w) in enumerate(zip(out, want)):

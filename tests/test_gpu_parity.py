@@ -116,7 +116,7 @@ def test_gradient_matches_reference_autograd(name):
             assert e_ref <= TOL, key
 
 
-@pytest.mark.parametrize("tile_rows,spb", [(16, 1), (32, 0), (32, 2), (16, 3)])
+@pytest.mark.parametrize("tile_rows,spb", [(16, 1), (32, 0), (32, 2), (16, 3), (24, 0), (24, 2)])
 def test_launch_choices_do_not_change_results(tile_rows, spb):
     print(f"[case] launch tile_rows={tile_rows} spb={spb}", flush=True)
     z, m = load("tiny2_grad")
@@ -411,14 +411,15 @@ def test_free_surface_receivers_on_mirror_rows(march):
             assert e <= TOL, (key, prec)
 
 
-@pytest.mark.parametrize("fs", [False, True])
-def test_two_step_launches_equal_one_step(fs):
+@pytest.mark.parametrize("fs,tile_rows", [(False, 0), (True, 0), (False, 24), (True, 24)])
+def test_two_step_launches_equal_one_step(fs, tile_rows):
     """Two time steps per launch (temporal blocking) against one step per
     launch on a multi-tile grid with an odd step count and odd checkpoint /
     energy-chunk lengths, so launches of both kinds mix: seismograms and
     gradients bit-identical, energy maps within TOL (summation order differs),
     and seismograms bit-identical to the FP32 oracle."""
-    print(f"[case] two-step vs one-step launches, free_surface={fs}", flush=True)
+    print(f"[case] two-step vs one-step launches, free_surface={fs}, tile_rows={tile_rows}",
+          flush=True)
     from geobrain_amd import synthetic
     nz, nx, nt, S = 70, 300, 241, 5
     vp = synthetic.grf_model(nz, nx, 5.0, seed=6)
@@ -434,7 +435,7 @@ def test_two_step_launches_equal_one_step(fs):
     res = {}
     for spl in (1, 2):
         out, ts = run(z, m, grad=("vp", "rho", "w"),
-                      launch=K.LaunchChoice(0, 2, steps_per_launch=spl))
+                      launch=K.LaunchChoice(tile_rows, 2, steps_per_launch=spl))
         loss = (out["p"] ** 2).sum() + (out["vz"] ** 2).sum()
         loss.backward()
         res[spl] = ({k: np_(out[k]) for k in out},
