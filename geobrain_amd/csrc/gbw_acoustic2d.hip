@@ -544,7 +544,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   __shared__ float4 sP[SR][LW2];
   __shared__ float4 sVX[SR][LW2];
   __shared__ float4 sVZ[SR][LW2];
-  __shared__ float4 sE[RECOMP ? 1 : 3][RECOMP ? 1 : TZ][RECOMP ? 1 : GW];  // energy sums
+  float4 eacc[RECOMP ? 1 : 3][NS];  // energy sums of the slot's cells (tile slots)
   const Dims d = a.d;
   const int tid = threadIdx.x;
   const int x0 = (tile % a.tiles_x) * TX;
@@ -609,7 +609,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
     c_omk3[n] = v1 ? omk3[idx[n]] : f4zero();
     c_a2[n] = v1 ? a2[idx[n]] : f4zero();
     if constexpr (!RECOMP) {
-      if (tt) sE[0][r - 6][g - 2] = sE[1][r - 6][g - 2] = sE[2][r - 6][g - 2] = f4zero();
+      eacc[0][n] = eacc[1][n] = eacc[2][n] = f4zero();
     }
   }
 
@@ -637,17 +637,20 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
 
   // p-update of the slot's group (acoustic_2d.py:213-237); returns the new
   // group, the bracket D in dv
+  // each slot's own p, vx, vz live in registers across the phases; LDS holds
+  // them for the neighbours' stencils
+  float4 own_p[NS], own_vx[NS], own_vz[NS];
   // p-update of the slot's group (acoustic_2d.py:213-237): computed for all
   // four cells, then cells outside the P-region keep p (a select, not a
   // branch; whole groups inside the region skip the selects).  The source
   // term is added when the shot's source lies in the staged region.
   auto p_update = [&](int n, int r, int c, bool src_here, int scell, float supd,
                       float4& dv) -> float4 {
-    const float4 xa = sVX[r][c - 1], xb = sVX[r][c], xc = sVX[r][c + 1];
+    const float4 xa = sVX[r][c - 1], xb = own_vx[n], xc = sVX[r][c + 1];
     const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
     const float4 zm2 = sVZ[r - 2][c], zm1 = sVZ[r - 1][c];
-    const float4 z00 = sVZ[r][c], zp1 = sVZ[r + 1][c];
-    const float4 p0 = sP[r][c];
+    const float4 z00 = own_vz[n], zp1 = sVZ[r + 1][c];
+    const float4 p0 = own_p[n];
     float4 out, brv;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -686,10 +689,10 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   // v-update of the slot's group (acoustic_2d.py:243-265) from the p in sP;
   // cells outside the X / Z regions keep their value
   auto v_update = [&](int n, int r, int c, float4& vx, float4& vz, float4& gxv, float4& gzv) {
-    const float4 pa = sP[r][c - 1], pb = sP[r][c], pcc = sP[r][c + 1];
+    const float4 pa = sP[r][c - 1], pb = own_p[n], pcc = sP[r][c + 1];
     const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pcc.x, pcc.y, pcc.z, pcc.w};
     const float4 pu = sP[r - 1][c], pd1 = sP[r + 1][c], pd2 = sP[r + 2][c];
-    const float4 vx0 = sVX[r][c], vz0 = sVZ[r][c];
+    const float4 vx0 = own_vx[n], vz0 = own_vz[n];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       {
@@ -777,14 +780,12 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
       if (item < SR * W2) {
         const int r = item / W2, c = item % W2 + 1;
         const bool ok = fl[n] & F_OK;
-        if (fl[n] & F_PV) {
-          sP[r][c] = rp[n];
-          sVX[r][c] = rvx[n];
-        } else {
-          sP[r][c] = f4zero();
-          sVX[r][c] = f4zero();
-        }
-        sVZ[r][c] = ok ? rvz[n] : f4zero();
+        own_p[n] = (fl[n] & F_PV) ? rp[n] : f4zero();
+        own_vx[n] = (fl[n] & F_PV) ? rvx[n] : f4zero();
+        own_vz[n] = ok ? rvz[n] : f4zero();
+        sP[r][c] = own_p[n];
+        sVX[r][c] = own_vx[n];
+        sVZ[r][c] = own_vz[n];
       }
     }
     if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
@@ -797,7 +798,8 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         const int r = item / W2, c = item % W2 + 1;
         float4 dv;
         const float4 pn = p_update(n, r, c, src_here, scell, supd0, dv);
-        sP[r][c] = pn;  // only this thread reads this group in this phase
+        own_p[n] = pn;
+        sP[r][c] = pn;  // nothing reads sP in this phase
         if constexpr (RECOMP) {
           if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK))
             reinterpret_cast<float4*>(a.dbuf + so)[idx[n]] = dv;
@@ -806,9 +808,14 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
     }
     __syncthreads();
     if (d.fs && rf >= 2 && rf + 2 <= TZ + 10) {  // p'[f-1] = -p'[f+1]
-      for (int g = tid; g < W2; g += NTH) {
-        const float4 v = sP[rf + 2][g + 1];
-        sP[rf][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const int item = tid + n * NTH;
+        if (item < SR * W2 && item / W2 == rf) {
+          const float4 v = sP[rf + 2][item % W2 + 1];
+          own_p[n] = make_float4(-v.x, -v.y, -v.z, -v.w);
+          sP[rf][item % W2 + 1] = own_p[n];
+        }
       }
       __syncthreads();
     }
@@ -820,7 +827,9 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         const int r = item / W2, c = item % W2 + 1;
         float4 vx, vz, gxv, gzv;
         v_update(n, r, c, vx, vz, gxv, gzv);
-        // in place: the v-update of a group reads no other group's v
+        own_vx[n] = vx;
+        own_vz[n] = vz;
+        // nothing reads sVX / sVZ in this phase
         sVX[r][c] = vx;
         sVZ[r][c] = vz;
         if constexpr (RECOMP) {
@@ -833,7 +842,14 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
     }
     __syncthreads();
     if (d.fs && rf >= 3 && rf + 1 <= TZ + 8) {  // vz'[f-1] = vz'[f]
-      for (int g = tid; g < W2; g += NTH) sVZ[rf][g + 1] = sVZ[rf + 1][g + 1];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const int item = tid + n * NTH;
+        if (item < SR * W2 && item / W2 == rf) {
+          own_vz[n] = sVZ[rf + 1][item % W2 + 1];
+          sVZ[rf][item % W2 + 1] = own_vz[n];
+        }
+      }
       __syncthreads();
     }
     // step it: receivers and energy of the tile
@@ -842,13 +858,13 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
       for (int n = 0; n < NS; ++n) {
         if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
           const int item = tid + n * NTH;
-          const int r = item / W2, g = item % W2;
-          const float4 pv = sP[r][g + 1], vx = sVX[r][g + 1], vz = sVZ[r][g + 1];
+          const int r = item / W2;
+          const float4 pv = own_p[n], vx = own_vx[n], vz = own_vz[n];
           if (fl[n] & F_RCV) record(n, s, a.it, z0 - 6 + r, pv, vx, vz);
           if (a.energy) {
-            add_sq(sE[0][r - 6][g - 2], pv);
-            add_sq(sE[1][r - 6][g - 2], vx);
-            add_sq(sE[2][r - 6][g - 2], vz);
+            add_sq(eacc[0][n], pv);
+            add_sq(eacc[1][n], vx);
+            add_sq(eacc[2][n], vz);
           }
         }
       }
@@ -859,12 +875,10 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
         if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
-          const int item = tid + n * NTH;
-          const int r = item / W2, c = item % W2 + 1;
           const unsigned ti = idx[n];
-          reinterpret_cast<float4*>(a.P1 + so)[ti] = sP[r][c];
-          reinterpret_cast<float4*>(a.VX1 + so)[ti] = sVX[r][c];
-          reinterpret_cast<float4*>(a.VZ1 + so)[ti] = sVZ[r][c];
+          reinterpret_cast<float4*>(a.P1 + so)[ti] = own_p[n];
+          reinterpret_cast<float4*>(a.VX1 + so)[ti] = own_vx[n];
+          reinterpret_cast<float4*>(a.VZ1 + so)[ti] = own_vz[n];
         }
       }
       __syncthreads();
@@ -878,6 +892,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         const int r = item / W2, c = item % W2 + 1;
         float4 dv;
         const float4 pn = p_update(n, r, c, src_here, scell, supd1, dv);
+        own_p[n] = pn;
         sP[r][c] = pn;
         if constexpr (RECOMP) {
           if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK))
@@ -887,9 +902,14 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
     }
     __syncthreads();
     if (d.fs && rf >= 5 && rf + 2 <= TZ + 7) {  // p''[f-1] = -p''[f+1]
-      for (int g = tid; g < W2; g += NTH) {
-        const float4 v = sP[rf + 2][g + 1];
-        sP[rf][g + 1] = make_float4(-v.x, -v.y, -v.z, -v.w);
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const int item = tid + n * NTH;
+        if (item < SR * W2 && item / W2 == rf) {
+          const float4 v = sP[rf + 2][item % W2 + 1];
+          own_p[n] = make_float4(-v.x, -v.y, -v.z, -v.w);
+          sP[rf][item % W2 + 1] = own_p[n];
+        }
       }
       __syncthreads();
     }
@@ -902,7 +922,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         const int gz = z0 - 6 + r;
         float4 vx, vz, gxv, gzv;
         v_update(n, r, c, vx, vz, gxv, gzv);
-        const float4 pv = sP[r][c];
+        const float4 pv = own_p[n];
         const unsigned ti = idx[n];
         reinterpret_cast<float4*>(a.P1 + so)[ti] = pv;
         reinterpret_cast<float4*>(a.VX1 + so)[ti] = vx;
@@ -916,9 +936,9 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         } else {
           if (fl[n] & F_RCV) record(n, s, a.it + 1, gz, pv, vx, vz);
           if (a.energy) {
-            add_sq(sE[0][r - 6][c - 3], pv);
-            add_sq(sE[1][r - 6][c - 3], vx);
-            add_sq(sE[2][r - 6][c - 3], vz);
+            add_sq(eacc[0][n], pv);
+            add_sq(eacc[1][n], vx);
+            add_sq(eacc[2][n], vz);
           }
         }
       }
@@ -939,9 +959,9 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
           for (int e = 0; e < 4; ++e) {
             if ((fl[n] >> (M_ENRG + e)) & 1u) {
               const long long o = (long long)(gz - d.nabc) * d.nx + (xb + e - d.nabc);
-              eg[o] = eg[o] + f4get(sE[0][r - 6][g - 2], e);
-              eg[inner + o] = eg[inner + o] + f4get(sE[1][r - 6][g - 2], e);
-              eg[2 * inner + o] = eg[2 * inner + o] + f4get(sE[2][r - 6][g - 2], e);
+              eg[o] = eg[o] + f4get(eacc[0][n], e);
+              eg[inner + o] = eg[inner + o] + f4get(eacc[1][n], e);
+              eg[2 * inner + o] = eg[2 * inner + o] + f4get(eacc[2][n], e);
             }
           }
         }
