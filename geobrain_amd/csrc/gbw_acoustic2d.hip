@@ -1784,6 +1784,8 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
   }
 
   const int rfl = d.f - (z0 - 6);  // LDS row of the free-surface row f
+  // this slot's own r (neighbours read it from LDS)
+  float4 own_r[NS];
   // q = a2 * lambda_v of one slot (X / Z regions only)
   auto make_q = [&](int n, const float4& lxv, const float4& lzv, float4& qx, float4& qz) {
 #pragma unroll
@@ -1892,10 +1894,13 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
         float4 rr = f4zero();
         if (fl[n] & preg_s) {
           const int gz = z0 - 6 + r;
-          const float4 qa = sQX[r][c - 1], qb = sQX[r][c], qc = sQX[r][c + 1];
+          // this slot's own q, recomputed from its registers rather than read back
+          float4 qb, z00;
+          make_q(n, lx[n], lz[n], qb, z00);
+          const float4 qa = sQX[r][c - 1], qc = sQX[r][c + 1];
           const float qx[12] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, qc.w};
           const float4 zm2 = sQZ[r - 2][c], zm1 = sQZ[r - 1][c];
-          const float4 z00 = sQZ[r][c], zp1 = sQZ[r + 1][c];
+          const float4 zp1 = sQZ[r + 1][c];
           float4 lpv;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
@@ -1934,6 +1939,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           }
           lp[n] = lpv;  // lambda_p'
         }
+        own_r[n] = rr;
         sR[r][c] = rr;
       }
       __syncthreads();
@@ -1949,7 +1955,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           const int gz = z0 - 6 + r;
           const bool own = (fl[n] & (F_T | F_OK)) == (F_T | F_OK);
           const unsigned ti = idx[n];
-          const float4 ra = sR[r][c - 1], rb = sR[r][c], rc = sR[r][c + 1];
+          const float4 ra = sR[r][c - 1], rb = own_r[n], rc = sR[r][c + 1];
           const float rs[12] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z, rc.w};
           const float4 ru = sR[r - 1][c], rd1v = sR[r + 1][c], rd2v = sR[r + 2][c];
           float4 lpo, lx4, lz4;
