@@ -184,7 +184,7 @@ def per_shot_gpu_baseline(w, d_obs, steps=200, k=50):
                       f"gradient, {times[-1]:.2f} s"}
 
 
-def uneven_batch(steps: int, warmup: int, tune: bool) -> dict:
+def uneven_batch(steps: int, warmup: int, tune: bool, launch=None) -> dict:
     """The uneven-batch workload: 96 independent problems from seed 7 (nz in
     [200, 600], nx in [200, 1200], nt in [1000, 4000], 1-8 shots, f0 in
     [8, 20] Hz, layered Gaussian-random velocity, receivers on every column),
@@ -197,7 +197,8 @@ def uneven_batch(steps: int, warmup: int, tune: bool) -> dict:
     dev = torch.device("cuda:0")
     specs = synthetic.uneven_batch(96, seed=7)
     problems = [B.problem_inputs(sp, dev) for sp in specs]
-    launch = autotune.tune_batch(problems, verbose=True) if tune else B.BatchLaunch()
+    if launch is None:
+        launch = autotune.tune_batch(problems, verbose=True) if tune else B.BatchLaunch()
     bf = B.BatchedForward(problems, launch)
     # live per-launch time of the batch kernel (an eager run; the graph's
     # launches are the same)
@@ -345,6 +346,9 @@ def main():
                     help="gradient: the 64-shot batched gradient (default, plus the uneven "
                          "batch); fwi-bf16: the BF16-checkpoint FWI precision study")
     ap.add_argument("--fwi-iters", type=int, default=20)
+    ap.add_argument("--batch-launch", type=str, default=None,
+                    help="pin the uneven batch's launch choice instead of autotuning: "
+                         "shots_per_block,steps_per_launch,tile_rows")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -499,7 +503,12 @@ def main():
     if not args.no_uneven:
         del d_obs, vp, common
         torch.cuda.empty_cache()
-        rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune)
+        blaunch = None
+        if args.batch_launch:
+            from geobrain_amd import batch as B
+            blaunch = B.BatchLaunch(*[int(v) for v in args.batch_launch.split(",")])
+        rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune,
+                                           launch=blaunch)
     print(json.dumps(rec), flush=True)
 
 
