@@ -15,6 +15,10 @@ Contents
 * ``ricker``: float64 closed form of ``wavelets/ricker.py:82-93``.
 * ``gardner``: ``models/acoustic.py:169`` (``310 * vp ** 0.25`` in float32 numpy).
 * ``chunk_len``: the chunk size ``torch.chunk`` uses (``acoustic_2d.py:430``).
+* ``forward3d``: ctypes front-end to ``acoustic3d_oracle.cpp``, a C++
+  restatement of ``geobrain/physics/wave/kernels/acoustic_3d.py`` (forward).
+* ``bc_pml_3d``: numpy restatement of ``geobrain/physics/wave/boundary/pml.py:101-195``.
+* ``pad_model_3d``: clamp-index restatement of ``kernels/acoustic_2d.py:70-120``.
 """
 
 from __future__ import annotations
@@ -59,6 +63,12 @@ def _load():
         for name in ("oracle_gradient_f32", "oracle_gradient_f64"):
             fn = getattr(lib, name)
             fn.argtypes = grad_args
+            fn.restype = I
+        fwd3_args = [I, I, I, I, I, D, D, I, I, P, P, P, P, I, P, P, P, P, P, P, I,
+                     P, P, P, P, P, P, P, P, P]
+        for name in ("oracle3_forward_f32", "oracle3_forward_f64"):
+            fn = getattr(lib, name)
+            fn.argtypes = fwd3_args
             fn.restype = I
         _lib = lib
     return _lib
@@ -222,3 +232,78 @@ def ricker(f0, dt, length_factor=2.2):
 def gardner(vp):
     """Gardner density ``310 * vp ** 0.25`` in float32 numpy (``models/acoustic.py:169``)."""
     return (310.0 * np.power(np.asarray(vp, dtype=np.float32), 0.25)).astype(np.float32)
+
+
+def forward3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_v, rcv_z, rcv_y,
+              rcv_x, vp, rho, damp, chunk=0, precision="f64", return_state=False):
+    """3D forward modelling; returns a dict with the reference's output keys
+    (``acoustic_3d.py:470-476``).  ``chunk`` is the ``torch.chunk`` length."""
+    idx = [np.ascontiguousarray(a, dtype=np.int64).reshape(-1)
+           for a in (src_z, src_y, src_x, rcv_z, rcv_y, rcv_x)]
+    S, R = idx[0].shape[0], idx[3].shape[0]
+    src_v = np.asarray(src_v, dtype=np.float32)
+    if src_v.ndim == 1:
+        src_v = np.broadcast_to(src_v, (S, src_v.shape[0]))
+    src_v = np.ascontiguousarray(src_v, dtype=np.float32)
+    nt = src_v.shape[1]
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    damp = np.ascontiguousarray(damp, dtype=np.float32)
+    assert vp.shape == (nz, ny, nx) and rho.shape == (nz, ny, nx)
+    assert damp.shape == (nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
+    dt_np = np.float32 if precision == "f32" else np.float64
+    rec = {k: np.zeros((S, nt, R), dt_np) for k in ("p", "vx", "vy", "vz")}
+    eng = {k: np.zeros((nz, ny, nx), dt_np) for k in ("p", "vx", "vy", "vz")}
+    n = S * (nz + 2 * nabc) * (ny + 2 * nabc) * (nx + 2 * nabc)
+    state = np.zeros((4, n), dt_np) if return_state else None
+    fn = getattr(_load(), "oracle3_forward_" + precision)
+    rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+            *[_ptr(a) for a in idx[:3]], _ptr(src_v), R, *[_ptr(a) for a in idx[3:]],
+            _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
+            *[_ptr(rec[k]) for k in ("p", "vx", "vy", "vz")],
+            *[_ptr(eng[k]) for k in ("p", "vx", "vy", "vz")], _ptr(state))
+    if rc == -2 or rc == -3:
+        raise IndexError("oracle3: source or receiver index out of range")
+    if rc:
+        raise ValueError(f"oracle3: invalid arguments ({rc})")
+    out = dict(rec)
+    for k in ("p", "vx", "vy", "vz"):
+        out["forward_wavefield_" + k] = eng[k]
+    if return_state:
+        out["state"] = state.reshape(4, S, nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
+    return out
+
+
+def bc_pml_3d(nx, ny, nz, dx, dy, dz, pml, vmax, free_surface=True):
+    """Quadratic PML damping on all six faces, ``boundary/pml.py:101-195``:
+    the same profile as ``bc_pml`` (spacing ``dx``), each face's layer i set to
+    the max of what is there and ``prof[i]``; the top face only without a free
+    surface.  Returns float64 (NZ, NY, NX)."""
+    NX, NY = nx + 2 * pml, ny + 2 * pml
+    NZ = nz + pml if free_surface else nz + 2 * pml
+    a = (pml - 1) * dx
+    kappa = -3.0 * vmax * math.log(1e-6) / (2.0 * a)
+    prof = np.array([kappa * (i * dx / a) * (i * dx / a) for i in range(pml)])
+    d = np.zeros((NZ, NY, NX))
+    for i in range(pml):
+        d[:, :, pml - i - 1] = np.maximum(d[:, :, pml - i - 1], prof[i])
+        d[:, :, NX - pml + i] = np.maximum(d[:, :, NX - pml + i], prof[i])
+    for i in range(pml):
+        d[:, pml - i - 1, :] = np.maximum(d[:, pml - i - 1, :], prof[i])
+        d[:, NY - pml + i, :] = np.maximum(d[:, NY - pml + i, :], prof[i])
+    for i in range(pml):
+        if not free_surface:
+            d[pml - i - 1] = np.maximum(d[pml - i - 1], prof[i])
+        d[NZ - pml + i] = np.maximum(d[NZ - pml + i], prof[i])
+    return d
+
+
+def pad_model_3d(v, nabc):
+    """Edge-replicated padding, ``acoustic_2d.py:70-120``: the chained face
+    copies give ``c[i,j,k] = v[clamp(i-nabc), clamp(j-nabc), clamp(k-nabc)]``."""
+    v = np.asarray(v)
+    nz, ny, nx = v.shape
+    iz = np.clip(np.arange(nz + 2 * nabc) - nabc, 0, nz - 1)
+    iy = np.clip(np.arange(ny + 2 * nabc) - nabc, 0, ny - 1)
+    ix = np.clip(np.arange(nx + 2 * nabc) - nabc, 0, nx - 1)
+    return v[np.ix_(iz, iy, ix)]

@@ -44,3 +44,16 @@ def adjoint_sources(z, m, fwd_p=None, obs_p=None):
         rng = np.random.default_rng(m["g_seed"])
         return tuple(rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(3))
     return (2.0 * (fwd_p - obs_p), None, None)
+
+
+FWD3D_CASES = ["t3d0_fwd", "t3d1_fwd", "t3d2_fwd"]
+OUT3D_KEYS = ("p", "vx", "vy", "vz", "forward_wavefield_p", "forward_wavefield_vx",
+              "forward_wavefield_vy", "forward_wavefield_vz")
+
+
+def oracle3d_args(z, m):
+    return dict(nz=m["nz"], ny=m["ny"], nx=m["nx"], nabc=m["nabc"], free_surface=m["free_surface"],
+                dt=m["dt"], dz=m["dx"], src_z=z["in_src_z"], src_y=z["in_src_y"],
+                src_x=z["in_src_x"], src_v=z["in_wavelet"], rcv_z=z["in_rcv_z"],
+                rcv_y=z["in_rcv_y"], rcv_x=z["in_rcv_x"], vp=z["in_vp"], rho=z["in_rho"],
+                damp=z["in_damp"])

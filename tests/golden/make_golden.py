@@ -4,9 +4,10 @@ Run ONLY in the build container, where the reference checkout exists:
 
     python tests/golden/make_golden.py [--only NAME ...]
 
-It loads two leaf modules of the reference by file path -- the FDTD kernel
-``geobrain/physics/wave/kernels/acoustic_2d.py`` (imports torch only) and the
-PML builder ``geobrain/physics/wave/boundary/pml.py`` (imports numpy only) --
+It loads leaf modules of the reference by file path -- the FDTD kernels
+``geobrain/physics/wave/kernels/acoustic_2d.py`` and ``acoustic_3d.py``
+(torch only; the 3D one imports its 2D sibling) and the PML builder
+``geobrain/physics/wave/boundary/pml.py`` (imports numpy only) --
 runs them on seeded inputs and stores inputs and outputs as ``.npz`` fixtures
 next to this script.  Nothing from the reference is copied; the fixtures are
 data.  These vectors pin what no closed form or invariant can: the reference's
@@ -47,6 +48,81 @@ def _load(name, rel):
 
 ker = _load("_gb_ref_acoustic_2d", "geobrain/physics/wave/kernels/acoustic_2d.py")
 pml = _load("_gb_ref_pml", "geobrain/physics/wave/boundary/pml.py")
+
+
+def _load_3d():
+    """The 3D kernel module imports ``pad_model_3d`` from its sibling
+    (``from .acoustic_2d import ...``): load it as a member of a bare package
+    whose path is the reference's kernels directory (the package's own
+    ``__init__`` is not run)."""
+    import types
+    pkg = types.ModuleType("_gb_ref_kernels")
+    pkg.__path__ = [os.path.join(REF, "geobrain/physics/wave/kernels")]
+    sys.modules["_gb_ref_kernels"] = pkg
+    spec = importlib.util.spec_from_file_location(
+        "_gb_ref_kernels.acoustic_3d", os.path.join(REF, "geobrain/physics/wave/kernels/acoustic_3d.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[spec.name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_case_3d(name, *, nz, ny, nx, nabc, fs, dx, dt, nt, vp, rho, src_z, src_y, src_x,
+                wavelet, rcv_z, rcv_y, rcv_x, segments=1):
+    """Forward outputs of the reference's acoustic_forward_kernel_3d."""
+    k3 = _load_3d()
+    vmax = float(np.max(vp))
+    # the propagator asks for free_surface=False (propagators/acoustic.py:140)
+    damp = torch.tensor(pml.bc_pml_3d(nx, ny, nz, dx, dx, dx, nabc, vmax, False),
+                        dtype=torch.float32)
+    L = lambda a: torch.tensor(a, dtype=torch.long)  # noqa: E731
+    t0 = time.time()
+    with torch.no_grad():
+        out = k3.acoustic_forward_kernel_3d(
+            nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, nt=nt, dt=dt, nabc=nabc, free_surface=fs,
+            src_x=L(src_x), src_y=L(src_y), src_z=L(src_z), src_n=len(src_z),
+            src_v=torch.tensor(wavelet, dtype=torch.float32), rcv_x=L(rcv_x), rcv_y=L(rcv_y),
+            rcv_z=L(rcv_z), rcv_n=len(rcv_z), damp=damp,
+            vp=torch.tensor(vp, dtype=torch.float32), rho=torch.tensor(rho, dtype=torch.float32),
+            checkpoint_segments=segments, device=torch.device("cpu"), dtype=torch.float32)
+    meta = dict(name=name, nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=bool(fs), dx=dx, dt=dt,
+                nt=nt, segments=segments, vmax=vmax, torch=torch.__version__)
+    z = dict(in_vp=vp, in_rho=rho, in_damp=damp.numpy(), in_wavelet=wavelet,
+             in_src_z=np.asarray(src_z), in_src_y=np.asarray(src_y), in_src_x=np.asarray(src_x),
+             in_rcv_z=np.asarray(rcv_z), in_rcv_y=np.asarray(rcv_y), in_rcv_x=np.asarray(rcv_x),
+             meta=json.dumps(meta))
+    for k, v in out.items():
+        z["out_" + k] = v.numpy()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **z)
+    print(f"{name}: done in {time.time() - t0:.1f}s")
+
+
+def cases_3d():
+    rng = np.random.default_rng(4321)
+    cases = []
+    for idx, (nz, ny, nx, nabc, fs, S, R, nt, segs, shared) in enumerate([
+        (10, 9, 12, 4, False, 2, 7, 40, 1, False),
+        (9, 8, 11, 4, True, 2, 6, 36, 3, True),    # free surface, shared 1-D wavelet, 3 chunks
+        (8, 10, 9, 3, False, 3, 5, 25, 2, False),  # odd step count
+    ]):
+        vp = (rng.uniform(1500, 3000, (nz, ny, nx))).astype(np.float32)
+        rho = rng.uniform(1800, 2400, (nz, ny, nx)).astype(np.float32)
+        src_z = rng.integers(0, 3, S)
+        src_y = rng.integers(0, ny, S)
+        src_x = rng.integers(0, nx, S)
+        rcv_z = rng.integers(0, nz, R)
+        rcv_y = rng.integers(0, ny, R)
+        rcv_x = rng.integers(0, nx, R)
+        if fs:  # a receiver on the mirrored plane above the surface
+            rcv_z[0] = -1
+        if idx == 2:  # a source on a padded corner cell no update touches
+            src_z[0], src_y[0], src_x[0] = -nabc, -nabc, -nabc
+        w = (rng.standard_normal(nt) if shared else rng.standard_normal((S, nt))).astype(np.float32)
+        cases.append((f"t3d{idx}_fwd", dict(nz=nz, ny=ny, nx=nx, nabc=nabc, fs=fs, dx=10.0,
+                                            dt=1e-3, nt=nt, vp=vp, rho=rho, src_z=src_z,
+                                            src_y=src_y, src_x=src_x, wavelet=w, rcv_z=rcv_z,
+                                            rcv_y=rcv_y, rcv_x=rcv_x, segments=segs)))
+    return cases
 
 
 def damp_for(nz, nx, dx, nabc, vmax):
@@ -267,6 +343,10 @@ def main():
         if args.only and name not in args.only:
             continue
         run_case(name, **kw)
+    for name, kw in cases_3d():
+        if args.only and name not in args.only:
+            continue
+        run_case_3d(name, **kw)
     if not args.only or "marmousi" in args.only:
         marmousi_case()
     if not args.only or "pml" in args.only:
@@ -277,6 +357,10 @@ def main():
                                                         (17, 6, 7.5, 3, 1234.5, True)]):
             out[f"args{i}"] = np.array([nx, nz, dx, n, vmax, float(fs)])
             out[f"pml{i}"] = pml.bc_pml(nx, nz, dx, dx, n, vmax, fs)
+        for i, (nx, ny, nz, dx, n, vmax, fs) in enumerate([(12, 9, 10, 10.0, 4, 3000.0, False),
+                                                            (20, 7, 11, 5.0, 5, 4200.0, True)]):
+            out[f"args3d{i}"] = np.array([nx, ny, nz, dx, n, vmax, float(fs)])
+            out[f"pml3d{i}"] = pml.bc_pml_3d(nx, ny, nz, dx, dx, dx, n, vmax, fs)
         np.savez_compressed(os.path.join(HERE, "pml.npz"), **out)
         print("pml: done")
 
