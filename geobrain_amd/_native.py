@@ -1,4 +1,5 @@
-"""ctypes binding of ``libgbw_acoustic2d.so`` (declared in ``include/gbw_acoustic2d.h``).
+"""ctypes binding of ``libgbw_acoustic2d.so`` (declared in ``include/gbw_acoustic2d.h``
+and ``include/gbw_acoustic3d.h``).
 
 The library is the only compute path: if it is missing, or no ROCm GPU is
 visible, calls raise ``RuntimeError``.  There is no CPU fallback.
@@ -32,6 +33,8 @@ EXPORTS = (
     "gbw_acoustic2d_batch_setup",
     "gbw_acoustic2d_batch_forward",
     "gbw_device_wall_clock_khz",
+    "gbw_acoustic3d_workspace_bytes",
+    "gbw_acoustic3d_forward",
 )
 
 
@@ -69,6 +72,32 @@ class Adjoint(ctypes.Structure):
     _fields_ = [
         ("g_p", c_vp), ("g_vx", c_vp), ("g_vz", c_vp), ("checkpoints", c_vp),
         ("grad_vp", c_vp), ("grad_rho", c_vp), ("grad_src_v", c_vp),
+    ]
+
+
+class Problem3(ctypes.Structure):
+    """``gbw_acoustic3d_problem``."""
+    _fields_ = [
+        ("nx", c_i32), ("ny", c_i32), ("nz", c_i32), ("dx", c_f32), ("dy", c_f32),
+        ("dz", c_f32), ("nt", c_i32), ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32),
+        ("src_n", c_i32), ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
+        ("shots_per_block", c_i32),
+    ]
+
+
+class Inputs3(ctypes.Structure):
+    """``gbw_acoustic3d_inputs``."""
+    _fields_ = [
+        ("src_x", c_vp), ("src_y", c_vp), ("src_z", c_vp), ("rcv_x", c_vp), ("rcv_y", c_vp),
+        ("rcv_z", c_vp), ("src_v", c_vp), ("damp", c_vp), ("vp", c_vp), ("rho", c_vp),
+    ]
+
+
+class Outputs3(ctypes.Structure):
+    """``gbw_acoustic3d_outputs``."""
+    _fields_ = [
+        ("rec_p", c_vp), ("rec_vx", c_vp), ("rec_vy", c_vp), ("rec_vz", c_vp),
+        ("energy_p", c_vp), ("energy_vx", c_vp), ("energy_vy", c_vp), ("energy_vz", c_vp),
     ]
 
 
@@ -130,6 +159,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
             ctypes.POINTER(Problem), ctypes.c_int, c_vp, ctypes.c_size_t, c_vp, c_vp]
         lib.gbw_device_wall_clock_khz.restype = ctypes.c_int
         lib.gbw_device_wall_clock_khz.argtypes = []
+        lib.gbw_acoustic3d_workspace_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic3d_workspace_bytes.argtypes = [ctypes.POINTER(Problem3)]
+        lib.gbw_acoustic3d_forward.restype = ctypes.c_int
+        lib.gbw_acoustic3d_forward.argtypes = [
+            ctypes.POINTER(Problem3), ctypes.POINTER(Inputs3), ctypes.POINTER(Outputs3),
+            c_vp, ctypes.c_size_t, c_vp]
         ver = lib.gbw_abi_version()
         if ver != ABI_VERSION:
             raise RuntimeError(f"geobrain_amd: ABI version {ver} != expected {ABI_VERSION}")

@@ -1,7 +1,8 @@
 """Build recipe for the HIP library (gfx950 only).
 
-``python -m geobrain_amd.build`` compiles ``csrc/gbw_acoustic2d.hip`` into
-``libgbw_acoustic2d.so`` next to this file.  The flags:
+``python -m geobrain_amd.build`` compiles ``csrc/gbw_acoustic2d.hip`` (2D
+propagator, batched forward, shared entry points) and ``csrc/gbw_acoustic3d.hip``
+(3D propagator) into ``libgbw_acoustic2d.so`` next to this file.  The flags:
 
 * ``--offload-arch=gfx950``: MI355X (CDNA4) code objects only.
 * ``-ffp-contract=off``: no fused multiply-add contraction, so every
@@ -19,7 +20,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "gbw_acoustic2d.hip")
+SRCS = [os.path.join(HERE, "csrc", "gbw_acoustic2d.hip"),
+        os.path.join(HERE, "csrc", "gbw_acoustic3d.hip")]
 LIB = os.path.join(HERE, "libgbw_acoustic2d.so")
 
 HIPCC_FLAGS = [
@@ -39,7 +41,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     """Compile the library; returns its path.  Always recompiles."""
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *SRCS]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

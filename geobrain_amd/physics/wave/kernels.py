@@ -329,3 +329,140 @@ def acoustic_forward_kernel(
     if snapshot_every > 0:
         out['snapshots'] = snaps.cpu()
     return out
+
+
+# ---------------------------------------------------------------------------
+# 3D
+# ---------------------------------------------------------------------------
+
+def pad_model_3d(v: Tensor, pml: int, nz: int, ny: int, nx: int,
+                 device: torch.device = torch.device("cpu")) -> Tensor:
+    """Edge-replicating pad of ``(nz, ny, nx)`` to ``(nz + 2 pml, ny + 2 pml, nx + 2 pml)``
+    (``acoustic_2d.py:69-120``): cell ``(i, j, k)`` takes
+    ``v[clamp(i - pml), clamp(j - pml), clamp(k - pml)]``.  The HIP library
+    applies the same map on the fly; this is the tensor form."""
+    ii = torch.arange(nz + 2 * pml, device=v.device).sub(pml).clamp_(0, nz - 1)
+    jj = torch.arange(ny + 2 * pml, device=v.device).sub(pml).clamp_(0, ny - 1)
+    kk = torch.arange(nx + 2 * pml, device=v.device).sub(pml).clamp_(0, nx - 1)
+    return v.to(torch.float32)[ii][:, jj][:, :, kk].to(device)
+
+
+_shots_per_block_3d: int = 0
+
+
+def set_shots_per_block_3d(n: int) -> None:
+    """Shots one thread of the 3D kernels steps through (0 = library default)."""
+    global _shots_per_block_3d
+    _shots_per_block_3d = int(n)
+
+
+def acoustic_forward_kernel_3d(
+    nx: int,
+    ny: int,
+    nz: int,
+    dx: float,
+    dy: float,
+    dz: float,
+    nt: int,
+    dt: float,
+    nabc: int,
+    free_surface: bool,
+    src_x: Tensor,
+    src_y: Tensor,
+    src_z: Tensor,
+    src_n: int,
+    src_v: Tensor,
+    rcv_x: Tensor,
+    rcv_y: Tensor,
+    rcv_z: Tensor,
+    rcv_n: int,
+    damp: Tensor,
+    vp: Tensor,
+    rho: Tensor,
+    checkpoint_segments: int = 1,
+    device: torch.device = torch.device('cpu'),
+    dtype: torch.dtype = torch.float32,
+) -> Dict[str, Tensor]:
+    """3D forward simulation (signature of ``acoustic_3d.py:275-301``).
+
+    Returns ``'p', 'vx', 'vy', 'vz'`` ``(src_n, nt, rcv_n)`` and the energy
+    maps ``'forward_wavefield_p/vx/vy/vz'`` ``(nz, ny, nx)``, summed per
+    ``torch.chunk`` chunk of ``checkpoint_segments`` as the reference sums
+    them (``acoustic_3d.py:433-476``).  All ``nt`` steps run in the HIP library
+    (``gbw_acoustic3d_forward``).
+
+    Forward modelling only: the 3D gradient is not implemented, so a ``vp``,
+    ``rho`` or ``src_v`` that requires grad raises ``NotImplementedError``
+    rather than returning outputs without a graph.
+    """
+    if dtype != torch.float32:
+        raise NotImplementedError("geobrain_amd computes in float32 only (the reference default)")
+    dev = vp.device
+    for name, t in (("vp", vp), ("rho", rho), ("src_v", src_v), ("damp", damp)):
+        _on_gpu(t, name, dev)
+    if torch.is_grad_enabled() and any(t.requires_grad for t in (vp, rho, src_v)):
+        raise NotImplementedError("geobrain_amd: the 3D acoustic gradient is not implemented; "
+                                  "call under torch.no_grad() or detach the inputs")
+    if not torch.cuda.is_available():
+        raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
+    sx, sy, sz = (_host_index(a, n) for a, n in ((src_x, "src_x"), (src_y, "src_y"),
+                                                 (src_z, "src_z")))
+    rx, ry, rz = (_host_index(a, n) for a, n in ((rcv_x, "rcv_x"), (rcv_y, "rcv_y"),
+                                                 (rcv_z, "rcv_z")))
+    if not (sx.shape == sy.shape == sz.shape) or sx.shape[0] != src_n:
+        raise ValueError(f"src_x/src_y/src_z must each have src_n={src_n} entries")
+    if not (rx.shape == ry.shape == rz.shape) or rx.shape[0] != rcv_n:
+        raise ValueError(f"rcv_x/rcv_y/rcv_z must each have rcv_n={rcv_n} entries")
+    if tuple(vp.shape) != (nz, ny, nx) or tuple(rho.shape) != (nz, ny, nx):
+        raise ValueError(f"vp and rho must have shape ({nz}, {ny}, {nx})")
+    padded = (nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
+    if tuple(damp.shape) != padded:
+        raise ValueError(f"damp must have shape {padded}")
+    if src_v.dim() == 1:
+        shared = 1
+        if src_v.shape[0] != nt:
+            raise ValueError(f"src_v has {src_v.shape[0]} samples, nt={nt}")
+    elif src_v.dim() == 2:
+        shared = 0
+        if tuple(src_v.shape) != (src_n, nt):
+            raise ValueError(f"src_v must have shape ({src_n}, {nt}) or ({nt},)")
+    else:
+        raise NotImplementedError("src_v must be (src_n, nt) or (nt,)")
+    lib = _native.load()
+    prob = _native.Problem3()
+    prob.nx, prob.ny, prob.nz = int(nx), int(ny), int(nz)
+    prob.dx, prob.dy, prob.dz = float(dx), float(dy), float(dz)
+    prob.nt, prob.dt = int(nt), float(dt)
+    prob.nabc, prob.free_surface = int(nabc), int(bool(free_surface))
+    prob.src_n, prob.rcv_n = int(src_n), int(rcv_n)
+    prob.wavelet_shared = shared
+    prob.energy_chunk = chunk_length(nt, checkpoint_segments)
+    prob.shots_per_block = _shots_per_block_3d
+    f32 = torch.float32
+    vp_, rho_ = vp.detach().to(f32).contiguous(), rho.detach().to(f32).contiguous()
+    src_v_, damp_ = src_v.detach().to(f32).contiguous(), damp.detach().to(f32).contiguous()
+    ws_bytes = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(prob))
+    if ws_bytes == 0:
+        _native.check(-1)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    out = {k: torch.empty((src_n, nt, rcv_n), dtype=f32, device=dev)
+           for k in ('p', 'vx', 'vy', 'vz')}
+    for k in ('p', 'vx', 'vy', 'vz'):
+        out['forward_wavefield_' + k] = torch.empty((nz, ny, nx), dtype=f32, device=dev)
+    ins = _native.Inputs3()
+    ins.src_x, ins.src_y, ins.src_z = sx.ctypes.data, sy.ctypes.data, sz.ctypes.data
+    if rx.size:
+        ins.rcv_x, ins.rcv_y, ins.rcv_z = rx.ctypes.data, ry.ctypes.data, rz.ctypes.data
+    ins.src_v, ins.damp = src_v_.data_ptr(), damp_.data_ptr()
+    ins.vp, ins.rho = vp_.data_ptr(), rho_.data_ptr()
+    outs = _native.Outputs3()
+    outs.rec_p, outs.rec_vx = out['p'].data_ptr(), out['vx'].data_ptr()
+    outs.rec_vy, outs.rec_vz = out['vy'].data_ptr(), out['vz'].data_ptr()
+    outs.energy_p = out['forward_wavefield_p'].data_ptr()
+    outs.energy_vx = out['forward_wavefield_vx'].data_ptr()
+    outs.energy_vy = out['forward_wavefield_vy'].data_ptr()
+    outs.energy_vz = out['forward_wavefield_vz'].data_ptr()
+    rc = lib.gbw_acoustic3d_forward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(outs),
+                                    ws.data_ptr(), ws_bytes, _stream_handle(dev))
+    _native.check(rc)
+    return out

@@ -18,8 +18,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared_functions():
-    text = open(os.path.join(ROOT, "include", "gbw_acoustic2d.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gbw_\w+)\s*\(", text, re.M)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gbw_\w+)\s*\(", text, re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -178,3 +182,75 @@ def test_pad_model_tensor_form():
     v = torch.arange(12, dtype=torch.float32).reshape(3, 4)
     p = wave.pad_model(v, 2, 3, 4)
     assert p.shape == (7, 8) and p[0, 0] == 0 and p[-1, -1] == 11 and p[3, 0] == 4
+
+
+def test_bc_pml_3d_matches_reference():
+    z = np.load(os.path.join(ROOT, "tests", "golden", "pml.npz"))
+    for i in range(2):
+        nx, ny, nz, dx, n, vmax, fs = z[f"args3d{i}"]
+        got = wave.bc_pml_3d(int(nx), int(ny), int(nz), dx, dx, dx, int(n), vmax, bool(fs))
+        np.testing.assert_array_equal(got, z[f"pml3d{i}"])
+        got = wave.create_boundary('pml', int(nx), int(nz), dx, dx, int(n), vmax=vmax,
+                                   free_surface=bool(fs), ny=int(ny), dy=dx)
+        np.testing.assert_array_equal(got, z[f"pml3d{i}"])
+    # the 3D fixtures' damping arrays came from the reference's bc_pml_3d
+    for name in ("t3d0_fwd", "t3d2_fwd"):
+        zz, m = load(name)
+        d = wave.bc_pml_3d(m["nx"], m["ny"], m["nz"], m["dx"], m["dx"], m["dx"], m["nabc"],
+                           m["vmax"], False)
+        np.testing.assert_array_equal(d.astype(np.float32), zz["in_damp"])
+
+
+def test_create_boundary_3d_errors():
+    with pytest.raises(ValueError, match="vmax"):
+        wave.create_boundary('pml', 10, 10, 5.0, 5.0, 4, ny=10, dy=5.0)
+    with pytest.raises(ValueError, match="dy"):
+        wave.create_boundary('pml', 10, 10, 5.0, 5.0, 4, vmax=3000.0, ny=10)
+    with pytest.raises(NotImplementedError, match="3-D boundary type"):
+        wave.create_boundary('gerjan', 10, 10, 5.0, 5.0, 4, ny=10, dy=5.0)
+
+
+def test_pad_model_3d_tensor_form():
+    v = torch.arange(24, dtype=torch.float32).reshape(2, 3, 4)
+    p = wave.pad_model_3d(v, 2, 2, 3, 4)
+    assert p.shape == (6, 7, 8)
+    i = np.clip(np.arange(6) - 2, 0, 1)
+    j = np.clip(np.arange(7) - 2, 0, 2)
+    k = np.clip(np.arange(8) - 2, 0, 3)
+    np.testing.assert_array_equal(p.numpy(), v.numpy()[np.ix_(i, j, k)])
+
+
+def _problem3(**kw):
+    p = _native.Problem3()
+    base = dict(nx=100, ny=80, nz=60, dx=10.0, dy=10.0, dz=10.0, nt=500, dt=1e-3, nabc=10,
+                free_surface=0, src_n=4, rcv_n=50, wavelet_shared=0, energy_chunk=0,
+                shots_per_block=0)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_workspace_queries_3d():
+    lib = _native.load()
+    ws = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3()))
+    plane = 80 * 100 * 128 * 4  # NZ * NY * (NX rounded up to 64) floats
+    assert ws >= 6 * plane + 4 * 4 * plane + 4 * 60 * 80 * 100 * 4
+    for bad, msg in ((dict(ny=0), "Grid dimensions must be positive"),
+                     (dict(nabc=0), "nabc"), (dict(nt=0), "nt must be positive"),
+                     (dict(src_n=0), "src_n"), (dict(dt=0.0), "dt and dz")):
+        assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(**bad))) == 0
+        assert msg in _native.last_error()
+
+
+def test_product_path_3d_refuses_cpu_tensors():
+    z, m = load("t3d0_fwd")
+    T = torch.tensor
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        wave.acoustic_forward_kernel_3d(
+            nx=m["nx"], ny=m["ny"], nz=m["nz"], dx=m["dx"], dy=m["dx"], dz=m["dx"], nt=m["nt"],
+            dt=m["dt"], nabc=m["nabc"], free_surface=m["free_surface"],
+            src_x=T(z["in_src_x"]), src_y=T(z["in_src_y"]), src_z=T(z["in_src_z"]),
+            src_n=len(z["in_src_x"]), src_v=T(z["in_wavelet"]), rcv_x=T(z["in_rcv_x"]),
+            rcv_y=T(z["in_rcv_y"]), rcv_z=T(z["in_rcv_z"]), rcv_n=len(z["in_rcv_x"]),
+            damp=T(z["in_damp"]), vp=T(z["in_vp"]), rho=T(z["in_rho"]))
