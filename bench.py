@@ -241,6 +241,96 @@ def uneven_batch(steps: int, warmup: int, tune: bool, launch=None) -> dict:
                              "avg_launch_us": round(kt[1] / max(kt[0], 1) * 1e3, 2)}}
 
 
+def workload_3d(n_shots=8, n=200, nt=1000, nabc=20, dx=10.0, dt=8e-4, f0=10.0):
+    """3D forward: n^3 grid (+nabc per side), depth-linear 1500-4500 m/s plus a
+    3D Gaussian random field (lh 200 m, lv 50 m, std 200, seed 0) clipped to
+    [1500, 4700], Gardner density, shots along the y = n/2 line at z = 2,
+    receivers every 4th cell of the z = 2 plane, 10 Hz Ricker."""
+    from geobrain_amd import synthetic
+    from geobrain_amd.physics import wave
+    vp = synthetic.grf_model_3d(n, n, n, dx, seed=0)
+    rr = np.arange(0, n, 4, dtype=np.int64)
+    ry, rx = np.meshgrid(rr, rr, indexing="ij")
+    return dict(
+        nz=n, ny=n, nx=n, nt=nt, nabc=nabc, dx=dx, dt=dt, vp=vp, rho=synthetic.gardner(vp),
+        damp=wave.bc_pml_3d(n, n, n, dx, dx, dx, nabc, float(vp.max()), False).astype(np.float32),
+        wavelet=np.tile(synthetic.ricker_padded(f0, dt, nt), (n_shots, 1)),
+        src_x=synthetic.surface_shots(n, n_shots), src_y=np.full(n_shots, n // 2, np.int64),
+        src_z=np.full(n_shots, 2, np.int64), rcv_x=rx.reshape(-1), rcv_y=ry.reshape(-1),
+        rcv_z=np.full(rx.size, 2, np.int64))
+
+
+def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 60) -> dict:
+    """Times acoustic_forward_kernel_3d (all nt steps = one bench step) on
+    ``workload_3d``; roofline of the time step at its algorithmic 32 B per
+    cell-update (p, vx, vy, vz each read and written once)."""
+    import torch
+    from geobrain_amd import _native
+    from geobrain_amd.physics import wave
+    dev = torch.device("cuda:0")
+    w = workload_3d()
+    S, nt = w["wavelet"].shape
+    T = lambda a, **kw: torch.as_tensor(a, device=dev, **kw)  # noqa: E731
+    args = dict(nx=w["nx"], ny=w["ny"], nz=w["nz"], dx=w["dx"], dy=w["dx"], dz=w["dx"], nt=nt,
+                dt=w["dt"], nabc=w["nabc"], free_surface=False, src_x=w["src_x"],
+                src_y=w["src_y"], src_z=w["src_z"], src_n=S, src_v=T(w["wavelet"]),
+                rcv_x=w["rcv_x"], rcv_y=w["rcv_y"], rcv_z=w["rcv_z"], rcv_n=w["rcv_x"].size,
+                damp=T(w["damp"]), vp=T(w["vp"]), rho=T(w["rho"]))
+    for _ in range(warmup):
+        wave.acoustic_forward_kernel_3d(**args)
+    torch.cuda.synchronize()
+    _native.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = wave.acoustic_forward_kernel_3d(**args)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    kt = _native.kernel_times()
+    _native.set_kernel_timing(False)
+    pad = [w["nz"] + 2 * w["nabc"], w["ny"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]]
+    cells = S * pad[0] * pad[1] * pad[2]
+    step_kernels = [k for k in kt if k in ("p3_step_kernel", "v3_step_kernel", "fwd3_kernel")]
+    kernels = {k: {"launches": n, "total_s": round(ms / 1e3, 4),
+                   "avg_launch_us": round(ms / n * 1e3, 2)} for k, (n, ms) in kt.items()}
+    # one time step = one launch of each step kernel
+    step_us = sum(kernels[k]["avg_launch_us"] for k in step_kernels)
+    b_step = 32.0 * cells
+    achieved = b_step / (step_us * 1e-6) / 1e9
+    rec = {
+        "metric": "Gcell-updates/s, 3D forward (seismograms + energy maps)",
+        "value": round(cells * nt / wall / 1e9, 3), "unit": "Gcell-updates/s",
+        "ms_per_step": round(wall * 1e3, 2), "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"3D forward: {S} shots, {w['nz']}^3 grid (+{w['nabc']} PML), "
+                               f"{nt} steps, {w['rcv_x'].size} receivers",
+                   "shots": S, "grid": [w["nz"], w["ny"], w["nx"]], "padded": pad, "nt": nt},
+        "roofline": {"bound": "hbm", "kernel": "+".join(step_kernels), "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_time_step_us": round(step_us, 2),
+                     "bytes_per_time_step": b_step},
+        "kernels": kernels,
+        "check": {"p_max": float(out["p"].abs().max())},
+        "cpu_baseline": None,
+    }
+    if cpu:
+        import oracle
+        oracle.build()
+        sl = slice(0, 1)
+        t0 = time.perf_counter()
+        oracle.forward3d(nz=w["nz"], ny=w["ny"], nx=w["nx"], nabc=w["nabc"], free_surface=False,
+                         dt=w["dt"], dz=w["dx"], src_z=w["src_z"][sl], src_y=w["src_y"][sl],
+                         src_x=w["src_x"][sl], src_v=w["wavelet"][sl, :cpu_steps],
+                         rcv_z=w["rcv_z"], rcv_y=w["rcv_y"], rcv_x=w["rcv_x"], vp=w["vp"],
+                         rho=w["rho"], damp=w["damp"], precision="f32")
+        t = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rec["cpu_baseline"] = {
+            "value": round(pad[0] * pad[1] * pad[2] * cpu_steps / t / 1e9, 5),
+            "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 C++ 3D oracle (OpenMP), forward, 1 shot x {cpu_steps} steps of "
+                      f"the same grid, {t:.1f} s wall"}
+    return rec
+
+
 def fwi_precision(n_iter: int = 20, shots: int = 128, nz: int = 400, nx: int = 2000,
                   nt: int = 5000) -> dict:
     """The precision workload: FWI of a 400 x 2000 layered Gaussian-random model
@@ -342,15 +432,17 @@ def main():
                     help="pin a launch choice instead of autotuning: comma-separated "
                          "tile_rows,shots_per_block,march,bwd_tile_rows,bwd_shots_per_block,"
                          "steps_per_launch")
-    ap.add_argument("--workload", choices=["gradient", "fwi-bf16"], default="gradient",
+    ap.add_argument("--workload", choices=["gradient", "fwi-bf16", "fwd3d"], default="gradient",
                     help="gradient: the 64-shot batched gradient (default, plus the uneven "
-                         "batch); fwi-bf16: the BF16-checkpoint FWI precision study")
+                         "batch and the 3D forward); fwi-bf16: the BF16-checkpoint FWI "
+                         "precision study; fwd3d: the 3D forward alone")
     ap.add_argument("--fwi-iters", type=int, default=20)
     ap.add_argument("--batch-launch", type=str, default=None,
                     help="pin the uneven batch's launch choice instead of autotuning: "
                          "shots_per_block,steps_per_launch,tile_rows")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -375,6 +467,12 @@ def main():
 
     if args.workload == "fwi-bf16":
         print(json.dumps(fwi_precision(n_iter=args.fwi_iters)), flush=True)
+        return
+    if args.workload == "fwd3d":
+        rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)
+        rec.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
+        print(json.dumps(rec), flush=True)
         return
     dev = torch.device("cuda:0")
     w = workload(n_shots=args.shots, nt=args.nt)
@@ -509,6 +607,9 @@ def main():
             blaunch = B.BatchLaunch(*[int(v) for v in args.batch_launch.split(",")])
         rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune,
                                            launch=blaunch)
+    if not args.no_3d:
+        torch.cuda.empty_cache()
+        rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu)
     print(json.dumps(rec), flush=True)
 
 

@@ -11,6 +11,8 @@ on the host; results are float32 numpy arrays (velocities in m/s).
   deviation, clipped to a velocity range.  The field is synthesised by
   filtering white noise in the Fourier domain on a grid padded by three
   correlation lengths, so it does not wrap around.
+* ``grf_model_3d``: the same construction on a ``(nz, ny, nx)`` grid, the
+  lateral length ``lh`` applying to both horizontal axes.
 * ``smooth_model``: Gaussian smoothing with sigma a fraction of ``min(nz, nx)``.
 * ``ricker_padded``: the propagator's Ricker wavelet zero-padded to ``nt``.
 * ``gardner``: density ``310 vp^0.25`` as ``AcousticModel`` computes it.
@@ -48,6 +50,24 @@ def grf_model(nz, nx, dx, seed=0, v_top=1500.0, v_bottom=4500.0, lh=300.0, lv=50
     field = field[oz:oz + nz, ox:ox + nx]
     field = (field - field.mean()) / field.std() * std
     depth = np.linspace(0.0, 1.0, nz)[:, None]
+    vp = v_top + (v_bottom - v_top) * depth + field
+    return np.clip(vp, vclip[0], vclip[1]).astype(np.float32)
+
+
+def grf_model_3d(nz, ny, nx, dx, seed=0, v_top=1500.0, v_bottom=4500.0, lh=200.0, lv=50.0,
+                 std=200.0, vclip=(1500.0, 4700.0)):
+    rng = np.random.default_rng(seed)
+    ez, eh = int(np.ceil(3 * lv / dx)), int(np.ceil(3 * lh / dx))
+    pz, py, px = nz + 2 * ez, ny + 2 * eh, nx + 2 * eh
+    noise = rng.standard_normal((pz, py, px))
+    kz = np.fft.fftfreq(pz, d=dx)[:, None, None]
+    ky = np.fft.fftfreq(py, d=dx)[None, :, None]
+    kx = np.fft.rfftfreq(px, d=dx)[None, None, :]
+    spec = np.exp(-np.pi ** 2 * (kz ** 2 * lv ** 2 + (ky ** 2 + kx ** 2) * lh ** 2))
+    field = np.fft.irfftn(np.fft.rfftn(noise) * np.sqrt(spec), s=(pz, py, px))
+    field = field[ez:ez + nz, eh:eh + ny, eh:eh + nx]
+    field = (field - field.mean()) / field.std() * std
+    depth = np.linspace(0.0, 1.0, nz)[:, None, None]
     vp = v_top + (v_bottom - v_top) * depth + field
     return np.clip(vp, vclip[0], vclip[1]).astype(np.float32)
 

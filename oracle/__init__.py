@@ -15,8 +15,9 @@ Contents
 * ``ricker``: float64 closed form of ``wavelets/ricker.py:82-93``.
 * ``gardner``: ``models/acoustic.py:169`` (``310 * vp ** 0.25`` in float32 numpy).
 * ``chunk_len``: the chunk size ``torch.chunk`` uses (``acoustic_2d.py:430``).
-* ``forward3d``: ctypes front-end to ``acoustic3d_oracle.cpp``, a C++
-  restatement of ``geobrain/physics/wave/kernels/acoustic_3d.py`` (forward).
+* ``forward3d`` / ``gradient3d``: ctypes front-ends to ``acoustic3d_oracle.cpp``,
+  a C++ restatement of ``geobrain/physics/wave/kernels/acoustic_3d.py``
+  (forward, and the discrete adjoint of its step for the gradient).
 * ``bc_pml_3d``: numpy restatement of ``geobrain/physics/wave/boundary/pml.py:101-195``.
 * ``pad_model_3d``: clamp-index restatement of ``kernels/acoustic_2d.py:70-120``.
 """
@@ -69,6 +70,12 @@ def _load():
         for name in ("oracle3_forward_f32", "oracle3_forward_f64"):
             fn = getattr(lib, name)
             fn.argtypes = fwd3_args
+            fn.restype = I
+        grad3_args = [I, I, I, I, I, D, D, I, I, P, P, P, P, I, I, P, P, P, P, P, P, I,
+                      P, P, P, P, P, P, P]
+        for name in ("oracle3_gradient_f32", "oracle3_gradient_f64"):
+            fn = getattr(lib, name)
+            fn.argtypes = grad3_args
             fn.restype = I
         _lib = lib
     return _lib
@@ -272,6 +279,49 @@ def forward3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_v
     if return_state:
         out["state"] = state.reshape(4, S, nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
     return out
+
+
+def gradient3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_v, rcv_z, rcv_y,
+               rcv_x, vp, rho, damp, g_p=None, g_vx=None, g_vy=None, g_vz=None, chunk=0,
+               precision="f64"):
+    """Gradient of ``sum(g_p*p + g_vx*vx + g_vy*vy + g_vz*vz)`` of the 3D
+    forward w.r.t. vp, rho and the wavelet (``src_v`` of shape (S, nt), or
+    (nt,) shared by every shot, whose gradient then has shape (nt,)).
+    Returns ``(grad_vp, grad_rho, grad_w)``."""
+    idx = [np.ascontiguousarray(a, dtype=np.int64).reshape(-1)
+           for a in (src_z, src_y, src_x, rcv_z, rcv_y, rcv_x)]
+    S, R = idx[0].shape[0], idx[3].shape[0]
+    src_v = np.ascontiguousarray(src_v, dtype=np.float32)
+    shared = src_v.ndim == 1
+    nt = src_v.shape[-1]
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    rho = np.ascontiguousarray(rho, dtype=np.float32)
+    damp = np.ascontiguousarray(damp, dtype=np.float32)
+    assert vp.shape == (nz, ny, nx) and rho.shape == (nz, ny, nx)
+    assert damp.shape == (nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
+    dtp = np.float32 if precision == "f32" else np.float64
+
+    def conv(g):
+        if g is None:
+            return None
+        g = np.ascontiguousarray(g, dtype=dtp)
+        assert g.shape == (S, nt, R)
+        return g
+
+    gs = [conv(g) for g in (g_p, g_vx, g_vy, g_vz)]
+    gvp = np.zeros((nz, ny, nx), dtp)
+    grho = np.zeros((nz, ny, nx), dtp)
+    gw = np.zeros((nt,) if shared else (S, nt), dtp)
+    fn = getattr(_load(), "oracle3_gradient_" + precision)
+    rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+            *[_ptr(a) for a in idx[:3]], _ptr(src_v), int(shared), R,
+            *[_ptr(a) for a in idx[3:]], _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
+            *[_ptr(g) for g in gs], _ptr(gvp), _ptr(grho), _ptr(gw))
+    if rc == -2 or rc == -3:
+        raise IndexError("oracle3: source or receiver index out of range")
+    if rc:
+        raise ValueError(f"oracle3: invalid arguments ({rc})")
+    return gvp, grho, gw
 
 
 def bc_pml_3d(nx, ny, nz, dx, dy, dz, pml, vmax, free_surface=True):

@@ -47,6 +47,7 @@ def adjoint_sources(z, m, fwd_p=None, obs_p=None):
 
 
 FWD3D_CASES = ["t3d0_fwd", "t3d1_fwd", "t3d2_fwd"]
+GRAD3D_CASES = ["t3d0_grad", "t3d1_grad", "t3d2_grad"]
 OUT3D_KEYS = ("p", "vx", "vy", "vz", "forward_wavefield_p", "forward_wavefield_vx",
               "forward_wavefield_vy", "forward_wavefield_vz")
 
@@ -57,3 +58,10 @@ def oracle3d_args(z, m):
                 src_x=z["in_src_x"], src_v=z["in_wavelet"], rcv_z=z["in_rcv_z"],
                 rcv_y=z["in_rcv_y"], rcv_x=z["in_rcv_x"], vp=z["in_vp"], rho=z["in_rho"],
                 damp=z["in_damp"])
+
+
+def adjoint_sources_3d(z, m):
+    """The seeded adjoint sources (g_p, g_vx, g_vy, g_vz) of a 3D gradient fixture."""
+    S, R, nt = len(z["in_src_z"]), len(z["in_rcv_z"]), m["nt"]
+    rng = np.random.default_rng(m["g_seed"])
+    return tuple(rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(4))

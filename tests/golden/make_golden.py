@@ -68,31 +68,53 @@ def _load_3d():
 
 
 def run_case_3d(name, *, nz, ny, nx, nabc, fs, dx, dt, nt, vp, rho, src_z, src_y, src_x,
-                wavelet, rcv_z, rcv_y, rcv_x, segments=1):
-    """Forward outputs of the reference's acoustic_forward_kernel_3d."""
+                wavelet, rcv_z, rcv_y, rcv_x, segments=1, g_seed=None, rho_grad=False,
+                w_grad=False):
+    """Outputs of the reference's acoustic_forward_kernel_3d; with ``g_seed``,
+    also the autograd gradient of ``sum(g_p p + g_vx vx + g_vy vy + g_vz vz)``
+    for seeded standard-normal ``g_*`` w.r.t. vp (and rho, src_v on request)."""
     k3 = _load_3d()
     vmax = float(np.max(vp))
     # the propagator asks for free_surface=False (propagators/acoustic.py:140)
     damp = torch.tensor(pml.bc_pml_3d(nx, ny, nz, dx, dx, dx, nabc, vmax, False),
                         dtype=torch.float32)
     L = lambda a: torch.tensor(a, dtype=torch.long)  # noqa: E731
+    vp_t = torch.tensor(vp, dtype=torch.float32)
+    rho_t = torch.tensor(rho, dtype=torch.float32)
+    w_t = torch.tensor(wavelet, dtype=torch.float32)
+    S, R = len(src_z), len(rcv_z)
     t0 = time.time()
-    with torch.no_grad():
+    grad = g_seed is not None
+    if grad:
+        vp_t.requires_grad_(True)
+        rho_t.requires_grad_(rho_grad)
+        w_t.requires_grad_(w_grad)
+    with torch.set_grad_enabled(grad):
         out = k3.acoustic_forward_kernel_3d(
             nx=nx, ny=ny, nz=nz, dx=dx, dy=dx, dz=dx, nt=nt, dt=dt, nabc=nabc, free_surface=fs,
-            src_x=L(src_x), src_y=L(src_y), src_z=L(src_z), src_n=len(src_z),
-            src_v=torch.tensor(wavelet, dtype=torch.float32), rcv_x=L(rcv_x), rcv_y=L(rcv_y),
-            rcv_z=L(rcv_z), rcv_n=len(rcv_z), damp=damp,
-            vp=torch.tensor(vp, dtype=torch.float32), rho=torch.tensor(rho, dtype=torch.float32),
-            checkpoint_segments=segments, device=torch.device("cpu"), dtype=torch.float32)
+            src_x=L(src_x), src_y=L(src_y), src_z=L(src_z), src_n=S, src_v=w_t,
+            rcv_x=L(rcv_x), rcv_y=L(rcv_y), rcv_z=L(rcv_z), rcv_n=R, damp=damp,
+            vp=vp_t, rho=rho_t, checkpoint_segments=segments, device=torch.device("cpu"),
+            dtype=torch.float32)
     meta = dict(name=name, nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=bool(fs), dx=dx, dt=dt,
                 nt=nt, segments=segments, vmax=vmax, torch=torch.__version__)
     z = dict(in_vp=vp, in_rho=rho, in_damp=damp.numpy(), in_wavelet=wavelet,
              in_src_z=np.asarray(src_z), in_src_y=np.asarray(src_y), in_src_x=np.asarray(src_x),
-             in_rcv_z=np.asarray(rcv_z), in_rcv_y=np.asarray(rcv_y), in_rcv_x=np.asarray(rcv_x),
-             meta=json.dumps(meta))
+             in_rcv_z=np.asarray(rcv_z), in_rcv_y=np.asarray(rcv_y), in_rcv_x=np.asarray(rcv_x))
+    if grad:
+        rng = np.random.default_rng(g_seed)
+        gs = [rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(4)]
+        loss = sum((torch.tensor(g) * out[k]).sum() for g, k in zip(gs, ("p", "vx", "vy", "vz")))
+        loss.backward()
+        meta.update(g_seed=g_seed, loss=float(loss.detach()))
+        z["grad_vp"] = vp_t.grad.numpy()
+        if rho_grad:
+            z["grad_rho"] = rho_t.grad.numpy()
+        if w_grad:
+            z["grad_w"] = w_t.grad.numpy()
+    z["meta"] = json.dumps(meta)
     for k, v in out.items():
-        z["out_" + k] = v.numpy()
+        z["out_" + k] = v.detach().numpy()
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **z)
     print(f"{name}: done in {time.time() - t0:.1f}s")
 
@@ -122,6 +144,11 @@ def cases_3d():
                                             dt=1e-3, nt=nt, vp=vp, rho=rho, src_z=src_z,
                                             src_y=src_y, src_x=src_x, wavelet=w, rcv_z=rcv_z,
                                             rcv_y=rcv_y, rcv_x=rcv_x, segments=segs)))
+    # gradients: the same three problems (new draws of the adjoint sources)
+    for idx in range(3):
+        name, kw = cases[idx]
+        cases.append((f"t3d{idx}_grad", dict(kw, g_seed=200 + idx, rho_grad=(idx != 0),
+                                             w_grad=True)))
     return cases
 
 

@@ -1,14 +1,17 @@
 """Pins the 3D CPU oracle: against the reference's own golden vectors
-(acoustic_forward_kernel_3d, bc_pml_3d), and against what the mathematics fixes
-independently (linearity, zero wavelet, mirror symmetry in y and x, the
-y <-> x exchange of a model symmetric under it, translation invariance before
-boundary contact, the energy-map / record identity).  CPU only."""
+(acoustic_forward_kernel_3d forward and its autograd gradient, bc_pml_3d), and
+against what the mathematics fixes independently (linearity, zero wavelet,
+mirror symmetry in y and x, the y <-> x exchange of a model symmetric under it,
+translation invariance before boundary contact, the energy-map / record
+identity; for the gradient, the dot-product test of the adjoint and central
+differences in FP64).  CPU only."""
 
 import numpy as np
 import pytest
 
 import oracle
-from tests._fixtures import FWD3D_CASES, OUT3D_KEYS, chunk_len, load, oracle3d_args, rel_l2
+from tests._fixtures import (FWD3D_CASES, GRAD3D_CASES, OUT3D_KEYS, adjoint_sources_3d,
+                             chunk_len, load, oracle3d_args, rel_l2)
 
 
 @pytest.mark.parametrize("name", FWD3D_CASES)
@@ -120,3 +123,72 @@ def test_oracle3d_energy_equals_sum_of_squared_records():
     for k in ("p", "vx", "vy", "vz"):
         e = out["forward_wavefield_" + k][rz, ry, rx]
         np.testing.assert_allclose(e, (out[k] ** 2).sum(axis=(0, 1)), rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", GRAD3D_CASES)
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_oracle3d_gradient_matches_reference_autograd(name, precision):
+    z, m = load(name)
+    g = adjoint_sources_3d(z, m)
+    gv, gr, gw = oracle.gradient3d(**oracle3d_args(z, m), g_p=g[0], g_vx=g[1], g_vy=g[2],
+                                   g_vz=g[3], chunk=chunk_len(m["nt"], m["segments"]),
+                                   precision=precision)
+    assert rel_l2(gv, z["grad_vp"]) < 1e-5
+    if "grad_rho" in z:
+        assert rel_l2(gr, z["grad_rho"]) < 1e-5
+    assert rel_l2(gw, z["grad_w"]) < 1e-5
+
+
+def _grad_case(seed, fs, nt=40, S=2, R=6, nz=9, ny=8, nx=10, nabc=3):
+    rng = np.random.default_rng(seed)
+    vp = rng.uniform(1600, 3000, (nz, ny, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, ny, nx)).astype(np.float32)
+    damp = oracle.bc_pml_3d(nx, ny, nz, 10.0, 10.0, 10.0, nabc, float(vp.max()), False)
+    return dict(nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=fs, dt=1e-3, dz=10.0,
+                src_z=rng.integers(0, nz, S), src_y=rng.integers(0, ny, S),
+                src_x=rng.integers(0, nx, S),
+                src_v=rng.standard_normal((S, nt)).astype(np.float32),
+                rcv_z=rng.integers(-1 if fs else 0, nz, R), rcv_y=rng.integers(0, ny, R),
+                rcv_x=rng.integers(0, nx, R), vp=vp, rho=rho, damp=damp.astype(np.float32))
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_oracle3d_dot_product(fs):
+    """<F w, g> == <w, F^T g>: the adjoint propagation is the exact transpose."""
+    a = _grad_case(3, fs)
+    S, nt = a["src_v"].shape
+    R = len(a["rcv_z"])
+    rng = np.random.default_rng(5)
+    gs = [rng.standard_normal((S, nt, R)) for _ in range(4)]
+    out = oracle.forward3d(**a, precision="f64")
+    lhs = sum(float(np.sum(out[k] * g)) for k, g in zip(("p", "vx", "vy", "vz"), gs))
+    _, _, gw = oracle.gradient3d(**a, g_p=gs[0], g_vx=gs[1], g_vy=gs[2], g_vz=gs[3], chunk=7,
+                                 precision="f64")
+    rhs = float(np.sum(a["src_v"].astype(np.float64) * gw))
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_oracle3d_gradient_finite_differences(fs):
+    """Central differences in FP64 on vp and rho: interior cells, a corner cell
+    (its gradient folds the padded cells it is copied to) and an edge cell."""
+    a = _grad_case(11, fs, nt=36)
+    S, nt = a["src_v"].shape
+    R = len(a["rcv_z"])
+    rng = np.random.default_rng(2)
+    gs = [rng.standard_normal((S, nt, R)) for _ in range(4)]
+
+    def loss(**kw):
+        out = oracle.forward3d(**dict(a, **kw), precision="f64")
+        return sum(float(np.sum(out[k] * g)) for k, g in zip(("p", "vx", "vy", "vz"), gs))
+
+    gv, gr, _ = oracle.gradient3d(**a, g_p=gs[0], g_vx=gs[1], g_vy=gs[2], g_vz=gs[3], chunk=8,
+                                  precision="f64")
+    # the oracle reads float32 model values: perturb by exactly representable steps
+    for cell in [(0, 0, 0), (4, 3, 5), (8, 7, 9), (2, 0, 6), (6, 5, 1)]:
+        for name, grad, h in (("vp", gv, 0.5), ("rho", gr, 0.25)):
+            up, dn = a[name].copy(), a[name].copy()
+            up[cell] += h
+            dn[cell] -= h
+            fd = (loss(**{name: up}) - loss(**{name: dn})) / (2 * h)
+            assert abs(fd - grad[cell]) <= 1e-5 * abs(grad).max(), (name, cell, fd, grad[cell])
