@@ -34,7 +34,9 @@ EXPORTS = (
     "gbw_acoustic2d_batch_forward",
     "gbw_device_wall_clock_khz",
     "gbw_acoustic3d_workspace_bytes",
+    "gbw_acoustic3d_checkpoint_bytes",
     "gbw_acoustic3d_forward",
+    "gbw_acoustic3d_backward",
 )
 
 
@@ -81,7 +83,7 @@ class Problem3(ctypes.Structure):
         ("nx", c_i32), ("ny", c_i32), ("nz", c_i32), ("dx", c_f32), ("dy", c_f32),
         ("dz", c_f32), ("nt", c_i32), ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32),
         ("src_n", c_i32), ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
-        ("shots_per_block", c_i32),
+        ("shots_per_block", c_i32), ("checkpoint_every", c_i32),
     ]
 
 
@@ -98,6 +100,15 @@ class Outputs3(ctypes.Structure):
     _fields_ = [
         ("rec_p", c_vp), ("rec_vx", c_vp), ("rec_vy", c_vp), ("rec_vz", c_vp),
         ("energy_p", c_vp), ("energy_vx", c_vp), ("energy_vy", c_vp), ("energy_vz", c_vp),
+        ("checkpoints", c_vp),
+    ]
+
+
+class Adjoint3(ctypes.Structure):
+    """``gbw_acoustic3d_adjoint``."""
+    _fields_ = [
+        ("g_p", c_vp), ("g_vx", c_vp), ("g_vy", c_vp), ("g_vz", c_vp), ("checkpoints", c_vp),
+        ("grad_vp", c_vp), ("grad_rho", c_vp), ("grad_src_v", c_vp),
     ]
 
 
@@ -160,10 +171,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_device_wall_clock_khz.restype = ctypes.c_int
         lib.gbw_device_wall_clock_khz.argtypes = []
         lib.gbw_acoustic3d_workspace_bytes.restype = ctypes.c_size_t
-        lib.gbw_acoustic3d_workspace_bytes.argtypes = [ctypes.POINTER(Problem3)]
+        lib.gbw_acoustic3d_workspace_bytes.argtypes = [ctypes.POINTER(Problem3), ctypes.c_int]
+        lib.gbw_acoustic3d_checkpoint_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic3d_checkpoint_bytes.argtypes = [ctypes.POINTER(Problem3)]
         lib.gbw_acoustic3d_forward.restype = ctypes.c_int
         lib.gbw_acoustic3d_forward.argtypes = [
             ctypes.POINTER(Problem3), ctypes.POINTER(Inputs3), ctypes.POINTER(Outputs3),
+            c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_acoustic3d_backward.restype = ctypes.c_int
+        lib.gbw_acoustic3d_backward.argtypes = [
+            ctypes.POINTER(Problem3), ctypes.POINTER(Inputs3), ctypes.POINTER(Adjoint3),
             c_vp, ctypes.c_size_t, c_vp]
         ver = lib.gbw_abi_version()
         if ver != ABI_VERSION:

@@ -130,6 +130,8 @@ struct Step3Args {
   const int* src_cell;     // [S] padded cell index in the plane
   const float* wav; long long wav_stride;
   float* energy;           // [group][4][nz][ny][nx] or null
+  float* dstore;           // recompute: [shot][plane] p-update bracket of this step, or null
+  float* pstore;           // recompute: [shot][plane] p after this step's update, or null
 };
 
 // p on z in [f+1, NZ-3], y in [2, NY-3], x in [2, NX-3]; then the source
@@ -162,8 +164,10 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
     const float4* VZ = reinterpret_cast<const float4*>(a.state + 3 * a.fstride + (long long)s * d.plane);
     const int sc = a.src_cell[s];
     const bool src_here = (long long)(sc >> 2) == q4;
-    if (!mask && !src_here) continue;
+    const bool store = a.dstore != nullptr;
+    if (!mask && !src_here && !store) continue;
     float4 p = P[q4];
+    float4 dsum = make_float4(0.f, 0.f, 0.f, 0.f);
     if (mask) {
       const float4 xa = VX[gl], xb = VX[q4], xc = VX[gr];
       const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
@@ -177,6 +181,7 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
         const float dvz = d4(g3(z00, e), g3(zm1, e), g3(zp1, e), g3(zm2, e));
         float sum = dvx + dvy;
         sum = sum + dvz;
+        s3(dsum, e, sum);
         const float u = g3(omk1, e) * g3(p, e);
         s3(p, e, u - g3(a1, e) * sum);
       }
@@ -186,6 +191,10 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
       s3(p, e, g3(p, e) + a.dt * a.wav[(long long)s * a.wav_stride + a.it]);
     }
     P[q4] = p;
+    if (store) {
+      reinterpret_cast<float4*>(a.dstore + (long long)s * d.plane)[q4] = dsum;
+      if (a.pstore) reinterpret_cast<float4*>(a.pstore + (long long)s * d.plane)[q4] = p;
+    }
   }
 }
 
@@ -341,16 +350,374 @@ __global__ void fill3_kernel(float* __restrict__ p, long long n, float v) {
     p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (reverse) step.  lam holds the adjoint of the state after step `it`
+// (its receivers' adjoint sources injected) and leaves with the adjoint of the
+// state before it.  Three launches, each transposing part of the step:
+//   adj3_q_kernel  vz[f-1] = vz[f] and the vz / vy / vx updates: q = alpha2 lam_v,
+//                  d alpha2 -= lam_v bracket(p'), lam_v *= 1 - kappa
+//   adj3_p_kernel  lam_p += -D^T q, p[f-1] = -p[f+1], the source term (wavelet
+//                  gradient), the p update: r = alpha1 lam_p, d alpha1 -= lam_p D,
+//                  lam_p *= 1 - kappa1
+//   adj3_v_kernel  lam_v += -B^T r
+// ---------------------------------------------------------------------------
+
+struct Adj3Args {
+  Dims3 d;
+  int S, spb, it, nt;
+  float dt;
+  float* lam;              // [field p|vx|vy|vz][shot][plane]
+  long long fstride;       // S * plane
+  float* q;                // [qx|qy|qz][shot][plane]
+  float* r;                // [shot][plane]
+  const float* coef;
+  const float* dstep;      // p-update bracket of step `it`, [shot][plane]
+  const float* pstep;      // p after step `it`, [shot][plane]; null without the density gradient
+  float* gA1;              // [group][plane]
+  float* gA2;              // [group][plane] or null
+  const int* src_cell;
+  float* gw;               // grad_src_v (S, nt), or null
+  float* gws;              // [S] source terms of this step for a shared wavelet, or null
+};
+
+struct Cell3 {
+  int g, y, z;
+  long long q4;
+};
+
+__device__ __forceinline__ bool cell3(const Dims3& d, Cell3& c) {
+  const long long gq = xcd_block3() * T3 + threadIdx.x;
+  if (gq >= d.groups) return false;
+  c.g = (int)(gq % d.GX);
+  const long long zy = gq / d.GX;
+  c.y = (int)(zy % d.NY);
+  c.z = (int)(zy / d.NY);
+  c.q4 = gq;
+  return true;
+}
+
+__device__ __forceinline__ float4 ld4(const float4* p, long long i, bool ok) {
+  return ok ? p[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
+  const Dims3 d = a.d;
+  Cell3 c;
+  if (!cell3(d, c)) return;
+  const int g = c.g, y = c.y, z = c.z;
+  const long long q4 = c.q4;
+  const bool zx = z >= d.f && z <= d.NZ - 2, zz = z >= d.f && z <= d.NZ - 3;
+  unsigned mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int x = 4 * g + e;
+    if (zx && y >= 1 && y <= d.NY - 2 && x >= 1 && x <= d.NX - 3) mx |= 1u << e;
+    if (zx && y >= 1 && y <= d.NY - 3 && x >= 1 && x <= d.NX - 2) my |= 1u << e;
+    if (zz && y >= 1 && y <= d.NY - 2 && x >= 1 && x <= d.NX - 2) mz |= 1u << e;
+  }
+  const bool fold = d.fs && z == d.f;  // transpose of vz[f-1] = vz[f]
+  if (!(mx | my | mz) && !fold) return;
+  const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
+  const long long p4 = d.plane >> 2;
+  const float4 omk2 = C[2 * p4 + q4], omk3 = C[3 * p4 + q4], omk4 = C[4 * p4 + q4];
+  const float4 a2 = C[5 * p4 + q4];
+  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
+  const long long row = (long long)z * d.NY + y;
+  const long long gl = row * d.GX + max(g - 1, 0), gr = row * d.GX + min(g + 1, d.GX - 1);
+  const bool rho = a.gA2 != nullptr;
+  const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
+  float4 acc2 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int s = s_begin; s < s_end; ++s) {
+    const long long so = (long long)s * d.plane;
+    float4* LX = reinterpret_cast<float4*>(a.lam + a.fstride + so);
+    float4* LY = reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so);
+    float4* LZ = reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so);
+    float4 lx = LX[q4], ly = LY[q4], lz = LZ[q4];
+    if (fold) {
+      const float4 up = LZ[q4 - sz4];
+      lz = make_float4(lz.x + up.x, lz.y + up.y, lz.z + up.z, lz.w + up.w);
+      LZ[q4 - sz4] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float4 qx = make_float4(0.f, 0.f, 0.f, 0.f), qy = qx, qz = qx;
+    float pxs[12];
+    float4 pb = qx, pyu = qx, pyd1 = qx, pyd2 = qx, pzu = qx, pzd1 = qx, pzd2 = qx;
+    if (rho) {  // brackets of the v updates, from p after the step
+      const float4* P1 = reinterpret_cast<const float4*>(a.pstep + so);
+      pb = P1[q4];
+      if (mx) {
+        const float4 pa = P1[gl], pc = P1[gr];
+        pxs[0] = pa.x; pxs[1] = pa.y; pxs[2] = pa.z; pxs[3] = pa.w;
+        pxs[8] = pc.x; pxs[9] = pc.y; pxs[10] = pc.z; pxs[11] = pc.w;
+      }
+      pxs[4] = pb.x; pxs[5] = pb.y; pxs[6] = pb.z; pxs[7] = pb.w;
+      if (my) {
+        pyu = P1[q4 - sy4];
+        pyd1 = P1[q4 + sy4];
+        pyd2 = P1[q4 + 2 * sy4];
+      }
+      if (mz) {
+        pzd1 = P1[q4 + sz4];
+        pzd2 = P1[q4 + 2 * sz4];
+        // the forward read the mirrored plane f-1 as -p[f+1]
+        pzu = fold ? make_float4(-pzd1.x, -pzd1.y, -pzd1.z, -pzd1.w) : P1[q4 - sz4];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float acc = g3(acc2, e);
+      if ((mz >> e) & 1u) {
+        const float l = g3(lz, e);
+        s3(qz, e, g3(a2, e) * l);
+        if (rho) acc = acc - l * d4(g3(pzd1, e), g3(pb, e), g3(pzd2, e), g3(pzu, e));
+        s3(lz, e, g3(omk4, e) * l);
+      }
+      if ((my >> e) & 1u) {
+        const float l = g3(ly, e);
+        s3(qy, e, g3(a2, e) * l);
+        if (rho) acc = acc - l * d4(g3(pyd1, e), g3(pb, e), g3(pyd2, e), g3(pyu, e));
+        s3(ly, e, g3(omk3, e) * l);
+      }
+      if ((mx >> e) & 1u) {
+        const float l = g3(lx, e);
+        s3(qx, e, g3(a2, e) * l);
+        if (rho) acc = acc - l * d4(pxs[5 + e], pxs[4 + e], pxs[6 + e], pxs[3 + e]);
+        s3(lx, e, g3(omk2, e) * l);
+      }
+      s3(acc2, e, acc);
+    }
+    float4* Q = reinterpret_cast<float4*>(a.q + so);
+    if (mx) { Q[q4] = qx; LX[q4] = lx; }
+    if (my) { Q[a.fstride / 4 + q4] = qy; LY[q4] = ly; }
+    if (mz || fold) LZ[q4] = lz;
+    if (mz) Q[a.fstride / 2 + q4] = qz;
+  }
+  if (rho && (mx | my | mz)) {
+    float4* G = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
+    const float4 o = G[q4];
+    G[q4] = make_float4(o.x + acc2.x, o.y + acc2.y, o.z + acc2.z, o.w + acc2.w);
+  }
+}
+
+// -D^T q at the cell group (z, y, g): c1 (q[i] - q[i-1]) + c2 (q[i+1] - q[i-2]) per axis,
+// added z, then y, then x; q outside the grid is zero
+__device__ __forceinline__ float4 gather_q(const Dims3& d, const float4* QX, const float4* QY,
+                                           const float4* QZ, int z, int y, int g) {
+  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
+  const long long q4 = ((long long)z * d.NY + y) * d.GX + g;
+  const float4 z0 = QZ[q4], zm1 = ld4(QZ, q4 - sz4, z >= 1), zm2 = ld4(QZ, q4 - 2 * sz4, z >= 2);
+  const float4 zp1 = ld4(QZ, q4 + sz4, z + 1 < d.NZ);
+  const float4 y0 = QY[q4], ym1 = ld4(QY, q4 - sy4, y >= 1), ym2 = ld4(QY, q4 - 2 * sy4, y >= 2);
+  const float4 yp1 = ld4(QY, q4 + sy4, y + 1 < d.NY);
+  const float4 xa = ld4(QX, q4 - 1, g >= 1), xb = QX[q4], xc = ld4(QX, q4 + 1, g + 1 < d.GX);
+  const float xs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
+  float4 out;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    float acc = 0.f;
+    acc = acc + d4(g3(z0, e), g3(zm1, e), g3(zp1, e), g3(zm2, e));
+    acc = acc + d4(g3(y0, e), g3(ym1, e), g3(yp1, e), g3(ym2, e));
+    acc = acc + d4(xs[4 + e], xs[3 + e], xs[5 + e], xs[2 + e]);
+    s3(out, e, acc);
+  }
+  return out;
+}
+
+__global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
+  const Dims3 d = a.d;
+  Cell3 c;
+  if (!cell3(d, c)) return;
+  const int g = c.g, y = c.y, z = c.z;
+  const long long q4 = c.q4;
+  unsigned mp = 0;
+  if (z >= d.f + 1 && z <= d.NZ - 3 && y >= 2 && y <= d.NY - 3)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = 4 * g + e;
+      if (x >= 2 && x <= d.NX - 3) mp |= 1u << e;
+    }
+  const bool fold = d.fs && z == d.f + 1;   // transpose of p[f-1] = -p[f+1]
+  const bool mirrored = d.fs && z == d.f - 1;  // zeroed by the fold at f+1
+  const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
+  const long long p4 = d.plane >> 2;
+  const float4 omk1 = C[q4], a1 = C[p4 + q4];
+  const long long sz4 = (long long)d.NY * d.GX;
+  const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
+  float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int s = s_begin; s < s_end; ++s) {
+    const long long so = (long long)s * d.plane;
+    const int sc = a.src_cell[s];
+    const bool src_here = (long long)(sc >> 2) == q4;
+    if (mirrored) {  // the source term on the mirrored plane is overwritten: no gradient
+      if (src_here) {
+        if (a.gws) a.gws[s] = 0.f;
+        if (a.gw) a.gw[(long long)s * a.nt + a.it] = 0.f;
+      }
+      continue;
+    }
+    float4* LP = reinterpret_cast<float4*>(a.lam + so);
+    const float4* QX = reinterpret_cast<const float4*>(a.q + so);
+    const float4* QY = reinterpret_cast<const float4*>(a.q + a.fstride + so);
+    const float4* QZ = reinterpret_cast<const float4*>(a.q + 2 * a.fstride + so);
+    float4 lp = LP[q4];
+    const float4 gq = gather_q(d, QX, QY, QZ, z, y, g);
+    lp = make_float4(lp.x + gq.x, lp.y + gq.y, lp.z + gq.z, lp.w + gq.w);
+    if (fold) {
+      float4 lm = LP[q4 - 2 * sz4];
+      const float4 gm = gather_q(d, QX, QY, QZ, z - 2, y, g);
+      lm = make_float4(lm.x + gm.x, lm.y + gm.y, lm.z + gm.z, lm.w + gm.w);
+      lp = make_float4(lp.x - lm.x, lp.y - lm.y, lp.z - lm.z, lp.w - lm.w);
+      LP[q4 - 2 * sz4] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (src_here) {
+      const float v = a.dt * g3(lp, sc & 3);
+      if (a.gws) a.gws[s] = v;
+      if (a.gw) a.gw[(long long)s * a.nt + a.it] = v;
+    }
+    if (mp) {
+      const float4 D = reinterpret_cast<const float4*>(a.dstep + so)[q4];
+      float4 rr = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (!((mp >> e) & 1u)) continue;
+        const float l = g3(lp, e);
+        s3(rr, e, g3(a1, e) * l);
+        s3(acc1, e, g3(acc1, e) - l * g3(D, e));
+        s3(lp, e, g3(omk1, e) * l);
+      }
+      reinterpret_cast<float4*>(a.r + so)[q4] = rr;
+    }
+    LP[q4] = lp;
+  }
+  if (mp && a.gA1) {
+    float4* G = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
+    const float4 o = G[q4];
+    G[q4] = make_float4(o.x + acc1.x, o.y + acc1.y, o.z + acc1.z, o.w + acc1.w);
+  }
+}
+
+// lam_v += c1 (r[i+1] - r[i]) + c2 (r[i+2] - r[i-1]) along each velocity's axis
+__global__ __launch_bounds__(T3) void adj3_v_kernel(Adj3Args a) {
+  const Dims3 d = a.d;
+  Cell3 c;
+  if (!cell3(d, c)) return;
+  const int g = c.g, y = c.y, z = c.z;
+  const long long q4 = c.q4;
+  // r is zero outside z in [f+1, NZ-3], y in [2, NY-3]: skip cells no term reaches
+  if (z < d.f - 1 || z > d.NZ - 2 || y > d.NY - 2) return;
+  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
+  const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
+  for (int s = s_begin; s < s_end; ++s) {
+    const long long so = (long long)s * d.plane;
+    const float4* Rr = reinterpret_cast<const float4*>(a.r + so);
+    float4* LX = reinterpret_cast<float4*>(a.lam + a.fstride + so);
+    float4* LY = reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so);
+    float4* LZ = reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so);
+    const float4 r0 = Rr[q4];
+    const float4 xa = ld4(Rr, q4 - 1, g >= 1), xc = ld4(Rr, q4 + 1, g + 1 < d.GX);
+    const float rs[12] = {xa.x, xa.y, xa.z, xa.w, r0.x, r0.y, r0.z, r0.w, xc.x, xc.y, xc.z, xc.w};
+    const float4 ym1 = ld4(Rr, q4 - sy4, y >= 1), yp1 = Rr[q4 + sy4];
+    const float4 yp2 = ld4(Rr, q4 + 2 * sy4, y + 2 < d.NY);
+    const float4 zm1 = ld4(Rr, q4 - sz4, z >= 1), zp1 = Rr[q4 + sz4];
+    const float4 zp2 = ld4(Rr, q4 + 2 * sz4, z + 2 < d.NZ);
+    float4 lx = LX[q4], ly = LY[q4], lz = LZ[q4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      s3(lx, e, g3(lx, e) + d4(rs[5 + e], rs[4 + e], rs[6 + e], rs[3 + e]));
+      s3(ly, e, g3(ly, e) + d4(g3(yp1, e), g3(r0, e), g3(yp2, e), g3(ym1, e)));
+      s3(lz, e, g3(lz, e) + d4(g3(zp1, e), g3(r0, e), g3(zp2, e), g3(zm1, e)));
+    }
+    LX[q4] = lx;
+    LY[q4] = ly;
+    LZ[q4] = lz;
+  }
+}
+
+// adjoint sources of step `it`: thread (shot, receiver cell) adds the g of
+// every receiver on that cell, in receiver order
+__global__ void inject3_kernel(long long plane, int S, int nt, int it, int R, int ncell,
+                               const int* ucell, const int* ustart, const int* urcv,
+                               const float* gp, const float* gvx, const float* gvy,
+                               const float* gvz, float* lam, long long fstride) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)S * ncell) return;
+  const int s = (int)(i / ncell), u = (int)(i % ncell);
+  const long long base = ((long long)s * nt + it) * R;
+  const long long q = (long long)s * plane + ucell[u];
+  const float* gs[4] = {gp, gvx, gvy, gvz};
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (!gs[f]) continue;
+    float v = lam[f * fstride + q];
+    for (int j = ustart[u]; j < ustart[u + 1]; ++j) v = v + gs[f][base + urcv[j]];
+    lam[f * fstride + q] = v;
+  }
+}
+
+// shared wavelet: grad[it] = sum over shots of the shot's source term, in shot order
+__global__ void wavelet_sum3_kernel(const float* gws, int S, float* gw, int it) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  float v = 0.f;
+  for (int s = 0; s < S; ++s) v = v + gws[s];
+  gw[it] = v;
+}
+
+// chain rule through alpha1 = den c c dt / dz and alpha2 = dt / (den dz), and
+// the transpose of pad_model_3d: each model cell gathers the padded cells that
+// copy it (edges and corners collect whole blocks of the absorbing layers)
+__global__ void fold3_kernel(Dims3 d, int groups, const float* gA1, const float* gA2,
+                             const float* vp, const float* rho, float dt, float dz,
+                             float* gvp, float* grho) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= (long long)d.nz * d.ny * d.nx) return;
+  const int e = (int)(m % d.nx);
+  const int b = (int)((m / d.nx) % d.ny);
+  const int a = (int)(m / ((long long)d.nx * d.ny));
+  const int i0 = a == 0 ? 0 : a + d.nabc, i1 = a == d.nz - 1 ? d.NZ - 1 : a + d.nabc;
+  const int j0 = b == 0 ? 0 : b + d.nabc, j1 = b == d.ny - 1 ? d.NY - 1 : b + d.nabc;
+  const int k0 = e == 0 ? 0 : e + d.nabc, k1 = e == d.nx - 1 ? d.NX - 1 : e + d.nabc;
+  const float vc = vp[m], den = rho[m];
+  float f1 = 2.f * den;
+  f1 = f1 * vc;
+  f1 = f1 * dt;
+  f1 = f1 / dz;
+  float r1 = vc * vc;
+  r1 = r1 * dt;
+  r1 = r1 / dz;
+  float r2 = den * den;
+  r2 = r2 * dz;
+  r2 = dt / r2;
+  float av = 0.f, ar = 0.f;
+  for (int i = i0; i <= i1; ++i)
+    for (int j = j0; j <= j1; ++j)
+      for (int k = k0; k <= k1; ++k) {
+        const long long q = ((long long)i * d.NY + j) * d.PX + k;
+        float A1 = gA1[q], A2 = gA2 ? gA2[q] : 0.f;
+        for (int gi = 1; gi < groups; ++gi) {
+          A1 = A1 + gA1[(long long)gi * d.plane + q];
+          if (gA2) A2 = A2 + gA2[(long long)gi * d.plane + q];
+        }
+        av = av + A1 * f1;
+        if (grho) ar = ar + (A1 * r1 - A2 * r2);
+      }
+  if (gvp) gvp[m] = av;
+  if (grho) grho[m] = ar;
+}
+
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Plan3 {
   Dims3 d;
-  int S, R, nt, spb, groups, echunk;
-  size_t off_coef, off_state, off_E, off_src, off_rcell, total;
+  int S, R, nt, spb, groups, echunk, k, nseg;
+  size_t off_coef, off_state, off_E, off_src, off_rcell;
+  size_t off_lam, off_q, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
+      off_urcv;
+  size_t total;
 };
 
-int make_plan3(const gbw_acoustic3d_problem* p, Plan3& pl) {
+// pass 0: forward; 1: backward; 2: backward with the density gradient
+int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   if (!p) return fail(GBW_ERR_INVALID, "problem is NULL");
+  if (pass < 0 || pass > 2) return fail(GBW_ERR_INVALID, "pass must be 0, 1 or 2");
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
     return fail(GBW_ERR_INVALID, "Grid dimensions must be positive");
   if (p->nabc < 1) return fail(GBW_ERR_INVALID, "nabc must be >= 1");
@@ -358,6 +725,9 @@ int make_plan3(const gbw_acoustic3d_problem* p, Plan3& pl) {
   if (p->src_n <= 0) return fail(GBW_ERR_INVALID, "src_n must be positive");
   if (p->rcv_n < 0) return fail(GBW_ERR_INVALID, "rcv_n must be >= 0");
   if (!(p->dt > 0.f) || !(p->dz > 0.f)) return fail(GBW_ERR_INVALID, "dt and dz must be positive");
+  if (p->checkpoint_every < 0) return fail(GBW_ERR_INVALID, "checkpoint_every must be >= 0");
+  if (pass > 0 && p->checkpoint_every == 0)
+    return fail(GBW_ERR_INVALID, "the gradient needs checkpoint_every > 0");
   Dims3& d = pl.d;
   d.nz = p->nz; d.ny = p->ny; d.nx = p->nx; d.nabc = p->nabc;
   d.NZ = p->nz + 2 * p->nabc; d.NY = p->ny + 2 * p->nabc; d.NX = p->nx + 2 * p->nabc;
@@ -375,14 +745,32 @@ int make_plan3(const gbw_acoustic3d_problem* p, Plan3& pl) {
   pl.groups = (pl.S + spb - 1) / spb;
   if (pl.groups > 65535) return fail(GBW_ERR_INVALID, "more than 65535 shot groups: raise shots_per_block");
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
+  pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
+  pl.nseg = pl.k > 0 ? (pl.nt + pl.k - 1) / pl.k : 0;
   const size_t inner = (size_t)d.nz * d.ny * d.nx;
+  const size_t fld = (size_t)pl.S * d.plane * sizeof(float);  // one field, all shots
+  const size_t R1 = (size_t)std::max(1, pl.R);
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o += al256(bytes); return at; };
   pl.off_coef = take(6 * (size_t)d.plane * sizeof(float));
-  pl.off_state = take(4 * (size_t)pl.S * d.plane * sizeof(float));
-  pl.off_E = take((size_t)pl.groups * 4 * inner * sizeof(float));
+  pl.off_state = take(4 * fld);
   pl.off_src = take((size_t)pl.S * sizeof(int));
-  pl.off_rcell = take((size_t)std::max(1, pl.R) * sizeof(int));
+  pl.off_rcell = take(R1 * sizeof(int));
+  if (pass == 0) {
+    pl.off_E = take((size_t)pl.groups * 4 * inner * sizeof(float));
+  } else {
+    pl.off_lam = take(4 * fld);
+    pl.off_q = take(3 * fld);
+    pl.off_r = take(fld);
+    pl.off_D = take((size_t)pl.k * fld);
+    pl.off_P1 = pass == 2 ? take((size_t)pl.k * fld) : 0;
+    pl.off_gA1 = take((size_t)pl.groups * d.plane * sizeof(float));
+    pl.off_gA2 = pass == 2 ? take((size_t)pl.groups * d.plane * sizeof(float)) : 0;
+    pl.off_gws = take((size_t)pl.S * sizeof(float));
+    pl.off_ucell = take(R1 * sizeof(int));
+    pl.off_ustart = take((R1 + 1) * sizeof(int));
+    pl.off_urcv = take(R1 * sizeof(int));
+  }
   pl.total = o;
   return 0;
 }
@@ -397,37 +785,18 @@ int fill3(float* p, long long n, float v, hipStream_t st) {
   return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t gbw_acoustic3d_workspace_bytes(const gbw_acoustic3d_problem* prob) {
-  Plan3 pl;
-  if (make_plan3(prob, pl)) return 0;
-  return pl.total;
-}
-
-int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acoustic3d_inputs* in,
-                           const gbw_acoustic3d_outputs* out, void* workspace,
-                           size_t workspace_bytes, void* stream) {
-  Plan3 pl;
-  int rc = make_plan3(prob, pl);
-  if (rc) return rc;
-  if (!in || !out) return fail(GBW_ERR_INVALID, "inputs / outputs are NULL");
-  if (!workspace || workspace_bytes < pl.total)
-    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(pl.total) + " bytes");
+// source and receiver cells, with the reference's IndexError semantics (vx
+// has NX-1 columns, vy NY-1 rows, vz NZ-1 planes, and every receiver reads all four)
+int geometry3(const Plan3& pl, const gbw_acoustic3d_inputs* in, std::vector<int>& src,
+              std::vector<int>& rcell) {
+  const Dims3& d = pl.d;
   if (!in->vp || !in->rho || !in->damp || !in->src_v)
     return fail(GBW_ERR_INVALID, "vp, rho, damp and src_v must be device pointers");
   if (!in->src_x || !in->src_y || !in->src_z) return fail(GBW_ERR_INVALID, "source positions are NULL");
   if (pl.R > 0 && (!in->rcv_x || !in->rcv_y || !in->rcv_z))
     return fail(GBW_ERR_INVALID, "receiver positions are NULL");
-  const bool rec = out->rec_p || out->rec_vx || out->rec_vy || out->rec_vz;
-  if (rec && !(out->rec_p && out->rec_vx && out->rec_vy && out->rec_vz))
-    return fail(GBW_ERR_INVALID, "rec_p, rec_vx, rec_vy and rec_vz must be given together");
-  const Dims3& d = pl.d;
-  // geometry, with the reference's IndexError semantics (vx has NX-1
-  // columns, vy NY-1 rows, vz NZ-1 planes)
-  std::vector<int> src(pl.S), rcell(std::max(1, pl.R), 0);
+  src.assign(pl.S, 0);
+  rcell.assign(std::max(1, pl.R), 0);
   for (int s = 0; s < pl.S; ++s) {
     const long long z = in->src_z[s] + d.nabc, y = in->src_y[s] + d.nabc, x = in->src_x[s] + d.nabc;
     if (z < 0 || z >= d.NZ || y < 0 || y >= d.NY || x < 0 || x >= d.NX)
@@ -440,6 +809,59 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
       return fail(GBW_ERR_INDEX, "index out of range: receiver " + std::to_string(r));
     rcell[r] = (int)((z * d.NY + y) * d.PX + x);
   }
+  return 0;
+}
+
+unsigned step_blocks3(const Dims3& d) { return (unsigned)(((d.groups + T3 - 1) / T3 + 7) / 8 * 8); }
+
+// one forward step of every shot (p, then v); records and energy when asked
+int forward_step3(Step3Args& a, const dim3& grid, hipStream_t st) {
+  {
+    LaunchTimer timer("p3_step_kernel", st);
+    p3_step_kernel<<<grid, T3, 0, st>>>(a);
+  }
+  LAUNCH_CHECK();
+  {
+    LaunchTimer timer("v3_step_kernel", st);
+    v3_step_kernel<<<grid, T3, 0, st>>>(a);
+  }
+  LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t gbw_acoustic3d_workspace_bytes(const gbw_acoustic3d_problem* prob, int pass) {
+  Plan3 pl;
+  if (make_plan3(prob, pass, pl)) return 0;
+  return pl.total;
+}
+
+size_t gbw_acoustic3d_checkpoint_bytes(const gbw_acoustic3d_problem* prob) {
+  Plan3 pl;
+  if (make_plan3(prob, 0, pl)) return 0;
+  return (size_t)pl.nseg * 4 * (size_t)pl.S * pl.d.plane * sizeof(float);
+}
+
+int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acoustic3d_inputs* in,
+                           const gbw_acoustic3d_outputs* out, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  Plan3 pl;
+  int rc = make_plan3(prob, 0, pl);
+  if (rc) return rc;
+  if (!in || !out) return fail(GBW_ERR_INVALID, "inputs / outputs are NULL");
+  if (!workspace || workspace_bytes < pl.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(pl.total) + " bytes");
+  const bool rec = out->rec_p || out->rec_vx || out->rec_vy || out->rec_vz;
+  if (rec && !(out->rec_p && out->rec_vx && out->rec_vy && out->rec_vz))
+    return fail(GBW_ERR_INVALID, "rec_p, rec_vx, rec_vy and rec_vz must be given together");
+  if (pl.k > 0 && !out->checkpoints)
+    return fail(GBW_ERR_INVALID, "checkpoint_every > 0 needs a checkpoint buffer");
+  const Dims3& d = pl.d;
+  std::vector<int> src, rcell;
+  if ((rc = geometry3(pl, in, src, rcell))) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_src), src.data(), src.size() * sizeof(int),
                          hipMemcpyHostToDevice, st));
@@ -467,20 +889,14 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
   a.src_cell = at3<int>(workspace, pl.off_src);
   a.wav = in->src_v; a.wav_stride = prob->wavelet_shared ? 0 : prob->nt;
   a.energy = energy ? E : nullptr;
-  const unsigned blocks = (unsigned)(((d.groups + T3 - 1) / T3 + 7) / 8 * 8);
-  const dim3 grid(blocks, pl.groups);
+  const dim3 grid(step_blocks3(d), pl.groups);
   for (int it = 0; it < pl.nt; ++it) {
+    if (pl.k > 0 && it % pl.k == 0)
+      HIP_TRY(hipMemcpyAsync(static_cast<char*>(out->checkpoints) +
+                                 (size_t)(it / pl.k) * 4 * fstride * sizeof(float),
+                             state, 4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
     a.it = it;
-    {
-      LaunchTimer timer("p3_step_kernel", st);
-      p3_step_kernel<<<grid, T3, 0, st>>>(a);
-    }
-    LAUNCH_CHECK();
-    {
-      LaunchTimer timer("v3_step_kernel", st);
-      v3_step_kernel<<<grid, T3, 0, st>>>(a);
-    }
-    LAUNCH_CHECK();
+    if ((rc = forward_step3(a, grid, st))) return rc;
     if (rec && pl.R > 0) {
       record3_kernel<<<(unsigned)(((long long)pl.S * pl.R + 255) / 256), 256, 0, st>>>(
           d, pl.S, pl.R, pl.nt, it, state, fstride, at3<int>(workspace, pl.off_rcell), out->rec_p,
@@ -493,8 +909,136 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
       LAUNCH_CHECK();
     }
   }
-  // the host vectors above were staged by hipMemcpyAsync before it returned;
-  // wait anyway so they outlive the copies
+  // the host vectors above must outlive their copies
+  HIP_TRY(hipStreamSynchronize(st));
+  return flush_timing(st);
+}
+
+int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoustic3d_inputs* in,
+                            const gbw_acoustic3d_adjoint* adj, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  if (!adj || !in) return fail(GBW_ERR_INVALID, "inputs / adjoint are NULL");
+  const int pass = adj->grad_rho ? 2 : 1;
+  Plan3 pl;
+  int rc = make_plan3(prob, pass, pl);
+  if (rc) return rc;
+  if (!workspace || workspace_bytes < pl.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(pl.total) + " bytes");
+  if (!adj->checkpoints) return fail(GBW_ERR_INVALID, "checkpoints are NULL");
+  const Dims3& d = pl.d;
+  std::vector<int> src, rcell;
+  if ((rc = geometry3(pl, in, src, rcell))) return rc;
+  // receivers grouped by cell, first appearance order; receivers of a cell in their order
+  std::vector<int> ucell, ustart, urcv;
+  {
+    std::vector<int> order(pl.R);
+    for (int r = 0; r < pl.R; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rcell[a] < rcell[b]; });
+    for (int i = 0; i < pl.R; ++i) {
+      if (i == 0 || rcell[order[i]] != rcell[order[i - 1]]) {
+        ucell.push_back(rcell[order[i]]);
+        ustart.push_back(i);
+      }
+      urcv.push_back(order[i]);
+    }
+    ustart.push_back(pl.R);
+    if (ucell.empty()) { ucell.push_back(0); urcv.push_back(0); }
+  }
+  const int ncell = pl.R > 0 ? (int)ustart.size() - 1 : 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_src), src.data(), src.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_ucell), ucell.data(), ucell.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_ustart), ustart.data(),
+                         ustart.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_urcv), urcv.data(), urcv.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
+  float* coef = at3<float>(workspace, pl.off_coef);
+  coef3_kernel<<<(unsigned)((d.plane + 255) / 256), 256, 0, st>>>(d, in->vp, in->rho, in->damp,
+                                                                    prob->dt, prob->dz, coef);
+  LAUNCH_CHECK();
+  const long long fstride = (long long)pl.S * d.plane;
+  float* state = at3<float>(workspace, pl.off_state);
+  float* lam = at3<float>(workspace, pl.off_lam);
+  float* q = at3<float>(workspace, pl.off_q);
+  float* r = at3<float>(workspace, pl.off_r);
+  float* D = at3<float>(workspace, pl.off_D);
+  float* P1 = pass == 2 ? at3<float>(workspace, pl.off_P1) : nullptr;
+  float* gA1 = at3<float>(workspace, pl.off_gA1);
+  float* gA2 = pass == 2 ? at3<float>(workspace, pl.off_gA2) : nullptr;
+  if ((rc = fill3(lam, 4 * fstride, 0.f, st))) return rc;
+  if ((rc = fill3(q, 3 * fstride, 0.f, st))) return rc;
+  if ((rc = fill3(r, fstride, 0.f, st))) return rc;
+  if ((rc = fill3(gA1, (long long)pl.groups * d.plane, 0.f, st))) return rc;
+  if (gA2 && (rc = fill3(gA2, (long long)pl.groups * d.plane, 0.f, st))) return rc;
+  const bool shared = prob->wavelet_shared != 0;
+  if (adj->grad_src_v &&
+      (rc = fill3(adj->grad_src_v, shared ? pl.nt : (long long)pl.S * pl.nt, 0.f, st)))
+    return rc;
+  Step3Args a{};
+  a.d = d; a.S = pl.S; a.spb = pl.spb; a.nt = pl.nt; a.dt = prob->dt;
+  a.state = state; a.fstride = fstride; a.coef = coef;
+  a.src_cell = at3<int>(workspace, pl.off_src);
+  a.wav = in->src_v; a.wav_stride = shared ? 0 : prob->nt;
+  Adj3Args b{};
+  b.d = d; b.S = pl.S; b.spb = pl.spb; b.nt = pl.nt; b.dt = prob->dt;
+  b.lam = lam; b.fstride = fstride; b.q = q; b.r = r; b.coef = coef;
+  b.gA1 = gA1; b.gA2 = gA2; b.src_cell = a.src_cell;
+  b.gw = (adj->grad_src_v && !shared) ? adj->grad_src_v : nullptr;
+  b.gws = (adj->grad_src_v && shared) ? at3<float>(workspace, pl.off_gws) : nullptr;
+  const dim3 grid(step_blocks3(d), pl.groups);
+  const bool inject = pl.R > 0 && (adj->g_p || adj->g_vx || adj->g_vy || adj->g_vz);
+  for (int c = pl.nseg - 1; c >= 0; --c) {
+    const int t0 = c * pl.k, t1 = std::min(pl.nt, t0 + pl.k);
+    HIP_TRY(hipMemcpyAsync(state,
+                           static_cast<const char*>(adj->checkpoints) +
+                               (size_t)c * 4 * fstride * sizeof(float),
+                           4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
+    for (int it = t0; it < t1; ++it) {  // recompute: brackets (and p) of every step
+      a.it = it;
+      a.dstore = D + (size_t)(it - t0) * fstride;
+      a.pstore = P1 ? P1 + (size_t)(it - t0) * fstride : nullptr;
+      if ((rc = forward_step3(a, grid, st))) return rc;
+    }
+    for (int it = t1 - 1; it >= t0; --it) {
+      if (inject) {
+        inject3_kernel<<<(unsigned)(((long long)pl.S * ncell + 255) / 256), 256, 0, st>>>(
+            d.plane, pl.S, pl.nt, it, pl.R, ncell, at3<int>(workspace, pl.off_ucell),
+            at3<int>(workspace, pl.off_ustart), at3<int>(workspace, pl.off_urcv), adj->g_p,
+            adj->g_vx, adj->g_vy, adj->g_vz, lam, fstride);
+        LAUNCH_CHECK();
+      }
+      b.it = it;
+      b.dstep = D + (size_t)(it - t0) * fstride;
+      b.pstep = P1 ? P1 + (size_t)(it - t0) * fstride : nullptr;
+      {
+        LaunchTimer timer("adj3_q_kernel", st);
+        adj3_q_kernel<<<grid, T3, 0, st>>>(b);
+      }
+      LAUNCH_CHECK();
+      {
+        LaunchTimer timer("adj3_p_kernel", st);
+        adj3_p_kernel<<<grid, T3, 0, st>>>(b);
+      }
+      LAUNCH_CHECK();
+      if (b.gws) {
+        wavelet_sum3_kernel<<<1, 64, 0, st>>>(b.gws, pl.S, adj->grad_src_v, it);
+        LAUNCH_CHECK();
+      }
+      {
+        LaunchTimer timer("adj3_v_kernel", st);
+        adj3_v_kernel<<<grid, T3, 0, st>>>(b);
+      }
+      LAUNCH_CHECK();
+    }
+  }
+  if (adj->grad_vp || adj->grad_rho) {
+    const long long inner = (long long)d.nz * d.ny * d.nx;
+    fold3_kernel<<<(unsigned)((inner + 255) / 256), 256, 0, st>>>(
+        d, pl.groups, gA1, gA2, in->vp, in->rho, prob->dt, prob->dz, adj->grad_vp, adj->grad_rho);
+    LAUNCH_CHECK();
+  }
   HIP_TRY(hipStreamSynchronize(st));
   return flush_timing(st);
 }

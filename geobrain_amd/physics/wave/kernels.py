@@ -356,6 +356,102 @@ def set_shots_per_block_3d(n: int) -> None:
     _shots_per_block_3d = int(n)
 
 
+def auto_checkpoint_interval_3d(nt: int, rho_grad: bool) -> int:
+    """Interval minimising checkpoints (16 B/cell each, ``nt/k`` of them) plus the
+    per-interval store the gradient recomputes (4 B/cell per step, 8 with the
+    density gradient, ``k`` of them)."""
+    per_step = 8.0 if rho_grad else 4.0
+    return max(1, min(nt, int(round(math.sqrt(16.0 * nt / per_step)))))
+
+
+@dataclass
+class _Spec3:
+    prob: _native.Problem3
+    idx: tuple  # host int64 (src_x, src_y, src_z, rcv_x, rcv_y, rcv_z)
+
+
+def _inputs3(spec: _Spec3, src_v: Tensor, damp: Tensor, vp: Tensor, rho: Tensor) -> _native.Inputs3:
+    sx, sy, sz, rx, ry, rz = spec.idx
+    ins = _native.Inputs3()
+    ins.src_x, ins.src_y, ins.src_z = sx.ctypes.data, sy.ctypes.data, sz.ctypes.data
+    if rx.size:
+        ins.rcv_x, ins.rcv_y, ins.rcv_z = rx.ctypes.data, ry.ctypes.data, rz.ctypes.data
+    ins.src_v, ins.damp = src_v.data_ptr(), damp.data_ptr()
+    ins.vp, ins.rho = vp.data_ptr(), rho.data_ptr()
+    return ins
+
+
+class Acoustic3DFunction(torch.autograd.Function):
+    """3D seismograms and energy maps as a differentiable function of
+    ``(vp, rho, src_v)``; ``damp`` and the geometry are constants."""
+
+    @staticmethod
+    def forward(ctx, vp, rho, src_v, damp, spec: _Spec3):
+        lib = _native.load()
+        prob = spec.prob
+        dev = vp.device
+        S, nt, R = prob.src_n, prob.nt, prob.rcv_n
+        nz, ny, nx = prob.nz, prob.ny, prob.nx
+        need_grad = any(ctx.needs_input_grad[:3])
+        if not need_grad:
+            prob.checkpoint_every = 0
+        ws_bytes = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(prob), 0)
+        if ws_bytes == 0:
+            _native.check(-1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        ck = None
+        if need_grad:
+            ck = torch.empty(lib.gbw_acoustic3d_checkpoint_bytes(ctypes.byref(prob)),
+                             dtype=torch.uint8, device=dev)
+        f32 = dict(dtype=torch.float32, device=dev)
+        rec = [torch.empty((S, nt, R), **f32) for _ in range(4)]
+        eng = [torch.empty((nz, ny, nx), **f32) for _ in range(4)]
+        outs = _native.Outputs3()
+        outs.rec_p, outs.rec_vx, outs.rec_vy, outs.rec_vz = (t.data_ptr() for t in rec)
+        outs.energy_p, outs.energy_vx, outs.energy_vy, outs.energy_vz = (t.data_ptr() for t in eng)
+        outs.checkpoints = _ptr(ck)
+        ins = _inputs3(spec, src_v, damp, vp, rho)
+        rc = lib.gbw_acoustic3d_forward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(outs),
+                                        ws.data_ptr(), ws_bytes, _stream_handle(dev))
+        _native.check(rc)
+        ctx.mark_non_differentiable(*eng)
+        ctx.spec = spec
+        ctx.ck = ck
+        ctx.save_for_backward(vp, rho, src_v, damp)
+        return (*rec, *eng)
+
+    @staticmethod
+    def backward(ctx, g_p, g_vx, g_vy, g_vz, *unused):
+        lib = _native.load()
+        vp, rho, src_v, damp = ctx.saved_tensors
+        spec = ctx.spec
+        prob = spec.prob
+        dev = vp.device
+
+        def prep(g):
+            return None if g is None else g.to(torch.float32).contiguous()
+
+        gs = [prep(g) for g in (g_p, g_vx, g_vy, g_vz)]
+        want_vp, want_rho, want_w = ctx.needs_input_grad[:3]
+        grad_vp = torch.empty_like(vp) if want_vp else None
+        grad_rho = torch.empty_like(rho) if want_rho else None
+        grad_w = torch.empty_like(src_v) if want_w else None
+        adj = _native.Adjoint3()
+        adj.g_p, adj.g_vx, adj.g_vy, adj.g_vz = (_ptr(g) for g in gs)
+        adj.checkpoints = _ptr(ctx.ck)
+        adj.grad_vp, adj.grad_rho, adj.grad_src_v = _ptr(grad_vp), _ptr(grad_rho), _ptr(grad_w)
+        ws_bytes = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(prob), 2 if want_rho else 1)
+        if ws_bytes == 0:
+            _native.check(-1)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        ins = _inputs3(spec, src_v, damp, vp, rho)
+        rc = lib.gbw_acoustic3d_backward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(adj),
+                                         ws.data_ptr(), ws_bytes, _stream_handle(dev))
+        _native.check(rc)
+        ctx.ck = None
+        return grad_vp, grad_rho, grad_w, None, None
+
+
 def acoustic_forward_kernel_3d(
     nx: int,
     ny: int,
@@ -386,23 +482,20 @@ def acoustic_forward_kernel_3d(
     """3D forward simulation (signature of ``acoustic_3d.py:275-301``).
 
     Returns ``'p', 'vx', 'vy', 'vz'`` ``(src_n, nt, rcv_n)`` and the energy
-    maps ``'forward_wavefield_p/vx/vy/vz'`` ``(nz, ny, nx)``, summed per
-    ``torch.chunk`` chunk of ``checkpoint_segments`` as the reference sums
-    them (``acoustic_3d.py:433-476``).  All ``nt`` steps run in the HIP library
-    (``gbw_acoustic3d_forward``).
-
-    Forward modelling only: the 3D gradient is not implemented, so a ``vp``,
-    ``rho`` or ``src_v`` that requires grad raises ``NotImplementedError``
-    rather than returning outputs without a graph.
+    maps ``'forward_wavefield_p/vx/vy/vz'`` ``(nz, ny, nx)`` (detached), summed
+    per ``torch.chunk`` chunk of ``checkpoint_segments`` as the reference sums
+    them (``acoustic_3d.py:415-476``).  All ``nt`` steps run in the HIP library
+    (``gbw_acoustic3d_forward``).  Gradients flow to ``vp``, ``rho`` and
+    ``src_v`` when they require grad: ``gbw_acoustic3d_backward``, the
+    hand-written adjoint of the step with forward checkpoints every
+    ``ceil(nt / checkpoint_segments)`` steps (or an interval balancing
+    checkpoint and recompute memory when ``checkpoint_segments == 1``).
     """
     if dtype != torch.float32:
         raise NotImplementedError("geobrain_amd computes in float32 only (the reference default)")
     dev = vp.device
     for name, t in (("vp", vp), ("rho", rho), ("src_v", src_v), ("damp", damp)):
         _on_gpu(t, name, dev)
-    if torch.is_grad_enabled() and any(t.requires_grad for t in (vp, rho, src_v)):
-        raise NotImplementedError("geobrain_amd: the 3D acoustic gradient is not implemented; "
-                                  "call under torch.no_grad() or detach the inputs")
     if not torch.cuda.is_available():
         raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
     sx, sy, sz = (_host_index(a, n) for a, n in ((src_x, "src_x"), (src_y, "src_y"),
@@ -428,7 +521,6 @@ def acoustic_forward_kernel_3d(
             raise ValueError(f"src_v must have shape ({src_n}, {nt}) or ({nt},)")
     else:
         raise NotImplementedError("src_v must be (src_n, nt) or (nt,)")
-    lib = _native.load()
     prob = _native.Problem3()
     prob.nx, prob.ny, prob.nz = int(nx), int(ny), int(nz)
     prob.dx, prob.dy, prob.dz = float(dx), float(dy), float(dz)
@@ -438,31 +530,14 @@ def acoustic_forward_kernel_3d(
     prob.wavelet_shared = shared
     prob.energy_chunk = chunk_length(nt, checkpoint_segments)
     prob.shots_per_block = _shots_per_block_3d
+    if checkpoint_segments > 1:
+        prob.checkpoint_every = chunk_length(nt, checkpoint_segments)
+    else:
+        prob.checkpoint_every = auto_checkpoint_interval_3d(nt, bool(rho.requires_grad))
+    spec = _Spec3(prob=prob, idx=(sx, sy, sz, rx, ry, rz))
     f32 = torch.float32
-    vp_, rho_ = vp.detach().to(f32).contiguous(), rho.detach().to(f32).contiguous()
-    src_v_, damp_ = src_v.detach().to(f32).contiguous(), damp.detach().to(f32).contiguous()
-    ws_bytes = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(prob))
-    if ws_bytes == 0:
-        _native.check(-1)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    out = {k: torch.empty((src_n, nt, rcv_n), dtype=f32, device=dev)
-           for k in ('p', 'vx', 'vy', 'vz')}
-    for k in ('p', 'vx', 'vy', 'vz'):
-        out['forward_wavefield_' + k] = torch.empty((nz, ny, nx), dtype=f32, device=dev)
-    ins = _native.Inputs3()
-    ins.src_x, ins.src_y, ins.src_z = sx.ctypes.data, sy.ctypes.data, sz.ctypes.data
-    if rx.size:
-        ins.rcv_x, ins.rcv_y, ins.rcv_z = rx.ctypes.data, ry.ctypes.data, rz.ctypes.data
-    ins.src_v, ins.damp = src_v_.data_ptr(), damp_.data_ptr()
-    ins.vp, ins.rho = vp_.data_ptr(), rho_.data_ptr()
-    outs = _native.Outputs3()
-    outs.rec_p, outs.rec_vx = out['p'].data_ptr(), out['vx'].data_ptr()
-    outs.rec_vy, outs.rec_vz = out['vy'].data_ptr(), out['vz'].data_ptr()
-    outs.energy_p = out['forward_wavefield_p'].data_ptr()
-    outs.energy_vx = out['forward_wavefield_vx'].data_ptr()
-    outs.energy_vy = out['forward_wavefield_vy'].data_ptr()
-    outs.energy_vz = out['forward_wavefield_vz'].data_ptr()
-    rc = lib.gbw_acoustic3d_forward(ctypes.byref(prob), ctypes.byref(ins), ctypes.byref(outs),
-                                    ws.data_ptr(), ws_bytes, _stream_handle(dev))
-    _native.check(rc)
-    return out
+    res = Acoustic3DFunction.apply(vp.to(f32).contiguous(), rho.to(f32).contiguous(),
+                                   src_v.to(f32).contiguous(), damp.to(f32).contiguous(), spec)
+    keys = ('p', 'vx', 'vy', 'vz', 'forward_wavefield_p', 'forward_wavefield_vx',
+            'forward_wavefield_vy', 'forward_wavefield_vz')
+    return dict(zip(keys, res))

@@ -9,10 +9,12 @@
  *       geobrain/physics/wave/kernels/acoustic_3d.py:275-476
  * and the per-step body it drives,
  *   acoustic_step_forward_3d(...)   geobrain/physics/wave/kernels/acoustic_3d.py:20-273
- * Forward modelling only (seismograms and energy maps); the 3D gradient is not
- * implemented.  Conventions are those of gbw_acoustic2d.h: 0 or a negative
- * gbw_status, gbw_last_error() for the message, DEVICE / HOST pointers, no
- * allocation (the caller passes a workspace), work on the caller's stream.
+ * Forward modelling (seismograms, energy maps, checkpoints) and the gradient
+ * w.r.t. vp, rho and src_v (the hand-written adjoint of the step, recomputing
+ * each checkpoint interval).  Conventions are those of gbw_acoustic2d.h: 0 or
+ * a negative gbw_status, gbw_last_error() for the message, DEVICE / HOST
+ * pointers, no allocation (the caller passes a workspace), work on the
+ * caller's stream.
  */
 #ifndef GBW_ACOUSTIC3D_H
 #define GBW_ACOUSTIC3D_H
@@ -41,6 +43,7 @@ typedef struct gbw_acoustic3d_problem {
   int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
   int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
   int32_t shots_per_block;   /* launch choice: shots one thread steps (0 = all)        */
+  int32_t checkpoint_every;  /* steps per checkpoint interval (0 = no checkpoints)     */
 } gbw_acoustic3d_problem;
 
 /* Geometry and model (acoustic_3d.py:286-297); positions are unpadded indices. */
@@ -67,16 +70,44 @@ typedef struct gbw_acoustic3d_outputs {
   float* energy_vx;          /* DEVICE (nz, ny, nx)  key 'forward_wavefield_vx'        */
   float* energy_vy;          /* DEVICE (nz, ny, nx)  key 'forward_wavefield_vy'        */
   float* energy_vz;          /* DEVICE (nz, ny, nx)  key 'forward_wavefield_vz'        */
+  void* checkpoints;         /* DEVICE checkpoint_bytes(), needed when checkpoint_every */
 } gbw_acoustic3d_outputs;
 
-/* Bytes of device workspace for gbw_acoustic3d_forward; 0 if invalid. */
-size_t gbw_acoustic3d_workspace_bytes(const gbw_acoustic3d_problem* prob);
+/* Gradient request: adjoint sources are dLoss/d(seismogram).  Any g_* may be
+ * NULL (zero); any grad_* may be NULL (not wanted). */
+typedef struct gbw_acoustic3d_adjoint {
+  const float* g_p;          /* DEVICE (src_n, nt, rcv_n)                              */
+  const float* g_vx;         /* DEVICE (src_n, nt, rcv_n)                              */
+  const float* g_vy;         /* DEVICE (src_n, nt, rcv_n)                              */
+  const float* g_vz;         /* DEVICE (src_n, nt, rcv_n)                              */
+  const void* checkpoints;   /* DEVICE buffer filled by gbw_acoustic3d_forward         */
+  float* grad_vp;            /* DEVICE (nz, ny, nx)   overwritten                      */
+  float* grad_rho;           /* DEVICE (nz, ny, nx)   overwritten; density as an input */
+  float* grad_src_v;         /* DEVICE shape of src_v; overwritten                     */
+} gbw_acoustic3d_adjoint;
 
-/* Forward modelling: all nt steps, seismograms and energy maps.
- * Replaces acoustic_forward_kernel_3d (acoustic_3d.py:275-476) without gradient. */
+/* Bytes of device workspace for gbw_acoustic3d_forward (pass 0),
+ * gbw_acoustic3d_backward without (pass 1) or with (pass 2) the density
+ * gradient; 0 if the problem is invalid. */
+size_t gbw_acoustic3d_workspace_bytes(const gbw_acoustic3d_problem* prob, int pass);
+
+/* Bytes of the checkpoint buffer (0 when checkpoint_every == 0). */
+size_t gbw_acoustic3d_checkpoint_bytes(const gbw_acoustic3d_problem* prob);
+
+/* Forward modelling: all nt steps, seismograms, energy maps and, when
+ * checkpoint_every > 0, the checkpoints the gradient needs.
+ * Replaces acoustic_forward_kernel_3d (acoustic_3d.py:275-476). */
 int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acoustic3d_inputs* in,
                            const gbw_acoustic3d_outputs* out, void* workspace,
                            size_t workspace_bytes, void* stream /* hipStream_t */);
+
+/* Gradient of sum(g_p*p + g_vx*vx + g_vy*vy + g_vz*vz) w.r.t. vp (through
+ * alpha1 and pad_model_3d), rho and src_v: the hand-written adjoint of the 3D
+ * step, recomputing each checkpoint interval.  Replaces PyTorch autograd
+ * through torch.utils.checkpoint(acoustic_step_forward_3d) (acoustic_3d.py:415-437). */
+int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoustic3d_inputs* in,
+                            const gbw_acoustic3d_adjoint* adj, void* workspace,
+                            size_t workspace_bytes, void* stream /* hipStream_t */);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,7 @@
 the reference's golden vectors and the CPU oracle.  Bar: seismograms bit for
 bit; energy maps bit for bit when one thread steps every shot (the default),
 relative L2 <= 1e-6 when the shots are split over several groups (the sums
-are then added in another order)."""
+are then added in another order); gradients relative L2 <= 1e-5."""
 
 import numpy as np
 import pytest
@@ -11,7 +11,8 @@ import torch
 import oracle
 from geobrain_amd.physics import wave
 from geobrain_amd.physics.wave import kernels as K
-from tests._fixtures import FWD3D_CASES, OUT3D_KEYS, chunk_len, load, oracle3d_args, rel_l2
+from tests._fixtures import (FWD3D_CASES, GRAD3D_CASES, OUT3D_KEYS, adjoint_sources_3d,
+                             chunk_len, load, oracle3d_args, rel_l2)
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -142,7 +143,7 @@ def test_3d_linear_in_wavelet():
     assert np.abs(o1["p"]).max() > 0
 
 
-def test_3d_index_errors_and_gradient_refusal():
+def test_3d_index_errors():
     print("[case] 3d errors", flush=True)
     a = random_case(10, 9, 11, 3, 5, 1, False, 31)
     bad = dict(a, rcv_z=np.where(np.arange(len(a["rcv_z"])) == 0, 10 + 3 - 1, a["rcv_z"]))
@@ -151,11 +152,101 @@ def test_3d_index_errors_and_gradient_refusal():
     bad = dict(a, src_y=np.array([9 + 3]))
     with pytest.raises(IndexError):
         run3(bad)
-    vp = _t(a["vp"], dtype=torch.float32).requires_grad_(True)
-    with pytest.raises(NotImplementedError, match="3D acoustic gradient"):
-        wave.acoustic_forward_kernel_3d(
-            nx=11, ny=9, nz=10, dx=10.0, dy=10.0, dz=10.0, nt=5, dt=1e-3, nabc=3,
-            free_surface=False, src_x=a["src_x"], src_y=a["src_y"], src_z=a["src_z"], src_n=1,
-            src_v=_t(a["src_v"], dtype=torch.float32), rcv_x=a["rcv_x"], rcv_y=a["rcv_y"],
-            rcv_z=a["rcv_z"], rcv_n=len(a["rcv_x"]), damp=_t(a["damp"]), vp=vp,
-            rho=_t(a["rho"], dtype=torch.float32))
+
+
+# --------------------------------------------------------------------------
+# gradient: bar relative L2 <= 1e-5 against the reference's autograd and the
+# FP32 oracle (the sums are associated differently, so not bit for bit)
+# --------------------------------------------------------------------------
+GTOL = 1e-5
+
+
+def grad3(a, gs, segments=1, spb=0, want=("vp", "rho", "w")):
+    """Gradient of sum(g . seismograms) w.r.t. vp, rho and src_v through the
+    HIP adjoint; returns (outputs, {name: grad})."""
+    K.set_shots_per_block_3d(spb)
+    try:
+        vp = _t(a["vp"], dtype=torch.float32).requires_grad_("vp" in want)
+        rho = _t(a["rho"], dtype=torch.float32).requires_grad_("rho" in want)
+        w = _t(a["src_v"], dtype=torch.float32).requires_grad_("w" in want)
+        out = wave.acoustic_forward_kernel_3d(
+            nx=a["nx"], ny=a["ny"], nz=a["nz"], dx=a["dz"], dy=a["dz"], dz=a["dz"],
+            nt=np.asarray(a["src_v"]).shape[-1], dt=a["dt"], nabc=a["nabc"],
+            free_surface=a["free_surface"], src_x=a["src_x"], src_y=a["src_y"], src_z=a["src_z"],
+            src_n=len(a["src_x"]), src_v=w, rcv_x=a["rcv_x"], rcv_y=a["rcv_y"],
+            rcv_z=a["rcv_z"], rcv_n=len(a["rcv_x"]), damp=_t(a["damp"], dtype=torch.float32),
+            vp=vp, rho=rho, checkpoint_segments=segments)
+        loss = sum((_t(g) * out[k]).sum() for g, k in zip(gs, REC) if g is not None)
+        loss.backward()
+    finally:
+        K.set_shots_per_block_3d(0)
+    torch.cuda.synchronize()
+    grads = {k: t.grad.cpu().numpy() for k, t in (("vp", vp), ("rho", rho), ("w", w))
+             if k in want}
+    return {k: v.detach().cpu().numpy() for k, v in out.items()}, grads
+
+
+@pytest.mark.parametrize("name", GRAD3D_CASES)
+def test_3d_gradient_matches_reference(name):
+    print(f"[case] 3d gradient golden {name}", flush=True)
+    z, m = load(name)
+    a = oracle3d_args(z, m)
+    gs = adjoint_sources_3d(z, m)
+    want = ("vp", "w") + (("rho",) if "grad_rho" in z else ())
+    out, g = grad3(a, gs, segments=m["segments"], want=want)
+    for k in REC:
+        np.testing.assert_array_equal(out[k], z["out_" + k], err_msg=k)
+    ref = oracle.gradient3d(**a, g_p=gs[0], g_vx=gs[1], g_vy=gs[2], g_vz=gs[3],
+                            chunk=chunk_len(m["nt"], m["segments"]), precision="f64")
+    for key, zk, i in (("vp", "grad_vp", 0), ("rho", "grad_rho", 1), ("w", "grad_w", 2)):
+        if key not in g:
+            continue
+        err = rel_l2(g[key], z[zk])
+        print(f"  {key}: rel-L2 vs reference {err:.2e}, vs FP64 oracle {rel_l2(g[key], ref[i]):.2e}")
+        assert err <= GTOL, key
+
+
+GRAD_CASES = [
+    # nz, ny, nx, nabc, nt, S, free surface, seed, shared wavelet, segments, shots per block
+    (14, 11, 17, 4, 30, 3, False, 41, False, 1, 0),
+    (14, 11, 17, 4, 30, 3, True, 42, True, 4, 0),
+    (12, 9, 61, 5, 26, 4, True, 43, False, 3, 2),    # NX = 71, two shot groups
+    (7, 7, 7, 1, 20, 2, False, 44, False, 2, 1),     # smallest absorbing layer, one shot per group
+]
+
+
+@pytest.mark.parametrize("case", GRAD_CASES)
+def test_3d_gradient_matches_oracle(case):
+    print(f"[case] 3d gradient oracle {case}", flush=True)
+    nz, ny, nx, nabc, nt, S, fs, seed, shared, segs, spb = case
+    a = random_case(nz, ny, nx, nabc, nt, S, fs, seed, shared)
+    R = len(a["rcv_z"])
+    # two receivers on one cell: their adjoint sources add in receiver order
+    for k in ("rcv_z", "rcv_y", "rcv_x"):
+        a[k] = np.concatenate([a[k], a[k][2:3]])
+    R += 1
+    rng = np.random.default_rng(seed + 1000)
+    gs = [rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(4)]
+    gs[1] = None  # no adjoint source on vx
+    _, g = grad3(a, gs, segments=segs, spb=spb)
+    want = oracle.gradient3d(**a, g_p=gs[0], g_vy=gs[2], g_vz=gs[3], chunk=chunk_len(nt, segs),
+                             precision="f32")
+    for key, i in (("vp", 0), ("rho", 1), ("w", 2)):
+        err = rel_l2(g[key], want[i])
+        print(f"  {key}: rel-L2 vs FP32 oracle {err:.2e}")
+        assert err <= GTOL, key
+
+
+def test_3d_gradient_independent_of_checkpoint_interval():
+    """The recompute reproduces the forward bit for bit, so the interval cannot
+    change the gradient."""
+    print("[case] 3d gradient checkpoint intervals", flush=True)
+    a = random_case(16, 12, 20, 4, 40, 2, True, 51)
+    rng = np.random.default_rng(7)
+    R = len(a["rcv_z"])
+    gs = [rng.standard_normal((2, 40, R)).astype(np.float32) for _ in range(4)]
+    _, g1 = grad3(a, gs, segments=1)
+    for segs in (3, 40):
+        _, g2 = grad3(a, gs, segments=segs)
+        for k in g1:
+            np.testing.assert_array_equal(g1[k], g2[k], err_msg=f"{k} segments={segs}")

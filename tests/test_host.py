@@ -233,13 +233,21 @@ def _problem3(**kw):
 
 def test_workspace_queries_3d():
     lib = _native.load()
-    ws = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3()))
+    ws = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3()), 0)
     plane = 80 * 100 * 128 * 4  # NZ * NY * (NX rounded up to 64) floats
     assert ws >= 6 * plane + 4 * 4 * plane + 4 * 60 * 80 * 100 * 4
+    p = _problem3(checkpoint_every=50)
+    bwd = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(p), 1)
+    bwd_rho = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(p), 2)
+    assert bwd >= (4 + 4 + 3 + 1 + 50) * 4 * plane and bwd_rho >= bwd + 50 * 4 * plane
+    assert lib.gbw_acoustic3d_checkpoint_bytes(ctypes.byref(p)) == 10 * 4 * 4 * plane
+    assert lib.gbw_acoustic3d_checkpoint_bytes(ctypes.byref(_problem3())) == 0
+    assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3()), 1) == 0
+    assert "checkpoint_every > 0" in _native.last_error()
     for bad, msg in ((dict(ny=0), "Grid dimensions must be positive"),
                      (dict(nabc=0), "nabc"), (dict(nt=0), "nt must be positive"),
                      (dict(src_n=0), "src_n"), (dict(dt=0.0), "dt and dz")):
-        assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(**bad))) == 0
+        assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(**bad)), 0) == 0
         assert msg in _native.last_error()
 
 
