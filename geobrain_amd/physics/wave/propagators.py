@@ -1,6 +1,6 @@
 """Propagator front-end, mirroring ``geobrain/physics/wave/core/base.py:296-357``
 (``AbstractPropagator``) and ``geobrain/physics/wave/propagators/acoustic.py:23-301``
-(``AcousticPropagator``, 2-D only).
+(``AcousticPropagator``, 2-D and 3-D).
 
 Behaviour kept from the reference, on purpose:
 * the PML ``vmax`` is read from the model once, at construction
@@ -9,7 +9,9 @@ Behaviour kept from the reference, on purpose:
 * ``forward()`` first calls ``model.forward()`` (Gardner refresh + clipping);
 * checkpointing is switched off when no model parameter requires grad
   (``acoustic.py:199-202``);
-* ``survey.receiver_masks`` is stored but not applied (``acoustic.py:119``).
+* ``survey.receiver_masks`` is stored but not applied (``acoustic.py:119``);
+* for a 3-D model, locations are ``(x, y, z)`` columns (``acoustic.py:93-116``)
+  and ``snapshot_every`` is not passed to the 3-D kernel (``acoustic.py:222-247``).
 """
 
 from __future__ import annotations
@@ -21,7 +23,7 @@ import torch
 import torch.nn as nn
 
 from .boundary import create_boundary
-from .kernels import acoustic_forward_kernel
+from .kernels import acoustic_forward_kernel, acoustic_forward_kernel_3d
 from .models import AbstractModel, AcousticModel
 from .result import WaveResult
 from .survey import Survey
@@ -52,7 +54,7 @@ class AbstractPropagator(nn.Module):
 
 
 class AcousticPropagator(AbstractPropagator):
-    """2-D acoustic propagator on the MI355X HIP library.
+    """2-D / 3-D acoustic propagator on the MI355X HIP library.
 
     ``AcousticPropagator(model, survey, device, dtype)``;
     ``forward(model=None, shot_index=None, checkpoint_segments=1, snapshot_every=0)``
@@ -71,8 +73,6 @@ class AcousticPropagator(AbstractPropagator):
             raise TypeError(f"Expected AcousticModel, got {type(model).__name__}")
         if not isinstance(survey, Survey):
             raise TypeError(f"Expected Survey, got {type(survey).__name__}")
-        if model.ndim != 2:
-            raise NotImplementedError("geobrain_amd provides the 2-D acoustic propagator only")
         self.survey = survey
         self.nt = survey.nt
         self.dt = survey.dt
@@ -83,26 +83,33 @@ class AcousticPropagator(AbstractPropagator):
         src_loc = self.source.get_loc()
         self.src_n = self.source.num
         self.wavelet = numpy2tensor(self.source.get_wavelet(), dtype).to(device)
-        self.src_x = numpy2tensor(src_loc[:, 0], torch.long).to(device)
-        self.src_z = numpy2tensor(src_loc[:, 1], torch.long).to(device)
         self.receiver = survey.receiver
         rcv_loc = self.receiver.get_loc()
         self.rcv_n = self.receiver.num
+        zc = 2 if model.ndim == 3 else 1  # columns: (x, z) in 2-D, (x, y, z) in 3-D
+        self.src_x = numpy2tensor(src_loc[:, 0], torch.long).to(device)
+        self.src_z = numpy2tensor(src_loc[:, zc], torch.long).to(device)
         self.rcv_x = numpy2tensor(rcv_loc[:, 0], torch.long).to(device)
-        self.rcv_z = numpy2tensor(rcv_loc[:, 1], torch.long).to(device)
+        self.rcv_z = numpy2tensor(rcv_loc[:, zc], torch.long).to(device)
         # host copies of the geometry: the library reads positions on the host
         self._src_x_h = np.asarray(src_loc[:, 0], np.int64)
-        self._src_z_h = np.asarray(src_loc[:, 1], np.int64)
+        self._src_z_h = np.asarray(src_loc[:, zc], np.int64)
         self._rcv_x_h = np.asarray(rcv_loc[:, 0], np.int64)
-        self._rcv_z_h = np.asarray(rcv_loc[:, 1], np.int64)
+        self._rcv_z_h = np.asarray(rcv_loc[:, zc], np.int64)
+        if model.ndim == 3:
+            self.src_y = numpy2tensor(src_loc[:, 1], torch.long).to(device)
+            self.rcv_y = numpy2tensor(rcv_loc[:, 1], torch.long).to(device)
+            self._src_y_h = np.asarray(src_loc[:, 1], np.int64)
+            self._rcv_y_h = np.asarray(rcv_loc[:, 1], np.int64)
         self.receiver_masks = survey.receiver_masks
 
     def _init_boundary(self, vmax: Optional[float] = None) -> None:
         if vmax is None:
             vmax = float(self.model.vp.detach().cpu().max())
+        kw = dict(ny=self.ny, dy=self.dy) if self.model.ndim == 3 else {}
         damp = create_boundary(bc_type=self.abc_type, nx=self.nx, nz=self.nz, dx=self.dx,
                                dz=self.dz, n_layers=self.nabc, vmax=vmax,
-                               alpha=self.model.abc_alpha, free_surface=False)
+                               alpha=self.model.abc_alpha, free_surface=False, **kw)
         self.damp = numpy2tensor(damp, self.dtype).to(self.device)
 
     def forward(self, model: Optional[AcousticModel] = None, shot_index: Optional[int] = None,
@@ -111,15 +118,20 @@ class AcousticPropagator(AbstractPropagator):
         model.forward()
         if checkpoint_segments > 1 and not any(p.requires_grad for p in model.parameters()):
             checkpoint_segments = 1
-        if shot_index is not None:
-            sl = slice(shot_index, shot_index + 1)
-            src_x, src_z = self._src_x_h[sl], self._src_z_h[sl]
-            wavelet = self.wavelet[sl]
-            src_n = 1
-        else:
-            src_x, src_z = self._src_x_h, self._src_z_h
-            wavelet = self.wavelet
-            src_n = self.src_n
+        sl = slice(shot_index, shot_index + 1) if shot_index is not None else slice(None)
+        src_x, src_z = self._src_x_h[sl], self._src_z_h[sl]
+        wavelet = self.wavelet[sl]
+        src_n = 1 if shot_index is not None else self.src_n
+        if self.model.ndim == 3:
+            result = acoustic_forward_kernel_3d(
+                nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz,
+                nt=self.nt, dt=self.dt, nabc=self.nabc, free_surface=self.free_surface,
+                src_x=src_x, src_y=self._src_y_h[sl], src_z=src_z, src_n=src_n,
+                src_v=wavelet, rcv_x=self._rcv_x_h, rcv_y=self._rcv_y_h, rcv_z=self._rcv_z_h,
+                rcv_n=self.rcv_n, damp=self.damp, vp=model.vp, rho=model.rho,
+                checkpoint_segments=checkpoint_segments, device=torch.device(self.device),
+                dtype=self.dtype)
+            return WaveResult.from_dict(result)
         result = acoustic_forward_kernel(
             nx=self.nx, nz=self.nz, dx=self.dx, dz=self.dz, nt=self.nt, dt=self.dt,
             nabc=self.nabc, free_surface=self.free_surface,

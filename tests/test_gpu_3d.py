@@ -250,3 +250,51 @@ def test_3d_gradient_independent_of_checkpoint_interval():
         _, g2 = grad3(a, gs, segments=segs)
         for k in g1:
             np.testing.assert_array_equal(g1[k], g2[k], err_msg=f"{k} segments={segs}")
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_3d_propagator_end_to_end(fs):
+    """AcousticPropagator on a 3-D model (Gardner density, (x, y, z) survey
+    columns, 3-D PML from create_boundary), forward and vp gradient, against
+    the FP32 oracle."""
+    print(f"[case] 3d propagator fs={fs}", flush=True)
+    nz, ny, nx, nt, dx, dt, nabc = 20, 16, 24, 150, 10.0, 1e-3, 6
+    vp = np.asarray(np.linspace(1500, 3000, nz)[:, None, None] * np.ones((1, ny, nx)), np.float32)
+    grid = wave.GridConfig(nx=nx, nz=nz, dx=dx, dz=dx, ny=ny, dy=dx)
+    bnd = wave.BoundaryConfig(type="pml", n_layers=nabc, free_surface=fs)
+    model = wave.AcousticModel(grid, bnd, vp=vp, rho=np.ones_like(vp), vp_grad=True,
+                               device="cuda")
+    w, _ = wave.RickerWavelet()(f0=20.0, dt=dt)
+    w = np.pad(w.numpy(), (0, nt - len(w)))
+    src = wave.Source(nt=nt, dt=dt, f0=20.0)
+    sxy = [(5, 4), (12, 8), (20, 11)]
+    for x, y in sxy:
+        src.add_source(x=x, y=y, z=1, wavelet=w)
+    rcv = wave.Receiver(nt=nt, dt=dt)
+    gx, gy = np.meshgrid(np.arange(0, nx, 3), np.arange(0, ny, 3))
+    rcv.add_receivers(x=gx.reshape(-1), y=gy.reshape(-1), z=np.full(gx.size, 2),
+                      rcv_type="pr")
+    prop = wave.AcousticPropagator(model, wave.Survey(src, rcv), device="cuda")
+    assert tuple(prop.damp.shape) == (nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
+    res = prop.forward(checkpoint_segments=4)
+    R = gx.size
+    gs = np.random.default_rng(3).standard_normal((3, nt, R)).astype(np.float32)
+    (res.p * _t(gs)).sum().backward()
+    rho = (310.0 * np.power(vp, 0.25)).astype(np.float32)
+    np.testing.assert_array_equal(model.rho.detach().cpu().numpy(), rho)
+    damp = wave.bc_pml_3d(nx, ny, nz, dx, dx, dx, nabc, float(vp.max()), False)
+    args = dict(nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=fs, dt=dt, dz=dx,
+                src_z=[1, 1, 1], src_y=[y for _, y in sxy], src_x=[x for x, _ in sxy],
+                src_v=np.stack([w] * 3).astype(np.float32), rcv_z=np.full(R, 2),
+                rcv_y=gy.reshape(-1), rcv_x=gx.reshape(-1), vp=vp, rho=rho,
+                damp=damp.astype(np.float32))
+    ref = oracle.forward3d(**args, chunk=chunk_len(nt, 4), precision="f32")
+    assert res.ndim == 3
+    for k in REC:
+        np.testing.assert_array_equal(res[k].detach().cpu().numpy(), ref[k], err_msg=k)
+    gv, _, _ = oracle.gradient3d(**args, g_p=gs, chunk=chunk_len(nt, 4), precision="f32")
+    err = rel_l2(model.vp.grad.cpu().numpy(), gv)
+    print(f"  vp gradient: rel-L2 vs FP32 oracle {err:.2e}")
+    assert err <= GTOL
+    one = prop.forward(shot_index=1)
+    np.testing.assert_array_equal(one.p.detach().cpu().numpy()[0], res.p.detach().cpu().numpy()[1])
