@@ -260,7 +260,8 @@ def workload_3d(n_shots=8, n=200, nt=1000, nabc=20, dx=10.0, dt=8e-4, f0=10.0):
         rcv_z=np.full(rx.size, 2, np.int64))
 
 
-def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 60) -> dict:
+def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 900,
+               nt: int = 1000) -> dict:
     """Times acoustic_forward_kernel_3d (all nt steps = one bench step) on
     ``workload_3d``; roofline of the time step at its algorithmic 32 B per
     cell-update (p, vx, vy, vz each read and written once)."""
@@ -268,7 +269,7 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
     from geobrain_amd import _native
     from geobrain_amd.physics import wave
     dev = torch.device("cuda:0")
-    w = workload_3d()
+    w = workload_3d(nt=nt)
     S, nt = w["wavelet"].shape
     T = lambda a, **kw: torch.as_tensor(a, device=dev, **kw)  # noqa: E731
     args = dict(nx=w["nx"], ny=w["ny"], nz=w["nz"], dx=w["dx"], dy=w["dx"], dz=w["dx"], nt=nt,
@@ -289,7 +290,8 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
     _native.set_kernel_timing(False)
     pad = [w["nz"] + 2 * w["nabc"], w["ny"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]]
     cells = S * pad[0] * pad[1] * pad[2]
-    step_kernels = [k for k in kt if k in ("p3_step_kernel", "v3_step_kernel", "fwd3_kernel")]
+    step_kernels = [k for k in kt if k in ("p3_step_kernel", "v3_step_kernel",
+                                           "fwd3_march_kernel<forward>")]
     kernels = {k: {"launches": n, "total_s": round(ms / 1e3, 4),
                    "avg_launch_us": round(ms / n * 1e3, 2)} for k, (n, ms) in kt.items()}
     # one time step = one launch of each step kernel
@@ -443,6 +445,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward workload")
+    ap.add_argument("--nt3d", type=int, default=1000, help="time steps of the 3D workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -469,7 +472,7 @@ def main():
         print(json.dumps(fwi_precision(n_iter=args.fwi_iters)), flush=True)
         return
     if args.workload == "fwd3d":
-        rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu)
+        rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu, nt=args.nt3d)
         rec.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                     "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
         print(json.dumps(rec), flush=True)
@@ -609,7 +612,7 @@ def main():
                                            launch=blaunch)
     if not args.no_3d:
         torch.cuda.empty_cache()
-        rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu)
+        rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d)
     print(json.dumps(rec), flush=True)
 
 

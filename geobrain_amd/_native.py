@@ -83,7 +83,8 @@ class Problem3(ctypes.Structure):
         ("nx", c_i32), ("ny", c_i32), ("nz", c_i32), ("dx", c_f32), ("dy", c_f32),
         ("dz", c_f32), ("nt", c_i32), ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32),
         ("src_n", c_i32), ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
-        ("shots_per_block", c_i32), ("checkpoint_every", c_i32),
+        ("shots_per_block", c_i32), ("checkpoint_every", c_i32), ("step_kernel", c_i32),
+        ("z_chunk", c_i32),
     ]
 
 

@@ -632,6 +632,315 @@ __global__ __launch_bounds__(T3) void adj3_v_kernel(Adj3Args a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused z-marching step (the default).  A workgroup owns a 64 x 16 (x, y)
+// tile, a range of z planes and a group of SPB shots, and marches down z.  At
+// plane zp it computes p' on the tile and on the halo cells the velocity
+// stencils read (one column left, two right, one row above, two below) into
+// LDS, then vx', vy' at zp and vz' at zp - 2, whose stencil needs p' of
+// zp - 3 .. zp (a register queue).  vz of zp - 2 .. zp + 1, the z reach of the
+// p stencil, is a register queue too, so every field is read from HBM once per
+// step.  The state is out of place (read A, write B): halo cells are
+// recomputed from A while neighbouring tiles write B.
+// ---------------------------------------------------------------------------
+constexpr int MT3 = 256;           // threads: 16 float4 groups x 16 rows
+constexpr int MLR = 19, MLG = 18;  // LDS tile: rows y0-1 .. y0+17, groups g0-1 .. g0+16
+
+struct March3Args {
+  Dims3 d;
+  int S, it, nt;
+  float dt;
+  const float* A;          // state before the step, [field][shot][plane]
+  float* B;                // state after the step
+  long long fstride;       // S * plane
+  const float* coef;
+  const int* src_cell;
+  const float* wav; long long wav_stride;
+  float* energy;           // [group][4][nz][ny][nx] or null
+  float* dstore;           // recompute: p-update bracket [shot][plane], or null
+  float* pstore;           // recompute: p after the update [shot][plane], or null
+  int tiles_x, tiles_y, zc, groups;
+  long long units;
+};
+
+__device__ __forceinline__ float4 z4() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+
+// p' of the float4 group (z, y, g) of one shot: the p-region update from the
+// old state, then the source term.  vz holds vz of planes z-2 .. z+1; the
+// group's own vx and vy are returned for the velocity update.
+template <bool DSUM>
+__device__ __forceinline__ float4 march_p(const Dims3& d, const float4* P, const float4* VX,
+                                          const float4* VY, const float4* vz, int z, int y,
+                                          int g, float4 omk1, float4 a1, int sc, float w,
+                                          float dt, float4& vxc, float4& vyc, float4& dsum) {
+  const long long sy4 = d.GX;
+  const long long q4 = ((long long)z * d.NY + y) * d.GX + g;
+  float4 p = P[q4];
+  vxc = VX[q4];
+  vyc = VY[q4];
+  unsigned mask = 0;
+  if (z >= d.f + 1 && z <= d.NZ - 3 && y >= 2 && y <= d.NY - 3)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int x = 4 * g + e;
+      if (x >= 2 && x <= d.NX - 3) mask |= 1u << e;
+    }
+  if (mask) {
+    const float4 xa = ld4(VX, q4 - 1, g >= 1), xc = ld4(VX, q4 + 1, g + 1 < d.GX);
+    const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, vxc.x, vxc.y, vxc.z, vxc.w,
+                           xc.x, xc.y, xc.z, xc.w};
+    const float4 ym2 = VY[q4 - 2 * sy4], ym1 = VY[q4 - sy4], yp1 = VY[q4 + sy4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (!((mask >> e) & 1u)) continue;
+      const float dvx = d4(vxs[4 + e], vxs[3 + e], vxs[5 + e], vxs[2 + e]);
+      const float dvy = d4(g3(vyc, e), g3(ym1, e), g3(yp1, e), g3(ym2, e));
+      const float dvz = d4(g3(vz[2], e), g3(vz[1], e), g3(vz[3], e), g3(vz[0], e));
+      float sum = dvx + dvy;
+      sum = sum + dvz;
+      if (DSUM) s3(dsum, e, sum);
+      const float u = g3(omk1, e) * g3(p, e);
+      s3(p, e, u - g3(a1, e) * sum);
+    }
+  }
+  if (sc >= 0 && (long long)(sc >> 2) == q4) {
+    const int e = sc & 3;
+    s3(p, e, g3(p, e) + dt * w);
+  }
+  return p;
+}
+
+// energy partial += acc over the interior lanes of group (z, y, g)
+__device__ __forceinline__ void energy_add(const Dims3& d, float* eg, int z, int y, int g,
+                                           float4 acc) {
+  if (z < d.nabc || z >= d.nabc + d.nz || y < d.nabc || y >= d.nabc + d.ny) return;
+  const long long ob = ((long long)(z - d.nabc) * d.ny + (y - d.nabc)) * d.nx - d.nabc;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int x = 4 * g + e;
+    if (x < d.nabc || x >= d.nabc + d.nx) continue;
+    eg[ob + x] = eg[ob + x] + g3(acc, e);
+  }
+}
+
+template <int SPB, bool REC>
+__global__ __launch_bounds__(MT3) void fwd3_march_kernel(March3Args a) {
+  __shared__ float4 lp[SPB][2][MLR][MLG];
+  const Dims3 d = a.d;
+  // XCD-aware: consecutive units (the shot groups of a tile, then neighbouring tiles) share an XCD
+  const long long L = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (L >= a.units) return;
+  const int grp = (int)(L % a.groups);
+  long long t = L / a.groups;
+  const int tx = (int)(t % a.tiles_x);
+  t /= a.tiles_x;
+  const int tyi = (int)(t % a.tiles_y);
+  const int zci = (int)(t / a.tiles_y);
+  const int g0 = tx * 16, y0 = tyi * 16;
+  const int zb = zci * a.zc, ze = min(d.NZ, zb + a.zc);
+  const int s0 = grp * SPB, ns = min(SPB, a.S - s0);
+  const int tid = threadIdx.x, gx = tid & 15, ty = tid >> 4;
+  const int g = g0 + gx, y = y0 + ty;
+  const bool tin = y < d.NY;
+  // halo item of threads 0..79: left column, right column, the row above, the two rows below
+  int hg = 0, hy = 0;
+  if (tid < 16) { hg = g0 - 1; hy = y0 + tid; }
+  else if (tid < 32) { hg = g0 + 16; hy = y0 + tid - 16; }
+  else if (tid < 48) { hg = g0 + tid - 32; hy = y0 - 1; }
+  else if (tid < 80) { hg = g0 + (tid - 48) % 16; hy = y0 + 16 + (tid - 48) / 16; }
+  const bool hslot = tid < 80;
+  const bool hin = hslot && hg >= 0 && hg < d.GX && hy >= 0 && hy < d.NY;
+  const int hr = hy - y0 + 1, hc = hg - g0 + 1;
+  const long long sz4 = (long long)d.NY * d.GX, p4 = d.plane >> 2;
+  const long long f4 = a.fstride >> 2;
+  const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
+  const float4* __restrict__ A4 = reinterpret_cast<const float4*>(a.A);
+  float4* __restrict__ B4 = reinterpret_cast<float4*>(a.B);
+  // this chunk holds the mirrored plane f-1: it also writes p[f-1] = -p'[f+1], vz[f-1] = vz'[f]
+  const bool fsm = d.fs && d.f - 1 >= zb && d.f - 1 < ze;
+  const int zs = max(zb - 1, 0);
+  const int zend = fsm ? max(ze + 1, d.f + 2) : ze + 1;
+  int sc[SPB];
+  float wv[SPB];
+  long long so4[SPB];
+#pragma unroll
+  for (int k = 0; k < SPB; ++k) {
+    const int s = s0 + min(k, ns - 1);
+    sc[k] = k < ns ? a.src_cell[s] : -1;
+    wv[k] = k < ns ? a.wav[(long long)s * a.wav_stride + a.it] : 0.f;
+    so4[k] = (long long)s * p4;
+  }
+  const long long row4 = (long long)y * d.GX + g;  // (y, g) within a plane
+  float4 vzq[SPB][4], pq[SPB][4];
+#pragma unroll
+  for (int k = 0; k < SPB; ++k) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int zz = zs - 2 + j;
+      vzq[k][j] = (tin && k < ns && zz >= 0) ? A4[3 * f4 + so4[k] + zz * sz4 + row4] : z4();
+      pq[k][j] = z4();
+    }
+    vzq[k][3] = z4();
+    pq[k][3] = z4();
+  }
+  const long long inner = (long long)d.nz * d.ny * d.nx;
+  float* eg = a.energy ? a.energy + (long long)grp * 4 * inner : nullptr;
+  int par = 0;
+  for (int zp = zs; zp <= zend; ++zp) {
+    const bool hp = zp < d.NZ;
+    float4 vxc[SPB], vyc[SPB];
+    // phase A: p' of plane zp on the tile and its halo, into LDS
+    if (hp && tin) {
+      const long long q4 = zp * sz4 + row4;
+      const float4 omk1 = C[q4], a1 = C[p4 + q4];
+#pragma unroll
+      for (int k = 0; k < SPB; ++k) {
+        if (k >= ns) continue;
+        const float4* P = A4 + so4[k];
+        vzq[k][3] = zp + 1 < d.NZ ? P[3 * f4 + q4 + sz4] : z4();
+        float4 ds = z4();
+        const float4 pn = march_p<REC>(d, P, P + f4, P + 2 * f4, vzq[k], zp, y, g, omk1, a1,
+                                       sc[k], wv[k], a.dt, vxc[k], vyc[k], ds);
+        lp[k][par][ty + 1][gx + 1] = pn;
+        pq[k][3] = pn;
+        if (REC && zp >= zb && zp < ze) {
+          reinterpret_cast<float4*>(a.dstore)[so4[k] + q4] = ds;
+          if (a.pstore) reinterpret_cast<float4*>(a.pstore)[so4[k] + q4] = pn;
+        }
+      }
+    }
+    if (hp && hslot) {
+      if (hin) {
+        const long long q4 = (zp * (long long)d.NY + hy) * d.GX + hg;
+        const float4 omk1 = C[q4], a1 = C[p4 + q4];
+#pragma unroll
+        for (int k = 0; k < SPB; ++k) {
+          if (k >= ns) continue;
+          const float4* P = A4 + so4[k];
+          const float4* VZ = P + 3 * f4;
+          float4 hv[4];
+          hv[0] = zp >= 2 ? VZ[q4 - 2 * sz4] : z4();
+          hv[1] = zp >= 1 ? VZ[q4 - sz4] : z4();
+          hv[2] = VZ[q4];
+          hv[3] = zp + 1 < d.NZ ? VZ[q4 + sz4] : z4();
+          float4 hx, hyv, ds;
+          lp[k][par][hr][hc] = march_p<false>(d, P, P + f4, P + 2 * f4, hv, zp, hy, hg, omk1, a1,
+                                              sc[k], wv[k], a.dt, hx, hyv, ds);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < SPB; ++k) lp[k][par][hr][hc] = z4();
+      }
+    }
+    if (!hp) {
+#pragma unroll
+      for (int k = 0; k < SPB; ++k) { vzq[k][3] = z4(); pq[k][3] = z4(); }
+    }
+    __syncthreads();
+    // phase B: vx', vy' at zp; vz' at zp - 2
+    if (tin) {
+      const int zv = zp - 2;
+      if (hp && zp >= zb && zp < ze) {
+        const long long q4 = zp * sz4 + row4;
+        const float4 omk2 = C[2 * p4 + q4], omk3 = C[3 * p4 + q4], a2 = C[5 * p4 + q4];
+        unsigned mx = 0, my = 0;
+        const bool zx = zp >= d.f && zp <= d.NZ - 2;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int x = 4 * g + e;
+          if (zx && y >= 1 && y <= d.NY - 2 && x >= 1 && x <= d.NX - 3) mx |= 1u << e;
+          if (zx && y >= 1 && y <= d.NY - 3 && x >= 1 && x <= d.NX - 2) my |= 1u << e;
+        }
+        float4 ep = z4(), ex = z4(), ey = z4();
+#pragma unroll
+        for (int k = 0; k < SPB; ++k) {
+          if (k >= ns) continue;
+          const float4 pa = lp[k][par][ty + 1][gx], pb = lp[k][par][ty + 1][gx + 1];
+          const float4 pc = lp[k][par][ty + 1][gx + 2];
+          const float4 pu = lp[k][par][ty][gx + 1], pd1 = lp[k][par][ty + 2][gx + 1];
+          const float4 pd2 = lp[k][par][ty + 3][gx + 1];
+          const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w,
+                                pc.x, pc.y, pc.z, pc.w};
+          float4 vx = vxc[k], vy = vyc[k];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            if ((mx >> e) & 1u) {
+              const float u = g3(omk2, e) * g3(vx, e);
+              s3(vx, e, u - g3(a2, e) * d4(ps[5 + e], ps[4 + e], ps[6 + e], ps[3 + e]));
+            }
+            if ((my >> e) & 1u) {
+              const float u = g3(omk3, e) * g3(vy, e);
+              s3(vy, e, u - g3(a2, e) * d4(g3(pd1, e), g3(pb, e), g3(pd2, e), g3(pu, e)));
+            }
+          }
+          float4* Bk = B4 + so4[k];
+          Bk[q4] = pb;
+          Bk[f4 + q4] = vx;
+          Bk[2 * f4 + q4] = vy;
+          if (eg) {
+            ep = make_float4(ep.x + pb.x * pb.x, ep.y + pb.y * pb.y, ep.z + pb.z * pb.z, ep.w + pb.w * pb.w);
+            ex = make_float4(ex.x + vx.x * vx.x, ex.y + vx.y * vx.y, ex.z + vx.z * vx.z, ex.w + vx.w * vx.w);
+            ey = make_float4(ey.x + vy.x * vy.x, ey.y + vy.y * vy.y, ey.z + vy.z * vy.z, ey.w + vy.w * vy.w);
+          }
+        }
+        if (eg) {
+          energy_add(d, eg, zp, y, g, ep);
+          energy_add(d, eg + inner, zp, y, g, ex);
+          energy_add(d, eg + 2 * inner, zp, y, g, ey);
+        }
+      }
+      if (hp && fsm && zp == d.f + 1) {  // p[f-1] = -p'[f+1] (plane f+1 may be the next chunk's)
+        const long long q4 = (long long)(d.f - 1) * sz4 + row4;
+#pragma unroll
+        for (int k = 0; k < SPB; ++k)
+          if (k < ns)
+            B4[so4[k] + q4] = make_float4(-pq[k][3].x, -pq[k][3].y, -pq[k][3].z, -pq[k][3].w);
+      }
+      const bool vin = zv >= zb && zv < ze;
+      if (vin || (fsm && zv == d.f)) {
+        const long long q4 = zv * sz4 + row4;
+        const float4 omk4 = C[4 * p4 + q4], a2 = C[5 * p4 + q4];
+        unsigned mz = 0;
+        if (zv >= d.f && zv <= d.NZ - 3 && y >= 1 && y <= d.NY - 2)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int x = 4 * g + e;
+            if (x >= 1 && x <= d.NX - 2) mz |= 1u << e;
+          }
+        const bool mirror = d.fs && zv == d.f;  // p[f-1] is read as -p[f+1]
+        float4 ez = z4();
+#pragma unroll
+        for (int k = 0; k < SPB; ++k) {
+          if (k >= ns) continue;
+          float4 vz = vzq[k][0];
+          if (mz) {
+            const float4 pm = mirror ? make_float4(-pq[k][2].x, -pq[k][2].y, -pq[k][2].z, -pq[k][2].w)
+                                     : pq[k][0];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              if (!((mz >> e) & 1u)) continue;
+              const float u = g3(omk4, e) * g3(vz, e);
+              s3(vz, e, u - g3(a2, e) * d4(g3(pq[k][2], e), g3(pq[k][1], e), g3(pq[k][3], e),
+                                            g3(pm, e)));
+            }
+          }
+          float4* Bk = B4 + so4[k];
+          if (vin) Bk[3 * f4 + q4] = vz;
+          if (fsm && zv == d.f) Bk[3 * f4 + q4 - sz4] = vz;
+          if (eg && vin) ez = make_float4(ez.x + vz.x * vz.x, ez.y + vz.y * vz.y, ez.z + vz.z * vz.z, ez.w + vz.w * vz.w);
+        }
+        if (eg && vin) energy_add(d, eg + 3 * inner, zv, y, g, ez);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SPB; ++k) {
+      vzq[k][0] = vzq[k][1]; vzq[k][1] = vzq[k][2]; vzq[k][2] = vzq[k][3];
+      pq[k][0] = pq[k][1]; pq[k][1] = pq[k][2]; pq[k][2] = pq[k][3];
+    }
+    par ^= 1;
+  }
+}
+
 // adjoint sources of step `it`: thread (shot, receiver cell) adds the g of
 // every receiver on that cell, in receiver order
 __global__ void inject3_kernel(long long plane, int S, int nt, int it, int R, int ncell,
@@ -708,6 +1017,9 @@ inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Plan3 {
   Dims3 d;
   int S, R, nt, spb, groups, echunk, k, nseg;
+  bool march;                       // fused z-marching step (state in and out of place)
+  int tiles_x, tiles_y, zc;         // z-marching decomposition
+  long long units;
   size_t off_coef, off_state, off_E, off_src, off_rcell;
   size_t off_lam, off_q, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
       off_urcv;
@@ -740,10 +1052,31 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   if (d.NZ < 6 || d.NY < 6 || d.NX < 6) return fail(GBW_ERR_INVALID, "padded grid must be at least 6 per axis");
   if (d.plane >= (1LL << 31)) return fail(GBW_ERR_INVALID, "padded volume exceeds 2^31 cells");
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
-  int spb = p->shots_per_block > 0 ? std::min(p->shots_per_block, pl.S) : pl.S;
+  if (p->step_kernel != 0 && p->step_kernel != 1)
+    return fail(GBW_ERR_INVALID, "step_kernel must be 0 or 1");
+  if (p->z_chunk < 0) return fail(GBW_ERR_INVALID, "z_chunk must be >= 0");
+  pl.march = p->step_kernel == 1;
+  int spb;
+  if (pl.march) {
+    spb = p->shots_per_block > 0 ? p->shots_per_block : 4;
+    if (spb != 1 && spb != 2 && spb != 4)
+      return fail(GBW_ERR_INVALID, "shots_per_block must be 0, 1, 2 or 4 with the z-marching kernel");
+  } else {
+    spb = p->shots_per_block > 0 ? std::min(p->shots_per_block, pl.S) : pl.S;
+  }
   pl.spb = spb;
   pl.groups = (pl.S + spb - 1) / spb;
   if (pl.groups > 65535) return fail(GBW_ERR_INVALID, "more than 65535 shot groups: raise shots_per_block");
+  pl.tiles_x = d.GX / 16;
+  pl.tiles_y = (d.NY + 15) / 16;
+  {
+    // about eight workgroups per CU, each marching at least 8 planes
+    const long long per_z = (long long)pl.tiles_x * pl.tiles_y * pl.groups;
+    long long nzc = std::max<long long>(1, (2048 + per_z / 2) / per_z);
+    nzc = std::min<long long>(nzc, std::max(1, d.NZ / 8));
+    pl.zc = p->z_chunk > 0 ? std::min(p->z_chunk, d.NZ) : (int)((d.NZ + nzc - 1) / nzc);
+    pl.units = per_z * ((d.NZ + pl.zc - 1) / pl.zc);
+  }
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
   pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
   pl.nseg = pl.k > 0 ? (pl.nt + pl.k - 1) / pl.k : 0;
@@ -753,7 +1086,7 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o += al256(bytes); return at; };
   pl.off_coef = take(6 * (size_t)d.plane * sizeof(float));
-  pl.off_state = take(4 * fld);
+  pl.off_state = take((pl.march ? 8 : 4) * fld);
   pl.off_src = take((size_t)pl.S * sizeof(int));
   pl.off_rcell = take(R1 * sizeof(int));
   if (pass == 0) {
@@ -814,7 +1147,7 @@ int geometry3(const Plan3& pl, const gbw_acoustic3d_inputs* in, std::vector<int>
 
 unsigned step_blocks3(const Dims3& d) { return (unsigned)(((d.groups + T3 - 1) / T3 + 7) / 8 * 8); }
 
-// one forward step of every shot (p, then v); records and energy when asked
+// one forward step of every shot (p, then v)
 int forward_step3(Step3Args& a, const dim3& grid, hipStream_t st) {
   {
     LaunchTimer timer("p3_step_kernel", st);
@@ -824,6 +1157,25 @@ int forward_step3(Step3Args& a, const dim3& grid, hipStream_t st) {
   {
     LaunchTimer timer("v3_step_kernel", st);
     v3_step_kernel<<<grid, T3, 0, st>>>(a);
+  }
+  LAUNCH_CHECK();
+  return 0;
+}
+
+// one z-marching step: state A -> B
+int march_step3(const Plan3& pl, March3Args& m, bool rec, hipStream_t st) {
+  m.tiles_x = pl.tiles_x; m.tiles_y = pl.tiles_y; m.zc = pl.zc; m.groups = pl.groups;
+  m.units = pl.units;
+  const unsigned blocks = (unsigned)((pl.units + 7) / 8 * 8);
+  LaunchTimer timer(rec ? "fwd3_march_kernel<recompute>" : "fwd3_march_kernel<forward>", st);
+  switch (pl.spb * 2 + (rec ? 1 : 0)) {
+    case 2: fwd3_march_kernel<1, false><<<blocks, MT3, 0, st>>>(m); break;
+    case 3: fwd3_march_kernel<1, true><<<blocks, MT3, 0, st>>>(m); break;
+    case 4: fwd3_march_kernel<2, false><<<blocks, MT3, 0, st>>>(m); break;
+    case 5: fwd3_march_kernel<2, true><<<blocks, MT3, 0, st>>>(m); break;
+    case 8: fwd3_march_kernel<4, false><<<blocks, MT3, 0, st>>>(m); break;
+    case 9: fwd3_march_kernel<4, true><<<blocks, MT3, 0, st>>>(m); break;
+    default: return fail(GBW_ERR_INVALID, "unsupported shots_per_block");
   }
   LAUNCH_CHECK();
   return 0;
@@ -890,16 +1242,28 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
   a.wav = in->src_v; a.wav_stride = prob->wavelet_shared ? 0 : prob->nt;
   a.energy = energy ? E : nullptr;
   const dim3 grid(step_blocks3(d), pl.groups);
+  float* cur = state;
+  float* nxt = state + 4 * fstride;
+  March3Args m{};
+  m.d = d; m.S = pl.S; m.nt = pl.nt; m.dt = prob->dt; m.fstride = fstride; m.coef = coef;
+  m.src_cell = a.src_cell; m.wav = a.wav; m.wav_stride = a.wav_stride; m.energy = a.energy;
+  if (pl.march && (rc = fill3(nxt, 4 * fstride, 0.f, st))) return rc;
   for (int it = 0; it < pl.nt; ++it) {
     if (pl.k > 0 && it % pl.k == 0)
       HIP_TRY(hipMemcpyAsync(static_cast<char*>(out->checkpoints) +
                                  (size_t)(it / pl.k) * 4 * fstride * sizeof(float),
-                             state, 4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
-    a.it = it;
-    if ((rc = forward_step3(a, grid, st))) return rc;
+                             cur, 4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (pl.march) {
+      m.it = it; m.A = cur; m.B = nxt;
+      if ((rc = march_step3(pl, m, false, st))) return rc;
+      std::swap(cur, nxt);
+    } else {
+      a.it = it;
+      if ((rc = forward_step3(a, grid, st))) return rc;
+    }
     if (rec && pl.R > 0) {
       record3_kernel<<<(unsigned)(((long long)pl.S * pl.R + 255) / 256), 256, 0, st>>>(
-          d, pl.S, pl.R, pl.nt, it, state, fstride, at3<int>(workspace, pl.off_rcell), out->rec_p,
+          d, pl.S, pl.R, pl.nt, it, cur, fstride, at3<int>(workspace, pl.off_rcell), out->rec_p,
           out->rec_vx, out->rec_vy, out->rec_vz);
       LAUNCH_CHECK();
     }
@@ -981,6 +1345,9 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   a.state = state; a.fstride = fstride; a.coef = coef;
   a.src_cell = at3<int>(workspace, pl.off_src);
   a.wav = in->src_v; a.wav_stride = shared ? 0 : prob->nt;
+  March3Args m{};
+  m.d = d; m.S = pl.S; m.nt = pl.nt; m.dt = prob->dt; m.fstride = fstride; m.coef = coef;
+  m.src_cell = a.src_cell; m.wav = a.wav; m.wav_stride = a.wav_stride;
   Adj3Args b{};
   b.d = d; b.S = pl.S; b.spb = pl.spb; b.nt = pl.nt; b.dt = prob->dt;
   b.lam = lam; b.fstride = fstride; b.q = q; b.r = r; b.coef = coef;
@@ -995,11 +1362,19 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
                            static_cast<const char*>(adj->checkpoints) +
                                (size_t)c * 4 * fstride * sizeof(float),
                            4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
+    float* cur = state;
+    float* nxt = state + 4 * fstride;
     for (int it = t0; it < t1; ++it) {  // recompute: brackets (and p) of every step
-      a.it = it;
-      a.dstore = D + (size_t)(it - t0) * fstride;
-      a.pstore = P1 ? P1 + (size_t)(it - t0) * fstride : nullptr;
-      if ((rc = forward_step3(a, grid, st))) return rc;
+      float* dst = D + (size_t)(it - t0) * fstride;
+      float* pst = P1 ? P1 + (size_t)(it - t0) * fstride : nullptr;
+      if (pl.march) {
+        m.it = it; m.A = cur; m.B = nxt; m.dstore = dst; m.pstore = pst;
+        if ((rc = march_step3(pl, m, true, st))) return rc;
+        std::swap(cur, nxt);
+      } else {
+        a.it = it; a.dstore = dst; a.pstore = pst;
+        if ((rc = forward_step3(a, grid, st))) return rc;
+      }
     }
     for (int it = t1 - 1; it >= t0; --it) {
       if (inject) {

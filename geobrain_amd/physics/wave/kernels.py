@@ -347,13 +347,31 @@ def pad_model_3d(v: Tensor, pml: int, nz: int, ny: int, nx: int,
     return v.to(torch.float32)[ii][:, jj][:, :, kk].to(device)
 
 
-_shots_per_block_3d: int = 0
+@dataclass
+class LaunchChoice3D:
+    """Launch parameters of the 3D library calls: ``step_kernel`` 0 (separate
+    p and v kernels, default) or 1 (fused z-marching step);
+    ``shots_per_block`` (any with the p / v kernels; 1, 2 or 4 with the
+    z-marching kernel); ``z_chunk`` (z planes per z-marching workgroup).
+    Zeros select the library's defaults."""
+    shots_per_block: int = 0
+    step_kernel: int = 0
+    z_chunk: int = 0
+
+
+_launch3: LaunchChoice3D = LaunchChoice3D()
+
+
+def set_launch_choice_3d(choice: Optional[LaunchChoice3D]) -> None:
+    """Pin the 3D launch parameters for subsequent calls (``None`` = defaults)."""
+    global _launch3
+    _launch3 = choice if choice is not None else LaunchChoice3D()
 
 
 def set_shots_per_block_3d(n: int) -> None:
-    """Shots one thread of the 3D kernels steps through (0 = library default)."""
-    global _shots_per_block_3d
-    _shots_per_block_3d = int(n)
+    """Shots per workgroup of the 3D kernels (0 = library default)."""
+    global _launch3
+    _launch3 = LaunchChoice3D(int(n), _launch3.step_kernel, _launch3.z_chunk)
 
 
 def auto_checkpoint_interval_3d(nt: int, rho_grad: bool) -> int:
@@ -529,7 +547,9 @@ def acoustic_forward_kernel_3d(
     prob.src_n, prob.rcv_n = int(src_n), int(rcv_n)
     prob.wavelet_shared = shared
     prob.energy_chunk = chunk_length(nt, checkpoint_segments)
-    prob.shots_per_block = _shots_per_block_3d
+    prob.shots_per_block = _launch3.shots_per_block
+    prob.step_kernel = _launch3.step_kernel
+    prob.z_chunk = _launch3.z_chunk
     if checkpoint_segments > 1:
         prob.checkpoint_every = chunk_length(nt, checkpoint_segments)
     else:

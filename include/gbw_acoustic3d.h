@@ -42,8 +42,12 @@ typedef struct gbw_acoustic3d_problem {
   int32_t rcv_n;             /* receivers shared by every shot                         */
   int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
   int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
-  int32_t shots_per_block;   /* launch choice: shots one thread steps (0 = all)        */
+  int32_t shots_per_block;   /* launch choice: shots per workgroup (0 = all); 1, 2 or 4
+                                with the z-marching kernel (0 = 4)                     */
   int32_t checkpoint_every;  /* steps per checkpoint interval (0 = no checkpoints)     */
+  int32_t step_kernel;       /* launch choice: 0 separate p and v step kernels (default),
+                                1 fused z-marching step                                */
+  int32_t z_chunk;           /* launch choice: z planes per z-marching workgroup (0 = auto) */
 } gbw_acoustic3d_problem;
 
 /* Geometry and model (acoustic_3d.py:286-297); positions are unpadded indices. */

@@ -25,9 +25,9 @@ def _t(a, **kw):
     return torch.as_tensor(np.asarray(a), device=DEV, **kw)
 
 
-def run3(a, segments=1, spb=0):
+def run3(a, segments=1, spb=0, kernel=0, zc=0):
     """acoustic_forward_kernel_3d on oracle-style arguments ``a``."""
-    K.set_shots_per_block_3d(spb)
+    K.set_launch_choice_3d(K.LaunchChoice3D(spb, kernel, zc))
     try:
         out = wave.acoustic_forward_kernel_3d(
             nx=a["nx"], ny=a["ny"], nz=a["nz"], dx=a["dz"], dy=a["dz"], dz=a["dz"],
@@ -39,7 +39,7 @@ def run3(a, segments=1, spb=0):
             vp=_t(a["vp"], dtype=torch.float32), rho=_t(a["rho"], dtype=torch.float32),
             checkpoint_segments=segments)
     finally:
-        K.set_shots_per_block_3d(0)
+        K.set_launch_choice_3d(None)
     torch.cuda.synchronize()
     return {k: v.cpu().numpy() for k, v in out.items()}
 
@@ -58,16 +58,22 @@ def _check(got, want, exact_energy=True):
 def test_3d_forward_matches_reference(name):
     print(f"[case] 3d golden {name}", flush=True)
     z, m = load(name)
-    got = run3(oracle3d_args(z, m), segments=m["segments"])
-    _check(got, {k: z["out_" + k] for k in OUT3D_KEYS})
+    want = {k: z["out_" + k] for k in OUT3D_KEYS}
+    # one shot group (the default): the energy sums keep the reference's order
+    _check(run3(oracle3d_args(z, m), segments=m["segments"]), want)
 
 
+@pytest.mark.parametrize("launch", [(1, 1), (4, 1), (1, 0), (0, 0)])
 @pytest.mark.parametrize("name", FWD3D_CASES)
-def test_3d_shot_groups(name):
-    print(f"[case] 3d golden {name} one shot per group", flush=True)
+def test_3d_launch_choices(name, launch):
+    """Every kernel and shot grouping gives the reference's seismograms bit for bit."""
+    spb, kernel = launch
+    print(f"[case] 3d golden {name} shots_per_block {spb} step_kernel {kernel}", flush=True)
     z, m = load(name)
-    got = run3(oracle3d_args(z, m), segments=m["segments"], spb=1)
-    _check(got, {k: z["out_" + k] for k in OUT3D_KEYS}, exact_energy=False)
+    got = run3(oracle3d_args(z, m), segments=m["segments"], spb=spb, kernel=kernel)
+    S = len(z["in_src_z"])
+    one_group = (kernel == 1 and spb >= S) or (kernel == 0 and spb in (0, S))
+    _check(got, {k: z["out_" + k] for k in OUT3D_KEYS}, exact_energy=one_group)
 
 
 def random_case(nz, ny, nx, nabc, nt, S, fs, seed, shared=False, R=9):
@@ -115,15 +121,22 @@ def test_3d_forward_matches_oracle(case):
     want = oracle.forward3d(**a, chunk=chunk_len(nt, 3), precision="f32")
     got = run3(a, segments=3)
     _check(got, want)
-    for spb in (1, 2):
-        _check(run3(a, segments=3, spb=spb), want, exact_energy=False)
+    for spb, kernel, zc in ((1, 1, 0), (2, 1, 3), (4, 1, 5), (1, 0, 0), (2, 0, 0)):
+        _check(run3(a, segments=3, spb=spb, kernel=kernel, zc=zc), want,
+               exact_energy=(spb >= S))
 
 
-def test_3d_medium_grid_matches_oracle():
-    print("[case] 3d oracle 40x36x90, 4 shots, 60 steps", flush=True)
+@pytest.mark.parametrize("zc", [0, 7, 8, 9])
+def test_3d_medium_grid_matches_oracle(zc):
+    """z-marching chunks of 7 and 9 planes put chunk boundaries at the mirrored
+    plane f-1, at f and at f+1 (f = 8)."""
+    print(f"[case] 3d oracle 40x36x90, 4 shots, 60 steps, z chunk {zc}", flush=True)
     a = random_case(40, 36, 90, 8, 60, 4, True, 11, R=40)
     want = oracle.forward3d(**a, chunk=60, precision="f32")
-    _check(run3(a), want)
+    _check(run3(a, spb=4, kernel=1, zc=zc), want)
+    _check(run3(a, spb=2, kernel=1, zc=zc), want, exact_energy=False)
+    if zc == 0:
+        _check(run3(a), want)
 
 
 def test_3d_linear_in_wavelet():
@@ -161,10 +174,10 @@ def test_3d_index_errors():
 GTOL = 1e-5
 
 
-def grad3(a, gs, segments=1, spb=0, want=("vp", "rho", "w")):
+def grad3(a, gs, segments=1, spb=0, want=("vp", "rho", "w"), kernel=0):
     """Gradient of sum(g . seismograms) w.r.t. vp, rho and src_v through the
     HIP adjoint; returns (outputs, {name: grad})."""
-    K.set_shots_per_block_3d(spb)
+    K.set_launch_choice_3d(K.LaunchChoice3D(spb, kernel, 0))
     try:
         vp = _t(a["vp"], dtype=torch.float32).requires_grad_("vp" in want)
         rho = _t(a["rho"], dtype=torch.float32).requires_grad_("rho" in want)
@@ -179,7 +192,7 @@ def grad3(a, gs, segments=1, spb=0, want=("vp", "rho", "w")):
         loss = sum((_t(g) * out[k]).sum() for g, k in zip(gs, REC) if g is not None)
         loss.backward()
     finally:
-        K.set_shots_per_block_3d(0)
+        K.set_launch_choice_3d(None)
     torch.cuda.synchronize()
     grads = {k: t.grad.cpu().numpy() for k, t in (("vp", vp), ("rho", rho), ("w", w))
              if k in want}
@@ -207,18 +220,21 @@ def test_3d_gradient_matches_reference(name):
 
 
 GRAD_CASES = [
-    # nz, ny, nx, nabc, nt, S, free surface, seed, shared wavelet, segments, shots per block
-    (14, 11, 17, 4, 30, 3, False, 41, False, 1, 0),
-    (14, 11, 17, 4, 30, 3, True, 42, True, 4, 0),
-    (12, 9, 61, 5, 26, 4, True, 43, False, 3, 2),    # NX = 71, two shot groups
-    (7, 7, 7, 1, 20, 2, False, 44, False, 2, 1),     # smallest absorbing layer, one shot per group
+    # nz, ny, nx, nabc, nt, S, free surface, seed, shared wavelet, segments, shots per block,
+    # step kernel
+    (14, 11, 17, 4, 30, 3, False, 41, False, 1, 0, 0),
+    (14, 11, 17, 4, 30, 3, True, 42, True, 4, 0, 0),
+    (12, 9, 61, 5, 26, 4, True, 43, False, 3, 2, 0),   # NX = 71, two shot groups
+    (7, 7, 7, 1, 20, 2, False, 44, False, 2, 1, 0),    # smallest absorbing layer, one shot per group
+    (14, 11, 17, 4, 30, 3, True, 45, False, 3, 4, 1),  # z-marching step
+    (12, 9, 61, 5, 26, 4, True, 46, True, 2, 2, 1),    # z-marching step, two shot groups
 ]
 
 
 @pytest.mark.parametrize("case", GRAD_CASES)
 def test_3d_gradient_matches_oracle(case):
     print(f"[case] 3d gradient oracle {case}", flush=True)
-    nz, ny, nx, nabc, nt, S, fs, seed, shared, segs, spb = case
+    nz, ny, nx, nabc, nt, S, fs, seed, shared, segs, spb, kernel = case
     a = random_case(nz, ny, nx, nabc, nt, S, fs, seed, shared)
     R = len(a["rcv_z"])
     # two receivers on one cell: their adjoint sources add in receiver order
@@ -228,7 +244,7 @@ def test_3d_gradient_matches_oracle(case):
     rng = np.random.default_rng(seed + 1000)
     gs = [rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(4)]
     gs[1] = None  # no adjoint source on vx
-    _, g = grad3(a, gs, segments=segs, spb=spb)
+    _, g = grad3(a, gs, segments=segs, spb=spb, kernel=kernel)
     want = oracle.gradient3d(**a, g_p=gs[0], g_vy=gs[2], g_vz=gs[3], chunk=chunk_len(nt, segs),
                              precision="f32")
     for key, i in (("vp", 0), ("rho", 1), ("w", 2)):
