@@ -260,11 +260,12 @@ def workload_3d(n_shots=8, n=200, nt=1000, nabc=20, dx=10.0, dt=8e-4, f0=10.0):
         rcv_z=np.full(rx.size, 2, np.int64))
 
 
-def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 900,
-               nt: int = 1000) -> dict:
+def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 1300,
+               nt: int = 1000, grad: bool = True) -> dict:
     """Times acoustic_forward_kernel_3d (all nt steps = one bench step) on
     ``workload_3d``; roofline of the time step at its algorithmic 32 B per
-    cell-update (p, vx, vy, vz each read and written once)."""
+    cell-update (p, vx, vy, vz each read and written once).  With ``grad``,
+    also one vp gradient of the same problem."""
     import torch
     from geobrain_amd import _native
     from geobrain_amd.physics import wave
@@ -298,6 +299,15 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
     step_us = sum(kernels[k]["avg_launch_us"] for k in step_kernels)
     b_step = 32.0 * cells
     achieved = b_step / (step_us * 1e-6) / 1e9
+    # measured HBM bytes per time step (rocprofv3 FETCH_SIZE / WRITE_SIZE) from
+    # the committed profile of this workload, when it has one
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_3d_{S}shot_{pad[0]}cube.json")
+    if os.path.exists(tpath) and pad[0] == pad[1] == pad[2]:
+        with open(tpath) as fh:
+            tdoc = json.load(fh)
+        if tdoc.get("padded") == pad and all(k in tdoc["kernels"] for k in step_kernels):
+            traffic = sum(tdoc["kernels"][k]["traffic"] for k in step_kernels)
     rec = {
         "metric": "Gcell-updates/s, 3D forward (seismograms + energy maps)",
         "value": round(cells * nt / wall / 1e9, 3), "unit": "Gcell-updates/s",
@@ -307,12 +317,39 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
                    "shots": S, "grid": [w["nz"], w["ny"], w["nx"]], "padded": pad, "nt": nt},
         "roofline": {"bound": "hbm", "kernel": "+".join(step_kernels), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_time_step_us": round(step_us, 2),
+                     "traffic": traffic, "avg_time_step_us": round(step_us, 2),
                      "bytes_per_time_step": b_step},
         "kernels": kernels,
         "check": {"p_max": float(out["p"].abs().max())},
         "cpu_baseline": None,
     }
+    if grad:
+        # gradient of a seismogram misfit w.r.t. vp: forward with checkpoints,
+        # recompute, adjoint, chain rule (one timed call after one warm-up)
+        d_obs = out["p"].detach().clone().mul_(0.9)
+        del out
+        vp = args["vp"].clone().requires_grad_(True)
+
+        def grad_step():
+            o = wave.acoustic_forward_kernel_3d(**dict(args, vp=vp))
+            ((o["p"] - d_obs) ** 2).sum().backward()
+
+        grad_step()
+        torch.cuda.synchronize()
+        _native.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        grad_step()
+        torch.cuda.synchronize()
+        gwall = time.perf_counter() - t0
+        gkt = _native.kernel_times()
+        _native.set_kernel_timing(False)
+        rec["gradient"] = {
+            "value": round(cells * nt / gwall / 1e9, 3), "unit": "Gcell-updates/s",
+            "seconds": round(gwall, 3), "grad_vp_max": float(vp.grad.abs().max()),
+            "kernels": {k: {"launches": n, "avg_launch_us": round(ms / n * 1e3, 2)}
+                        for k, (n, ms) in gkt.items()}}
+        del vp, d_obs
+        torch.cuda.empty_cache()
     if cpu:
         import oracle
         oracle.build()
@@ -427,7 +464,7 @@ def main():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=2500)
     ap.add_argument("--no-tune", action="store_true", help="library default launch choice")
     ap.add_argument("--no-eager", action="store_true", help="skip the per-shot PyTorch GPU baseline")
     ap.add_argument("--launch", type=str, default=None,
@@ -446,6 +483,7 @@ def main():
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward workload")
     ap.add_argument("--nt3d", type=int, default=1000, help="time steps of the 3D workload")
+    ap.add_argument("--no-grad3d", action="store_true", help="skip the 3D gradient timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the reference "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -472,7 +510,8 @@ def main():
         print(json.dumps(fwi_precision(n_iter=args.fwi_iters)), flush=True)
         return
     if args.workload == "fwd3d":
-        rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu, nt=args.nt3d)
+        rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu, nt=args.nt3d,
+                         grad=not args.no_grad3d)
         rec.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                     "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
         print(json.dumps(rec), flush=True)
@@ -612,7 +651,8 @@ def main():
                                            launch=blaunch)
     if not args.no_3d:
         torch.cuda.empty_cache()
-        rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d)
+        rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d,
+                                       grad=not args.no_grad3d)
     print(json.dumps(rec), flush=True)
 
 
