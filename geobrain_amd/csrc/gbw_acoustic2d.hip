@@ -1794,7 +1794,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     // D (and brackets) of step it for the tile slots (the others read a valid
     // group), issued before the next shot's prefetch so that waiting for them
     // does not wait for it; step it-1's are issued when that step starts
-    float4 dA[NS], gxA[RHO ? NS : 1], gzA[RHO ? NS : 1];
+    float4 dA[NS], dB[NS], gxA[RHO ? NS : 1], gzA[RHO ? NS : 1];
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
       const bool ok = fl[n] & F_OK;
@@ -1802,19 +1802,22 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
       lx[n] = ok ? rlx[n] : f4zero();
       lz[n] = ok ? rlz[n] : f4zero();
       dA[n] = reinterpret_cast<const float4*>(a.dbuf + so)[idx[n]];
+      if constexpr (!RHO) dB[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
       if constexpr (RHO) {
         gxA[n] = reinterpret_cast<const float4*>(a.gxbuf + so)[idx[n]];
         gzA[n] = reinterpret_cast<const float4*>(a.gzbuf + so)[idx[n]];
       }
       const int item = tid + n * NTH;
-      if (item < SR * W2) {
+      if (d.fs && item < SR * W2) {  // only the free-surface transposes read these
         const int r = item / W2, c = item % W2 + 1;
         sLP[r][c] = lp[n];
         sLZ[r][c] = lz[n];
       }
     }
     if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
-    __syncthreads();
+    // the q phase below writes only LDS arrays the previous shot last read
+    // before its final barrier; sLP / sLZ are read only with a free surface
+    if (d.fs) __syncthreads();
     // step it: q, with the transpose of vz[f-1] = vz[f] folded into lambda_vz
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
@@ -1860,7 +1863,8 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
             gzc[n] = gzA[n];
           }
         } else {
-          dcur[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
+          if constexpr (RHO) dcur[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
+          else dcur[n] = dB[n];
           if constexpr (RHO) {
             gxc[n] = reinterpret_cast<const float4*>(a.gxbuf1 + so)[idx[n]];
             gzc[n] = reinterpret_cast<const float4*>(a.gzbuf1 + so)[idx[n]];
@@ -2038,8 +2042,10 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
           lp[n] = lpo;
           lx[n] = lx4;
           if (k == 0) {
-            sLP[r][c] = lpo;  // read by the next step's transpose at row f+1
-            sLZ[r][c] = lz4;  // read by the next step's fold at row f
+            if (d.fs) {
+              sLP[r][c] = lpo;  // read by the next step's transpose at row f+1
+              sLZ[r][c] = lz4;  // read by the next step's fold at row f
+            }
             // transpose of vz[f-1] = vz[f] for the next step: row f-1 gives its
             // value to row f (added below, after the barrier) and becomes 0
             if (d.fs && gz == d.f - 1) lz4 = f4zero();
