@@ -134,17 +134,27 @@ struct Step3Args {
   float* pstore;           // recompute: [shot][plane] p after this step's update, or null
 };
 
-// p on z in [f+1, NZ-3], y in [2, NY-3], x in [2, NX-3]; then the source
+// element i of a float4 array at a 32-bit byte offset (i < 2^28): the load
+// takes a uniform base and a 32-bit lane offset
+__device__ __forceinline__ float4 ld4u(const float4* base, unsigned i) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 4));
+}
+__device__ __forceinline__ void st4u(float4* base, unsigned i, const float4& v) {
+  *reinterpret_cast<float4*>(reinterpret_cast<char*>(base) + (i << 4)) = v;
+}
+
+// p on z in [f+1, NZ-3], y in [2, NY-3], x in [2, NX-3]; then the source.
+// In-plane offsets are 32-bit (a plane of one field of one shot is < 2^28
+// float4), so loads use a uniform base and a 32-bit lane offset.
 __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
   const Dims3 d = a.d;
   const long long gq = xcd_block3() * T3 + threadIdx.x;  // float4 group of the plane
   if (gq >= d.groups) return;
-  const int g = (int)(gq % d.GX);
-  const long long zy = gq / d.GX;
-  const int y = (int)(zy % d.NY), z = (int)(zy / d.NY);
+  const unsigned q4 = (unsigned)gq;
+  const int g = (int)(q4 % (unsigned)d.GX);
+  const unsigned zy = q4 / (unsigned)d.GX;
+  const int y = (int)(zy % (unsigned)d.NY), z = (int)(zy / (unsigned)d.NY);
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
-  const long long row = (long long)z * d.NY + y;
-  const long long q4 = row * d.GX + g;  // float4 index
   unsigned mask = 0;
   if (z >= d.f + 1 && z <= d.NZ - 3 && y >= 2 && y <= d.NY - 3)
 #pragma unroll
@@ -153,29 +163,31 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
       if (x >= 2 && x <= d.NX - 3) mask |= 1u << e;
     }
   const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
-  const long long p4 = d.plane >> 2;
-  const float4 omk1 = C[q4], a1 = C[p4 + q4];
-  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
-  const long long gl = row * d.GX + max(g - 1, 0), gr = row * d.GX + min(g + 1, d.GX - 1);
+  const float4 omk1 = ld4u(C, q4), a1 = ld4u(C + (d.plane >> 2), q4);
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
+  const unsigned gl = g > 0 ? q4 - 1u : q4, gr = g + 1 < d.GX ? q4 + 1u : q4;
+  const bool store = a.dstore != nullptr;
   for (int s = s_begin; s < s_end; ++s) {
-    float4* P = reinterpret_cast<float4*>(a.state + (long long)s * d.plane);
-    const float4* VX = reinterpret_cast<const float4*>(a.state + a.fstride + (long long)s * d.plane);
-    const float4* VY = reinterpret_cast<const float4*>(a.state + 2 * a.fstride + (long long)s * d.plane);
-    const float4* VZ = reinterpret_cast<const float4*>(a.state + 3 * a.fstride + (long long)s * d.plane);
+    const long long so = (long long)s * d.plane;
+    float4* __restrict__ P = reinterpret_cast<float4*>(a.state + so);
+    const float4* __restrict__ VX = reinterpret_cast<const float4*>(a.state + a.fstride + so);
+    const float4* __restrict__ VY = reinterpret_cast<const float4*>(a.state + 2 * a.fstride + so);
+    const float4* __restrict__ VZ = reinterpret_cast<const float4*>(a.state + 3 * a.fstride + so);
     const int sc = a.src_cell[s];
-    const bool src_here = (long long)(sc >> 2) == q4;
-    const bool store = a.dstore != nullptr;
+    const bool src_here = (unsigned)(sc >> 2) == q4;
     if (!mask && !src_here && !store) continue;
-    float4 p = P[q4];
+    float4 p = ld4u(P, q4);
     float4 dsum = make_float4(0.f, 0.f, 0.f, 0.f);
     if (mask) {
-      const float4 xa = VX[gl], xb = VX[q4], xc = VX[gr];
+      const float4 xa = ld4u(VX, gl), xb = ld4u(VX, q4), xc = ld4u(VX, gr);
       const float vxs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
-      const float4 ym2 = VY[q4 - 2 * sy4], ym1 = VY[q4 - sy4], y00 = VY[q4], yp1 = VY[q4 + sy4];
-      const float4 zm2 = VZ[q4 - 2 * sz4], zm1 = VZ[q4 - sz4], z00 = VZ[q4], zp1 = VZ[q4 + sz4];
+      const float4 ym2 = ld4u(VY, q4 - 2u * sy4), ym1 = ld4u(VY, q4 - sy4), y00 = ld4u(VY, q4),
+                   yp1 = ld4u(VY, q4 + sy4);
+      const float4 zm2 = ld4u(VZ, q4 - 2u * sz4), zm1 = ld4u(VZ, q4 - sz4), z00 = ld4u(VZ, q4),
+                   zp1 = ld4u(VZ, q4 + sz4);
+      float4 pn;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (!((mask >> e) & 1u)) continue;
         const float dvx = d4(vxs[4 + e], vxs[3 + e], vxs[5 + e], vxs[2 + e]);
         const float dvy = d4(g3(y00, e), g3(ym1, e), g3(yp1, e), g3(ym2, e));
         const float dvz = d4(g3(z00, e), g3(zm1, e), g3(zp1, e), g3(zm2, e));
@@ -183,17 +195,27 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
         sum = sum + dvz;
         s3(dsum, e, sum);
         const float u = g3(omk1, e) * g3(p, e);
-        s3(p, e, u - g3(a1, e) * sum);
+        s3(pn, e, u - g3(a1, e) * sum);
+      }
+      if (mask == 15u) {
+        p = pn;
+      } else {  // cells outside the region keep their value (and a zero bracket)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool in = (mask >> e) & 1u;
+          s3(p, e, in ? g3(pn, e) : g3(p, e));
+          s3(dsum, e, in ? g3(dsum, e) : 0.f);
+        }
       }
     }
     if (src_here) {
       const int e = sc & 3;
       s3(p, e, g3(p, e) + a.dt * a.wav[(long long)s * a.wav_stride + a.it]);
     }
-    P[q4] = p;
+    st4u(P, q4, p);
     if (store) {
-      reinterpret_cast<float4*>(a.dstore + (long long)s * d.plane)[q4] = dsum;
-      if (a.pstore) reinterpret_cast<float4*>(a.pstore + (long long)s * d.plane)[q4] = p;
+      st4u(reinterpret_cast<float4*>(a.dstore + so), q4, dsum);
+      if (a.pstore) st4u(reinterpret_cast<float4*>(a.pstore + so), q4, p);
     }
   }
 }
@@ -203,12 +225,11 @@ __global__ __launch_bounds__(T3) void v3_step_kernel(Step3Args a) {
   const Dims3 d = a.d;
   const long long gq = xcd_block3() * T3 + threadIdx.x;
   if (gq >= d.groups) return;
-  const int g = (int)(gq % d.GX);
-  const long long zy = gq / d.GX;
-  const int y = (int)(zy % d.NY), z = (int)(zy / d.NY);
+  const unsigned q4 = (unsigned)gq;
+  const int g = (int)(q4 % (unsigned)d.GX);
+  const unsigned zy = q4 / (unsigned)d.GX;
+  const int y = (int)(zy % (unsigned)d.NY), z = (int)(zy / (unsigned)d.NY);
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
-  const long long row = (long long)z * d.NY + y;
-  const long long q4 = row * d.GX + g;
   const bool zx = z >= d.f && z <= d.NZ - 2, zz = z >= d.f && z <= d.NZ - 3;
   unsigned mx = 0, my = 0, mz = 0, me = 0;
 #pragma unroll
@@ -227,68 +248,61 @@ __global__ __launch_bounds__(T3) void v3_step_kernel(Step3Args a) {
   if (!(mx | my | mz) && !energy && !mirror_p) return;
   const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
   const long long p4 = d.plane >> 2;
-  const float4 omk2 = C[2 * p4 + q4], omk3 = C[3 * p4 + q4], omk4 = C[4 * p4 + q4];
-  const float4 a2 = C[5 * p4 + q4];
-  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
-  const long long gl = row * d.GX + max(g - 1, 0), gr = row * d.GX + min(g + 1, d.GX - 1);
+  const float4 omk2 = ld4u(C + 2 * p4, q4), omk3 = ld4u(C + 3 * p4, q4);
+  const float4 omk4 = ld4u(C + 4 * p4, q4), a2 = ld4u(C + 5 * p4, q4);
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
+  const unsigned gl = g > 0 ? q4 - 1u : q4, gr = g + 1 < d.GX ? q4 + 1u : q4;
   float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), ex = ep, ey = ep, ez = ep;
   auto add_sq = [](float4& acc, const float4& v) {
     acc = make_float4(acc.x + v.x * v.x, acc.y + v.y * v.y, acc.z + v.z * v.z, acc.w + v.w * v.w);
   };
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    float4* P = reinterpret_cast<float4*>(a.state + so);
-    float4* VX = reinterpret_cast<float4*>(a.state + a.fstride + so);
-    float4* VY = reinterpret_cast<float4*>(a.state + 2 * a.fstride + so);
-    float4* VZ = reinterpret_cast<float4*>(a.state + 3 * a.fstride + so);
+    float4* __restrict__ P = reinterpret_cast<float4*>(a.state + so);
+    float4* __restrict__ VX = reinterpret_cast<float4*>(a.state + a.fstride + so);
+    float4* __restrict__ VY = reinterpret_cast<float4*>(a.state + 2 * a.fstride + so);
+    float4* __restrict__ VZ = reinterpret_cast<float4*>(a.state + 3 * a.fstride + so);
     if (mirror_p) {  // no velocity of row f-1 is updated; nothing else reads this row here
-      const float4 v = P[q4 + 2 * sz4];
-      P[q4] = make_float4(-v.x, -v.y, -v.z, -v.w);
+      const float4 v = ld4u(P, q4 + 2u * sz4);
+      st4u(P, q4, make_float4(-v.x, -v.y, -v.z, -v.w));
       continue;
     }
-    const float4 pb = P[q4];
-    float4 vx = VX[q4], vy = VY[q4], vz = VZ[q4];
+    const float4 pb = ld4u(P, q4);
+    float4 vx = ld4u(VX, q4), vy = ld4u(VY, q4), vz = ld4u(VZ, q4);
     if (mx) {
-      const float4 pa = P[gl], pc = P[gr];
+      const float4 pa = ld4u(P, gl), pc = ld4u(P, gr);
       const float ps[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w, pc.x, pc.y, pc.z, pc.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (!((mx >> e) & 1u)) continue;
         const float u = g3(omk2, e) * g3(vx, e);
-        s3(vx, e, u - g3(a2, e) * d4(ps[5 + e], ps[4 + e], ps[6 + e], ps[3 + e]));
+        const float v = u - g3(a2, e) * d4(ps[5 + e], ps[4 + e], ps[6 + e], ps[3 + e]);
+        s3(vx, e, ((mx >> e) & 1u) ? v : g3(vx, e));
       }
-      VX[q4] = vx;
+      st4u(VX, q4, vx);
     }
     if (my) {
-      const float4 pu = P[q4 - sy4], pd1 = P[q4 + sy4], pd2 = P[q4 + 2 * sy4];
+      const float4 pu = ld4u(P, q4 - sy4), pd1 = ld4u(P, q4 + sy4), pd2 = ld4u(P, q4 + 2u * sy4);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (!((my >> e) & 1u)) continue;
         const float u = g3(omk3, e) * g3(vy, e);
-        s3(vy, e, u - g3(a2, e) * d4(g3(pd1, e), g3(pb, e), g3(pd2, e), g3(pu, e)));
+        const float v = u - g3(a2, e) * d4(g3(pd1, e), g3(pb, e), g3(pd2, e), g3(pu, e));
+        s3(vy, e, ((my >> e) & 1u) ? v : g3(vy, e));
       }
-      VY[q4] = vy;
+      st4u(VY, q4, vy);
     }
     if (mz) {
       // at z = f the plane above is the mirrored row f-1: read it as -p[f+1]
-      const float4 pd1 = P[q4 + sz4], pd2 = P[q4 + 2 * sz4];
-      float4 pu;
-      if (vz_up) {
-        pu = make_float4(-pd1.x, -pd1.y, -pd1.z, -pd1.w);
-      } else {
-        pu = P[q4 - sz4];
-      }
+      const float4 pd1 = ld4u(P, q4 + sz4), pd2 = ld4u(P, q4 + 2u * sz4);
+      const float4 pu = vz_up ? make_float4(-pd1.x, -pd1.y, -pd1.z, -pd1.w) : ld4u(P, q4 - sz4);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        if (!((mz >> e) & 1u)) continue;
         const float u = g3(omk4, e) * g3(vz, e);
-        s3(vz, e, u - g3(a2, e) * d4(g3(pd1, e), g3(pb, e), g3(pd2, e), g3(pu, e)));
+        const float v = u - g3(a2, e) * d4(g3(pd1, e), g3(pb, e), g3(pd2, e), g3(pu, e));
+        s3(vz, e, ((mz >> e) & 1u) ? v : g3(vz, e));
       }
-      VZ[q4] = vz;
-      if (vz_up) VZ[q4 - sz4] = vz;
-    } else if (vz_up) {
-      VZ[q4 - sz4] = vz;  // cells of row f outside the vz region: copy as is
+      st4u(VZ, q4, vz);
     }
+    if (vz_up) st4u(VZ, q4 - sz4, vz);  // the whole of row f, updated or not
     if (energy) {
       add_sq(ep, pb);
       add_sq(ex, vx);
