@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gbw_acoustic3d.h"
@@ -394,6 +395,21 @@ struct Adj3Args {
   float* gws;              // [S] source terms of this step for a shared wavelet, or null
 };
 
+// float4 array addressed by a 32-bit element index (byte offset i * 16 < 2^32):
+// loads and stores take a uniform base and a 32-bit lane offset
+template <typename T>
+struct A4 {
+  T* p;
+  __device__ __forceinline__ T& operator[](unsigned i) const {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return *reinterpret_cast<T*>(reinterpret_cast<B*>(p) + (i << 4));
+  }
+};
+
+__device__ __forceinline__ float4 ld4(const A4<const float4>& a, unsigned i, bool ok) {
+  return ok ? a[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 struct Cell3 {
   int g, y, z;
   long long q4;
@@ -419,7 +435,7 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
   Cell3 c;
   if (!cell3(d, c)) return;
   const int g = c.g, y = c.y, z = c.z;
-  const long long q4 = c.q4;
+  const unsigned q4 = (unsigned)c.q4;
   const bool zx = z >= d.f && z <= d.NZ - 2, zz = z >= d.f && z <= d.NZ - 3;
   unsigned mx = 0, my = 0, mz = 0;
 #pragma unroll
@@ -433,19 +449,18 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
   if (!(mx | my | mz) && !fold) return;
   const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
   const long long p4 = d.plane >> 2;
-  const float4 omk2 = C[2 * p4 + q4], omk3 = C[3 * p4 + q4], omk4 = C[4 * p4 + q4];
-  const float4 a2 = C[5 * p4 + q4];
-  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
-  const long long row = (long long)z * d.NY + y;
-  const long long gl = row * d.GX + max(g - 1, 0), gr = row * d.GX + min(g + 1, d.GX - 1);
+  const float4 omk2 = ld4u(C + 2 * p4, q4), omk3 = ld4u(C + 3 * p4, q4);
+  const float4 omk4 = ld4u(C + 4 * p4, q4), a2 = ld4u(C + 5 * p4, q4);
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
+  const unsigned gl = g > 0 ? q4 - 1u : q4, gr = g + 1 < d.GX ? q4 + 1u : q4;
   const bool rho = a.gA2 != nullptr;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   float4 acc2 = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    float4* LX = reinterpret_cast<float4*>(a.lam + a.fstride + so);
-    float4* LY = reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so);
-    float4* LZ = reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so);
+    const A4<float4> LX{reinterpret_cast<float4*>(a.lam + a.fstride + so)};
+    const A4<float4> LY{reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so)};
+    const A4<float4> LZ{reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so)};
     float4 lx = LX[q4], ly = LY[q4], lz = LZ[q4];
     if (fold) {
       const float4 up = LZ[q4 - sz4];
@@ -456,7 +471,7 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
     float pxs[12];
     float4 pb = qx, pyu = qx, pyd1 = qx, pyd2 = qx, pzu = qx, pzd1 = qx, pzd2 = qx;
     if (rho) {  // brackets of the v updates, from p after the step
-      const float4* P1 = reinterpret_cast<const float4*>(a.pstep + so);
+      const A4<const float4> P1{reinterpret_cast<const float4*>(a.pstep + so)};
       pb = P1[q4];
       if (mx) {
         const float4 pa = P1[gl], pc = P1[gr];
@@ -499,14 +514,16 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
       }
       s3(acc2, e, acc);
     }
-    float4* Q = reinterpret_cast<float4*>(a.q + so);
-    if (mx) { Q[q4] = qx; LX[q4] = lx; }
-    if (my) { Q[a.fstride / 4 + q4] = qy; LY[q4] = ly; }
+    const A4<float4> QX{reinterpret_cast<float4*>(a.q + so)};
+    const A4<float4> QY{reinterpret_cast<float4*>(a.q + a.fstride + so)};
+    const A4<float4> QZ{reinterpret_cast<float4*>(a.q + 2 * a.fstride + so)};
+    if (mx) { QX[q4] = qx; LX[q4] = lx; }
+    if (my) { QY[q4] = qy; LY[q4] = ly; }
     if (mz || fold) LZ[q4] = lz;
-    if (mz) Q[a.fstride / 2 + q4] = qz;
+    if (mz) QZ[q4] = qz;
   }
   if (rho && (mx | my | mz)) {
-    float4* G = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
+    const A4<float4> G{reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane)};
     const float4 o = G[q4];
     G[q4] = make_float4(o.x + acc2.x, o.y + acc2.y, o.z + acc2.z, o.w + acc2.w);
   }
@@ -514,10 +531,11 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
 
 // -D^T q at the cell group (z, y, g): c1 (q[i] - q[i-1]) + c2 (q[i+1] - q[i-2]) per axis,
 // added z, then y, then x; q outside the grid is zero
-__device__ __forceinline__ float4 gather_q(const Dims3& d, const float4* QX, const float4* QY,
-                                           const float4* QZ, int z, int y, int g) {
-  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
-  const long long q4 = ((long long)z * d.NY + y) * d.GX + g;
+__device__ __forceinline__ float4 gather_q(const Dims3& d, const A4<const float4>& QX,
+                                           const A4<const float4>& QY,
+                                           const A4<const float4>& QZ, int z, int y, int g) {
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
+  const unsigned q4 = ((unsigned)z * (unsigned)d.NY + (unsigned)y) * (unsigned)d.GX + (unsigned)g;
   const float4 z0 = QZ[q4], zm1 = ld4(QZ, q4 - sz4, z >= 1), zm2 = ld4(QZ, q4 - 2 * sz4, z >= 2);
   const float4 zp1 = ld4(QZ, q4 + sz4, z + 1 < d.NZ);
   const float4 y0 = QY[q4], ym1 = ld4(QY, q4 - sy4, y >= 1), ym2 = ld4(QY, q4 - 2 * sy4, y >= 2);
@@ -541,7 +559,7 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
   Cell3 c;
   if (!cell3(d, c)) return;
   const int g = c.g, y = c.y, z = c.z;
-  const long long q4 = c.q4;
+  const unsigned q4 = (unsigned)c.q4;
   unsigned mp = 0;
   if (z >= d.f + 1 && z <= d.NZ - 3 && y >= 2 && y <= d.NY - 3)
 #pragma unroll
@@ -553,14 +571,14 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
   const bool mirrored = d.fs && z == d.f - 1;  // zeroed by the fold at f+1
   const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
   const long long p4 = d.plane >> 2;
-  const float4 omk1 = C[q4], a1 = C[p4 + q4];
-  const long long sz4 = (long long)d.NY * d.GX;
+  const float4 omk1 = ld4u(C, q4), a1 = ld4u(C + p4, q4);
+  const unsigned sz4 = (unsigned)d.NY * (unsigned)d.GX;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
     const int sc = a.src_cell[s];
-    const bool src_here = (long long)(sc >> 2) == q4;
+    const bool src_here = (unsigned)(sc >> 2) == q4;
     if (mirrored) {  // the source term on the mirrored plane is overwritten: no gradient
       if (src_here) {
         if (a.gws) a.gws[s] = 0.f;
@@ -568,10 +586,10 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
       }
       continue;
     }
-    float4* LP = reinterpret_cast<float4*>(a.lam + so);
-    const float4* QX = reinterpret_cast<const float4*>(a.q + so);
-    const float4* QY = reinterpret_cast<const float4*>(a.q + a.fstride + so);
-    const float4* QZ = reinterpret_cast<const float4*>(a.q + 2 * a.fstride + so);
+    const A4<float4> LP{reinterpret_cast<float4*>(a.lam + so)};
+    const A4<const float4> QX{reinterpret_cast<const float4*>(a.q + so)};
+    const A4<const float4> QY{reinterpret_cast<const float4*>(a.q + a.fstride + so)};
+    const A4<const float4> QZ{reinterpret_cast<const float4*>(a.q + 2 * a.fstride + so)};
     float4 lp = LP[q4];
     const float4 gq = gather_q(d, QX, QY, QZ, z, y, g);
     lp = make_float4(lp.x + gq.x, lp.y + gq.y, lp.z + gq.z, lp.w + gq.w);
@@ -588,7 +606,7 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
       if (a.gw) a.gw[(long long)s * a.nt + a.it] = v;
     }
     if (mp) {
-      const float4 D = reinterpret_cast<const float4*>(a.dstep + so)[q4];
+      const float4 D = ld4u(reinterpret_cast<const float4*>(a.dstep + so), q4);
       float4 rr = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -598,12 +616,12 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
         s3(acc1, e, g3(acc1, e) - l * g3(D, e));
         s3(lp, e, g3(omk1, e) * l);
       }
-      reinterpret_cast<float4*>(a.r + so)[q4] = rr;
+      st4u(reinterpret_cast<float4*>(a.r + so), q4, rr);
     }
     LP[q4] = lp;
   }
   if (mp && a.gA1) {
-    float4* G = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
+    const A4<float4> G{reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane)};
     const float4 o = G[q4];
     G[q4] = make_float4(o.x + acc1.x, o.y + acc1.y, o.z + acc1.z, o.w + acc1.w);
   }
@@ -615,17 +633,17 @@ __global__ __launch_bounds__(T3) void adj3_v_kernel(Adj3Args a) {
   Cell3 c;
   if (!cell3(d, c)) return;
   const int g = c.g, y = c.y, z = c.z;
-  const long long q4 = c.q4;
+  const unsigned q4 = (unsigned)c.q4;
   // r is zero outside z in [f+1, NZ-3], y in [2, NY-3]: skip cells no term reaches
   if (z < d.f - 1 || z > d.NZ - 2 || y > d.NY - 2) return;
-  const long long sy4 = d.GX, sz4 = (long long)d.NY * d.GX;
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const float4* Rr = reinterpret_cast<const float4*>(a.r + so);
-    float4* LX = reinterpret_cast<float4*>(a.lam + a.fstride + so);
-    float4* LY = reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so);
-    float4* LZ = reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so);
+    const A4<const float4> Rr{reinterpret_cast<const float4*>(a.r + so)};
+    const A4<float4> LX{reinterpret_cast<float4*>(a.lam + a.fstride + so)};
+    const A4<float4> LY{reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so)};
+    const A4<float4> LZ{reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so)};
     const float4 r0 = Rr[q4];
     const float4 xa = ld4(Rr, q4 - 1, g >= 1), xc = ld4(Rr, q4 + 1, g + 1 < d.GX);
     const float rs[12] = {xa.x, xa.y, xa.z, xa.w, r0.x, r0.y, r0.z, r0.w, xc.x, xc.y, xc.z, xc.w};
@@ -647,7 +665,7 @@ __global__ __launch_bounds__(T3) void adj3_v_kernel(Adj3Args a) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused z-marching step (the default).  A workgroup owns a 64 x 16 (x, y)
+// Fused z-marching step (opt-in, step_kernel 1).  A workgroup owns a 64 x 16 (x, y)
 // tile, a range of z planes and a group of SPB shots, and marches down z.  At
 // plane zp it computes p' on the tile and on the halo cells the velocity
 // stencils read (one column left, two right, one row above, two below) into
