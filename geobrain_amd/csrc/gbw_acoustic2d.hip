@@ -1733,11 +1733,11 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
         !(d.fs && gz >= d.f - 1 && gz <= d.f + 1))
       f |= F_AINT;
     fl[n] = f;
-    c_omk1[n] = p1 ? omk1[idx[n]] : f4zero();
-    c_a1[n] = p1 ? a1[idx[n]] : f4zero();
-    c_omk2[n] = v1 ? omk2[idx[n]] : f4zero();
-    c_omk3[n] = v1 ? omk3[idx[n]] : f4zero();
-    c_a2[n] = a2[idx[n]];
+    c_omk1[n] = p1 ? ld4u(omk1, idx[n]) : f4zero();
+    c_a1[n] = p1 ? ld4u(a1, idx[n]) : f4zero();
+    c_omk2[n] = v1 ? ld4u(omk2, idx[n]) : f4zero();
+    c_omk3[n] = v1 ? ld4u(omk3, idx[n]) : f4zero();
+    c_a2[n] = ld4u(a2, idx[n]);
   }
 
   float4 rlp[NS], rlx[NS], rlz[NS];
@@ -1748,9 +1748,9 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     const float4* __restrict__ LZ = reinterpret_cast<const float4*>(a.LZ0 + so);
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
-      rlp[n] = LP[idx[n]];
-      rlx[n] = LX[idx[n]];
-      rlz[n] = LZ[idx[n]];
+      rlp[n] = ld4u(LP, idx[n]);
+      rlx[n] = ld4u(LX, idx[n]);
+      rlz[n] = ld4u(LZ, idx[n]);
     }
   };
   __syncthreads();  // sSrc
@@ -1801,11 +1801,11 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
       lp[n] = ok ? rlp[n] : f4zero();
       lx[n] = ok ? rlx[n] : f4zero();
       lz[n] = ok ? rlz[n] : f4zero();
-      dA[n] = reinterpret_cast<const float4*>(a.dbuf + so)[idx[n]];
-      if constexpr (!RHO) dB[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
+      dA[n] = ld4u(reinterpret_cast<const float4*>(a.dbuf + so), idx[n]);
+      if constexpr (!RHO) dB[n] = ld4u(reinterpret_cast<const float4*>(a.dbuf1 + so), idx[n]);
       if constexpr (RHO) {
-        gxA[n] = reinterpret_cast<const float4*>(a.gxbuf + so)[idx[n]];
-        gzA[n] = reinterpret_cast<const float4*>(a.gzbuf + so)[idx[n]];
+        gxA[n] = ld4u(reinterpret_cast<const float4*>(a.gxbuf + so), idx[n]);
+        gzA[n] = ld4u(reinterpret_cast<const float4*>(a.gzbuf + so), idx[n]);
       }
       const int item = tid + n * NTH;
       if (d.fs && item < SR * W2) {  // only the free-surface transposes read these
@@ -1832,7 +1832,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
             if (r >= 1) {
               up = sLZ[r - 1][c];
             } else {  // row f-1 lies above the staged rows
-              up = reinterpret_cast<const float4*>(a.LZ0 + so)[idx[n] - (d.PX >> 2)];
+              up = ld4u(reinterpret_cast<const float4*>(a.LZ0 + so), idx[n] - (unsigned)(d.PX >> 2));
             }
             v = make_float4(v.x + up.x, v.y + up.y, v.z + up.z, v.w + up.w);
           } else {
@@ -1863,11 +1863,11 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
             gzc[n] = gzA[n];
           }
         } else {
-          if constexpr (RHO) dcur[n] = reinterpret_cast<const float4*>(a.dbuf1 + so)[idx[n]];
+          if constexpr (RHO) dcur[n] = ld4u(reinterpret_cast<const float4*>(a.dbuf1 + so), idx[n]);
           else dcur[n] = dB[n];
           if constexpr (RHO) {
-            gxc[n] = reinterpret_cast<const float4*>(a.gxbuf1 + so)[idx[n]];
-            gzc[n] = reinterpret_cast<const float4*>(a.gzbuf1 + so)[idx[n]];
+            gxc[n] = ld4u(reinterpret_cast<const float4*>(a.gxbuf1 + so), idx[n]);
+            gzc[n] = ld4u(reinterpret_cast<const float4*>(a.gzbuf1 + so), idx[n]);
           }
         }
       }
@@ -2055,9 +2055,9 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
             sQX[r][c] = qx;
             sQZ[r][c] = qz;
           } else if (own) {
-            reinterpret_cast<float4*>(a.LP1 + so)[ti] = lpo;
-            reinterpret_cast<float4*>(a.LX1 + so)[ti] = lx4;
-            reinterpret_cast<float4*>(a.LZ1 + so)[ti] = lz4;
+            st4u(reinterpret_cast<float4*>(a.LP1 + so), ti, lpo);
+            st4u(reinterpret_cast<float4*>(a.LX1 + so), ti, lx4);
+            st4u(reinterpret_cast<float4*>(a.LZ1 + so), ti, lz4);
           }
         }
       }
@@ -2088,16 +2088,16 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
       const int item = tid + n * NTH;
       const int tr = item / W2 - 6, h = item % W2 - 2;
-      const float4 v = g1[idx[n]], a0 = sG[0][tr][h], a1v = sG[1][tr][h];
+      const float4 v = ld4u(g1, idx[n]), a0 = sG[0][tr][h], a1v = sG[1][tr][h];
       float4 w = make_float4(v.x + a0.x, v.y + a0.y, v.z + a0.z, v.w + a0.w);
       w = make_float4(w.x + a1v.x, w.y + a1v.y, w.z + a1v.z, w.w + a1v.w);
-      g1[idx[n]] = w;
+      st4u(g1, idx[n], w);
       if constexpr (RHO) {
         float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
-        const float4 u = g2[idx[n]], b0 = sG2[0][tr][h], b1 = sG2[1][tr][h];
+        const float4 u = ld4u(g2, idx[n]), b0 = sG2[0][tr][h], b1 = sG2[1][tr][h];
         float4 t = make_float4(u.x + b0.x, u.y + b0.y, u.z + b0.z, u.w + b0.w);
         t = make_float4(t.x + b1.x, t.y + b1.y, t.z + b1.z, t.w + b1.w);
-        g2[idx[n]] = t;
+        st4u(g2, idx[n], t);
       }
     }
   }

@@ -135,15 +135,6 @@ struct Step3Args {
   float* pstore;           // recompute: [shot][plane] p after this step's update, or null
 };
 
-// element i of a float4 array at a 32-bit byte offset (i < 2^28): the load
-// takes a uniform base and a 32-bit lane offset
-__device__ __forceinline__ float4 ld4u(const float4* base, unsigned i) {
-  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 4));
-}
-__device__ __forceinline__ void st4u(float4* base, unsigned i, const float4& v) {
-  *reinterpret_cast<float4*>(reinterpret_cast<char*>(base) + (i << 4)) = v;
-}
-
 // p on z in [f+1, NZ-3], y in [2, NY-3], x in [2, NX-3]; then the source.
 // In-plane offsets are 32-bit (a plane of one field of one shot is < 2^28
 // float4), so loads use a uniform base and a 32-bit lane offset.

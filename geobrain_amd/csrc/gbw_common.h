@@ -1,6 +1,7 @@
-// Host-side utilities shared by the library's source files: the per-thread
-// error text behind gbw_last_error(), HIP status checks, and the optional
-// per-kernel event timing behind gbw_set_kernel_timing() / gbw_kernel_times().
+// Utilities shared by the library's source files: the per-thread error text
+// behind gbw_last_error(), HIP status checks, the optional per-kernel event
+// timing behind gbw_set_kernel_timing() / gbw_kernel_times(), and float4
+// accesses at 32-bit byte offsets.
 #ifndef GBW_COMMON_H
 #define GBW_COMMON_H
 
@@ -106,6 +107,15 @@ inline int flush_timing(hipStream_t st) {
   }
   t_pending.clear();
   return 0;
+}
+
+// element i of a float4 array at a 32-bit byte offset (i < 2^28): the access
+// takes a uniform base and a 32-bit lane offset instead of a 64-bit address
+__device__ __forceinline__ float4 ld4u(const float4* base, unsigned i) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (i << 4));
+}
+__device__ __forceinline__ void st4u(float4* base, unsigned i, const float4& v) {
+  *reinterpret_cast<float4*>(reinterpret_cast<char*>(base) + (i << 4)) = v;
 }
 
 }  // namespace gbw
