@@ -1242,6 +1242,9 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
                          hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_rcell), rcell.data(), rcell.size() * sizeof(int),
                          hipMemcpyHostToDevice, st));
+  // pageable copies are staged before hipMemcpyAsync returns, but keep the
+  // host vectors alive until the copies are done regardless
+  HIP_TRY(hipStreamSynchronize(st));
   float* coef = at3<float>(workspace, pl.off_coef);
   coef3_kernel<<<(unsigned)((d.plane + 255) / 256), 256, 0, st>>>(d, in->vp, in->rho, in->damp,
                                                                     prob->dt, prob->dz, coef);
@@ -1296,8 +1299,6 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
       LAUNCH_CHECK();
     }
   }
-  // the host vectors above must outlive their copies
-  HIP_TRY(hipStreamSynchronize(st));
   return flush_timing(st);
 }
 
@@ -1341,6 +1342,7 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
                          ustart.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_urcv), urcv.data(), urcv.size() * sizeof(int),
                          hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host vectors go out of scope
   float* coef = at3<float>(workspace, pl.off_coef);
   coef3_kernel<<<(unsigned)((d.plane + 255) / 256), 256, 0, st>>>(d, in->vp, in->rho, in->damp,
                                                                     prob->dt, prob->dz, coef);
@@ -1437,7 +1439,6 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
         d, pl.groups, gA1, gA2, in->vp, in->rho, prob->dt, prob->dz, adj->grad_vp, adj->grad_rho);
     LAUNCH_CHECK();
   }
-  HIP_TRY(hipStreamSynchronize(st));
   return flush_timing(st);
 }
 
