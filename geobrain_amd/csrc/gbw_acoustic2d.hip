@@ -2287,7 +2287,8 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   d.f = d.fs ? p->nabc : 1;
   d.plane = (long long)d.NZ * d.PX;
   if (d.NZ < 6 || d.NX < 6) return fail(GBW_ERR_INVALID, "padded grid must be at least 6 x 6");
-  if (d.plane >= (1LL << 31)) return fail(GBW_ERR_INVALID, "padded plane exceeds 2^31 cells");
+  // in-plane float4 accesses use 32-bit byte offsets
+  if (d.plane >= (1LL << 30)) return fail(GBW_ERR_INVALID, "padded plane exceeds 2^30 cells");
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
   pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
   if (pass >= 1 && pl.k == 0) return fail(GBW_ERR_INVALID, "the gradient needs checkpoint_every > 0");

@@ -1073,7 +1073,8 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   d.plane = (long long)d.NZ * d.NY * d.PX;
   d.groups = d.plane / 4;
   if (d.NZ < 6 || d.NY < 6 || d.NX < 6) return fail(GBW_ERR_INVALID, "padded grid must be at least 6 per axis");
-  if (d.plane >= (1LL << 31)) return fail(GBW_ERR_INVALID, "padded volume exceeds 2^31 cells");
+  // in-volume float4 accesses use 32-bit byte offsets
+  if (d.plane >= (1LL << 30)) return fail(GBW_ERR_INVALID, "padded volume exceeds 2^30 cells");
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
   if (p->step_kernel != 0 && p->step_kernel != 1)
     return fail(GBW_ERR_INVALID, "step_kernel must be 0 or 1");

@@ -61,6 +61,8 @@ def test_workspace_queries():
     assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(nx=0)), 0) == 0
     assert "Grid dimensions must be positive" in _native.last_error()
     assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(tile_rows=5)), 0) == 0
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(nx=40000, nz=40000)), 0) == 0
+    assert "exceeds 2^30 cells" in _native.last_error()
     assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(_problem(checkpoint_every=0)), 1) == 0
 
 
@@ -246,7 +248,10 @@ def test_workspace_queries_3d():
     assert "checkpoint_every > 0" in _native.last_error()
     for bad, msg in ((dict(ny=0), "Grid dimensions must be positive"),
                      (dict(nabc=0), "nabc"), (dict(nt=0), "nt must be positive"),
-                     (dict(src_n=0), "src_n"), (dict(dt=0.0), "dt and dz")):
+                     (dict(src_n=0), "src_n"), (dict(dt=0.0), "dt and dz"),
+                     (dict(nx=1100, ny=1000, nz=1000), "exceeds 2^30 cells"),
+                     (dict(step_kernel=2), "step_kernel"),
+                     (dict(step_kernel=1, shots_per_block=3), "shots_per_block")):
         assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(**bad)), 0) == 0
         assert msg in _native.last_error()
 
