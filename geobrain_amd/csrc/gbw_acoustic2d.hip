@@ -1814,7 +1814,6 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
         sLZ[r][c] = lz[n];
       }
     }
-    if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
     // the q phase below writes only LDS arrays the previous shot last read
     // before its final barrier; sLP / sLZ are read only with a free surface
     if (d.fs) __syncthreads();
@@ -1851,6 +1850,8 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int step = a.it - k;
+      // the next shot's loads overlap this shot's second step
+      if (k == 1 && s + 1 < s_end) load(s + 1);
       const unsigned preg_s = k == 0 ? F_P1 : F_P2;
       const unsigned vreg_s = k == 0 ? F_V1 : F_T;
       float4 dcur[NS], gxc[RHO ? NS : 1], gzc[RHO ? NS : 1];
