@@ -314,3 +314,31 @@ def test_3d_propagator_end_to_end(fs):
     assert err <= GTOL
     one = prop.forward(shot_index=1)
     np.testing.assert_array_equal(one.p.detach().cpu().numpy()[0], res.p.detach().cpu().numpy()[1])
+
+
+def test_3d_shots_independent():
+    """A 4-shot call equals each shot run alone, bit for bit (seismograms)."""
+    print("[case] 3d shot independence 30x26x40", flush=True)
+    a = random_case(30, 26, 40, 6, 50, 4, True, 61, R=20)
+    full = run3(a)
+    for s in range(4):
+        one = dict(a, src_z=a["src_z"][s:s + 1], src_y=a["src_y"][s:s + 1],
+                   src_x=a["src_x"][s:s + 1], src_v=np.asarray(a["src_v"])[s:s + 1])
+        o = run3(one)
+        for k in REC:
+            np.testing.assert_array_equal(o[k][0], full[k][s], err_msg=f"{k} shot {s}")
+
+
+def test_3d_dot_product_fp32():
+    """<F w, g> against <w, F^T g> through the HIP forward and adjoint (80x70x90, 4 shots)."""
+    print("[case] 3d dot product 80x70x90", flush=True)
+    a = random_case(80, 70, 90, 10, 120, 4, False, 71, R=50)
+    S, nt = np.asarray(a["src_v"]).shape
+    R = len(a["rcv_z"])
+    rng = np.random.default_rng(9)
+    gs = [rng.standard_normal((S, nt, R)).astype(np.float32) for _ in range(4)]
+    out, g = grad3(a, gs, segments=6, want=("w",))
+    lhs = sum(float(np.sum(out[k].astype(np.float64) * gk)) for k, gk in zip(REC, gs))
+    rhs = float(np.sum(np.asarray(a["src_v"], np.float64) * g["w"]))
+    print(f"  lhs {lhs:.9e} rhs {rhs:.9e} rel {abs(lhs - rhs) / abs(lhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
