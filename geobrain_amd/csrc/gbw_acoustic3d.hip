@@ -60,11 +60,23 @@ __device__ __forceinline__ float d4(float a, float b, float c, float d) {
   return t1 + t2;
 }
 
-// workgroup index -> block of 256 groups: consecutive workgroups go to
-// different XCDs (round-robin over 8), so give each XCD a contiguous range
+// workgroup index -> work index: consecutive workgroups go to different XCDs
+// (round-robin over 8), so give each XCD a contiguous range
 __device__ __forceinline__ long long xcd_block3() {
   const long long b = blockIdx.x;
   return (b & 7) * (long long)(gridDim.x >> 3) + (b >> 3);
+}
+
+// step kernels: thread -> float4 group of the plane, z, y, x order
+__device__ __forceinline__ bool lin_cell3(const Dims3& d, int& g, int& y, int& z, unsigned& q4) {
+  const long long gq = xcd_block3() * T3 + threadIdx.x;
+  if (gq >= d.groups) return false;
+  q4 = (unsigned)gq;
+  g = (int)(q4 % (unsigned)d.GX);
+  const unsigned zy = q4 / (unsigned)d.GX;
+  y = (int)(zy % (unsigned)d.NY);
+  z = (int)(zy / (unsigned)d.NY);
+  return true;
 }
 
 // coefficients (acoustic_3d.py:401-431), pad_model_3d as a clamp map
@@ -140,12 +152,9 @@ struct Step3Args {
 // float4), so loads use a uniform base and a 32-bit lane offset.
 __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
   const Dims3 d = a.d;
-  const long long gq = xcd_block3() * T3 + threadIdx.x;  // float4 group of the plane
-  if (gq >= d.groups) return;
-  const unsigned q4 = (unsigned)gq;
-  const int g = (int)(q4 % (unsigned)d.GX);
-  const unsigned zy = q4 / (unsigned)d.GX;
-  const int y = (int)(zy % (unsigned)d.NY), z = (int)(zy / (unsigned)d.NY);
+  int g, y, z;
+  unsigned q4;  // float4 group of the plane
+  if (!lin_cell3(d, g, y, z, q4)) return;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   unsigned mask = 0;
   if (z >= d.f + 1 && z <= d.NZ - 3 && y >= 2 && y <= d.NY - 3)
@@ -215,12 +224,9 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
 // the p mirror, vx / vy / vz on their regions, the vz mirror, energy sums
 __global__ __launch_bounds__(T3) void v3_step_kernel(Step3Args a) {
   const Dims3 d = a.d;
-  const long long gq = xcd_block3() * T3 + threadIdx.x;
-  if (gq >= d.groups) return;
-  const unsigned q4 = (unsigned)gq;
-  const int g = (int)(q4 % (unsigned)d.GX);
-  const unsigned zy = q4 / (unsigned)d.GX;
-  const int y = (int)(zy % (unsigned)d.NY), z = (int)(zy / (unsigned)d.NY);
+  int g, y, z;
+  unsigned q4;
+  if (!lin_cell3(d, g, y, z, q4)) return;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   const bool zx = z >= d.f && z <= d.NZ - 2, zz = z >= d.f && z <= d.NZ - 3;
   unsigned mx = 0, my = 0, mz = 0, me = 0;
@@ -357,21 +363,25 @@ __global__ void fill3_kernel(float* __restrict__ p, long long n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint (reverse) step.  lam holds the adjoint of the state after step `it`
-// (its receivers' adjoint sources injected) and leaves with the adjoint of the
-// state before it.  Three launches, each transposing part of the step:
-//   adj3_q_kernel  vz[f-1] = vz[f] and the vz / vy / vx updates: q = alpha2 lam_v,
-//                  d alpha2 -= lam_v bracket(p'), lam_v *= 1 - kappa
-//   adj3_p_kernel  lam_p += -D^T q, p[f-1] = -p[f+1], the source term (wavelet
-//                  gradient), the p update: r = alpha1 lam_p, d alpha1 -= lam_p D,
-//                  lam_p *= 1 - kappa1
-//   adj3_v_kernel  lam_v += -B^T r
+// Adjoint (reverse) step.  The transpose of step `it` runs in three parts:
+//   q part   the adjoint sources of step `it`, vz[f-1] = vz[f] and the vz / vy / vx
+//            updates: q = alpha2 lam_v, d alpha2 -= lam_v bracket(p'),
+//            lam_v *= 1 - kappa
+//   p part   lam_p += -D^T q, p[f-1] = -p[f+1], the source term (wavelet
+//            gradient), the p update: r = alpha1 lam_p, d alpha1 -= lam_p D,
+//            lam_p *= 1 - kappa1
+//   v part   lam_v += -B^T r
+// Two launches per reverse step: adj3_vq_kernel runs the v part of step it+1
+// and the q part of step `it` in one pass over lam_v (both are pointwise in
+// lam_v once r of step it+1 is written), adj3_p_kernel the p part.  The v part
+// of step 0 changes no gradient and is not run.
 // ---------------------------------------------------------------------------
 
 struct Adj3Args {
   Dims3 d;
   int S, spb, it, nt;
   float dt;
+  int vpart;               // adj3_vq_kernel: run the v part of step it+1 first
   float* lam;              // [field p|vx|vy|vz][shot][plane]
   long long fstride;       // S * plane
   float* q;                // [qx|qy|qz][shot][plane]
@@ -384,6 +394,14 @@ struct Adj3Args {
   const int* src_cell;
   float* gw;               // grad_src_v (S, nt), or null
   float* gws;              // [S] source terms of this step for a shared wavelet, or null
+  // adjoint sources: receiver cells sorted by cell, the receivers of each in
+  // receiver order, and per (z, y) row the range of its cells; null: none
+  const int* rowstart;     // [NZ * NY + 1] into ucell
+  const int* ucell;        // in-plane cell index
+  const int* ustart;       // [ncell + 1] into urcv
+  const int* urcv;         // receiver numbers
+  int R;
+  const float* g_p; const float* g_vx; const float* g_vy; const float* g_vz;  // (S, nt, R) or null
 };
 
 // float4 array addressed by a 32-bit element index (byte offset i * 16 < 2^32):
@@ -403,30 +421,51 @@ __device__ __forceinline__ float4 ld4(const A4<const float4>& a, unsigned i, boo
 
 struct Cell3 {
   int g, y, z;
-  long long q4;
+  unsigned q4;
 };
 
 __device__ __forceinline__ bool cell3(const Dims3& d, Cell3& c) {
-  const long long gq = xcd_block3() * T3 + threadIdx.x;
-  if (gq >= d.groups) return false;
-  c.g = (int)(gq % d.GX);
-  const long long zy = gq / d.GX;
-  c.y = (int)(zy % d.NY);
-  c.z = (int)(zy / d.NY);
-  c.q4 = gq;
-  return true;
+  return lin_cell3(d, c.g, c.y, c.z, c.q4);
 }
 
 __device__ __forceinline__ float4 ld4(const float4* p, long long i, bool ok) {
   return ok ? p[i] : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
+// receiver cells of the float4 group q4 (plane row `row`): ucell[u0 .. u1)
+__device__ __forceinline__ void rcv_cells(const Adj3Args& a, int row, unsigned q4, int& u0,
+                                          int& u1) {
+  int lo = a.rowstart[row], hi = a.rowstart[row + 1];
+  const int end = hi, c0 = (int)(q4 << 2);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a.ucell[mid] < c0) lo = mid + 1;
+    else hi = mid;
+  }
+  u0 = u1 = lo;
+  while (u1 < end && a.ucell[u1] < c0 + 4) ++u1;
+}
+
+// v += the adjoint sources of cells ucell[u0 .. u1) (group starting at cell c0),
+// the receivers of a cell in receiver order
+__device__ __forceinline__ void add_sources(const Adj3Args& a, const float* gf, long long base,
+                                            int u0, int u1, int c0, float4& v) {
+  for (int u = u0; u < u1; ++u) {
+    const int e = a.ucell[u] - c0;
+    float x = g3(v, e);
+    for (int j = a.ustart[u]; j < a.ustart[u + 1]; ++j) x = x + gf[base + a.urcv[j]];
+    s3(v, e, x);
+  }
+}
+
+// v part of step it+1, adjoint sources of step `it`, q part of step `it`
+__global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
   const Dims3 d = a.d;
   Cell3 c;
   if (!cell3(d, c)) return;
   const int g = c.g, y = c.y, z = c.z;
   const unsigned q4 = (unsigned)c.q4;
+  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
   const bool zx = z >= d.f && z <= d.NZ - 2, zz = z >= d.f && z <= d.NZ - 3;
   unsigned mx = 0, my = 0, mz = 0;
 #pragma unroll
@@ -436,27 +475,76 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
     if (zx && y >= 1 && y <= d.NY - 3 && x >= 1 && x <= d.NX - 2) my |= 1u << e;
     if (zz && y >= 1 && y <= d.NY - 2 && x >= 1 && x <= d.NX - 2) mz |= 1u << e;
   }
-  const bool fold = d.fs && z == d.f;  // transpose of vz[f-1] = vz[f]
-  if (!(mx | my | mz) && !fold) return;
+  const bool fold = d.fs && z == d.f;       // transpose of vz[f-1] = vz[f]
+  // with a free surface lam_vz of plane f-1 is folded into plane f by that
+  // plane's thread, which recomputes it; here it stays 0 in memory
+  const bool above = d.fs && z == d.f - 1;
+  // the v part reaches z in [f-1, NZ-2], y <= NY-2 (r is zero beyond)
+  const bool vin = a.vpart && z >= d.f - 1 && z <= d.NZ - 2 && y <= d.NY - 2;
+  const bool vin_up = fold && a.vpart && y <= d.NY - 2;  // the v part of plane f-1's cell
+  int u0 = 0, u1 = 0, w0 = 0, w1 = 0;  // receiver cells of this group and of plane f-1's
+  if (a.rowstart) {
+    rcv_cells(a, z * d.NY + y, q4, u0, u1);
+    if (fold) rcv_cells(a, (z - 1) * d.NY + y, q4 - sz4, w0, w1);
+  }
+  const bool rcv = u1 > u0;
+  if (!(mx | my | mz) && !fold && !vin && !rcv) return;
   const float4* __restrict__ C = reinterpret_cast<const float4*>(a.coef);
   const long long p4 = d.plane >> 2;
   const float4 omk2 = ld4u(C + 2 * p4, q4), omk3 = ld4u(C + 3 * p4, q4);
   const float4 omk4 = ld4u(C + 4 * p4, q4), a2 = ld4u(C + 5 * p4, q4);
-  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
   const unsigned gl = g > 0 ? q4 - 1u : q4, gr = g + 1 < d.GX ? q4 + 1u : q4;
   const bool rho = a.gA2 != nullptr;
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
+  const int c0 = (int)(q4 << 2);
   float4 acc2 = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
     const A4<float4> LX{reinterpret_cast<float4*>(a.lam + a.fstride + so)};
     const A4<float4> LY{reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so)};
     const A4<float4> LZ{reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so)};
-    float4 lx = LX[q4], ly = LY[q4], lz = LZ[q4];
+    const A4<const float4> Rr{reinterpret_cast<const float4*>(a.r + so)};
+    float4 lx = LX[q4], ly = LY[q4], lz = above ? make_float4(0.f, 0.f, 0.f, 0.f) : LZ[q4];
+    if (vin) {  // lam_v += c1 (r[i+1] - r[i]) + c2 (r[i+2] - r[i-1]) along each axis
+      const float4 r0 = Rr[q4];
+      const float4 xa = ld4(Rr, q4 - 1, g >= 1), xc = ld4(Rr, q4 + 1, g + 1 < d.GX);
+      const float rs[12] = {xa.x, xa.y, xa.z, xa.w, r0.x, r0.y, r0.z, r0.w, xc.x, xc.y, xc.z, xc.w};
+      const float4 ym1 = ld4(Rr, q4 - sy4, y >= 1), yp1 = Rr[q4 + sy4];
+      const float4 yp2 = ld4(Rr, q4 + 2 * sy4, y + 2 < d.NY);
+      const float4 zm1 = ld4(Rr, q4 - sz4, z >= 1), zp1 = Rr[q4 + sz4];
+      const float4 zp2 = ld4(Rr, q4 + 2 * sz4, z + 2 < d.NZ);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        s3(lx, e, g3(lx, e) + d4(rs[5 + e], rs[4 + e], rs[6 + e], rs[3 + e]));
+        s3(ly, e, g3(ly, e) + d4(g3(yp1, e), g3(r0, e), g3(yp2, e), g3(ym1, e)));
+        s3(lz, e, g3(lz, e) + d4(g3(zp1, e), g3(r0, e), g3(zp2, e), g3(zm1, e)));
+      }
+    }
+    const long long base = ((long long)s * a.nt + a.it) * a.R;
+    if (rcv) {
+      if (a.g_p) {
+        const A4<float4> LP{reinterpret_cast<float4*>(a.lam + so)};
+        float4 lp = LP[q4];
+        add_sources(a, a.g_p, base, u0, u1, c0, lp);
+        LP[q4] = lp;
+      }
+      if (a.g_vx) add_sources(a, a.g_vx, base, u0, u1, c0, lx);
+      if (a.g_vy) add_sources(a, a.g_vy, base, u0, u1, c0, ly);
+      if (a.g_vz && !above) add_sources(a, a.g_vz, base, u0, u1, c0, lz);
+    }
     if (fold) {
-      const float4 up = LZ[q4 - sz4];
+      // lam_vz of plane f-1 after the v part and its sources (it starts the step at 0)
+      float4 up = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (vin_up) {
+        const unsigned qm = q4 - sz4;
+        const int zm = z - 1;
+        const float4 r0 = Rr[qm], zm1 = ld4(Rr, qm - sz4, zm >= 1), zp1 = Rr[qm + sz4];
+        const float4 zp2 = ld4(Rr, qm + 2 * sz4, zm + 2 < d.NZ);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s3(up, e, 0.f + d4(g3(zp1, e), g3(r0, e), g3(zp2, e), g3(zm1, e)));
+      }
+      if (w1 > w0 && a.g_vz) add_sources(a, a.g_vz, base, w0, w1, c0 - (int)(sz4 << 2), up);
       lz = make_float4(lz.x + up.x, lz.y + up.y, lz.z + up.z, lz.w + up.w);
-      LZ[q4 - sz4] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     float4 qx = make_float4(0.f, 0.f, 0.f, 0.f), qy = qx, qz = qx;
     float pxs[12];
@@ -508,10 +596,13 @@ __global__ __launch_bounds__(T3) void adj3_q_kernel(Adj3Args a) {
     const A4<float4> QX{reinterpret_cast<float4*>(a.q + so)};
     const A4<float4> QY{reinterpret_cast<float4*>(a.q + a.fstride + so)};
     const A4<float4> QZ{reinterpret_cast<float4*>(a.q + 2 * a.fstride + so)};
-    if (mx) { QX[q4] = qx; LX[q4] = lx; }
-    if (my) { QY[q4] = qy; LY[q4] = ly; }
-    if (mz || fold) LZ[q4] = lz;
+    const bool upd = vin || rcv;
+    if (mx) QX[q4] = qx;
+    if (my) QY[q4] = qy;
     if (mz) QZ[q4] = qz;
+    if (mx || upd) LX[q4] = lx;
+    if (my || upd) LY[q4] = ly;
+    if ((mz || fold || upd) && !above) LZ[q4] = lz;
   }
   if (rho && (mx | my | mz)) {
     const A4<float4> G{reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane)};
@@ -615,43 +706,6 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
     const A4<float4> G{reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane)};
     const float4 o = G[q4];
     G[q4] = make_float4(o.x + acc1.x, o.y + acc1.y, o.z + acc1.z, o.w + acc1.w);
-  }
-}
-
-// lam_v += c1 (r[i+1] - r[i]) + c2 (r[i+2] - r[i-1]) along each velocity's axis
-__global__ __launch_bounds__(T3) void adj3_v_kernel(Adj3Args a) {
-  const Dims3 d = a.d;
-  Cell3 c;
-  if (!cell3(d, c)) return;
-  const int g = c.g, y = c.y, z = c.z;
-  const unsigned q4 = (unsigned)c.q4;
-  // r is zero outside z in [f+1, NZ-3], y in [2, NY-3]: skip cells no term reaches
-  if (z < d.f - 1 || z > d.NZ - 2 || y > d.NY - 2) return;
-  const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
-  const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
-  for (int s = s_begin; s < s_end; ++s) {
-    const long long so = (long long)s * d.plane;
-    const A4<const float4> Rr{reinterpret_cast<const float4*>(a.r + so)};
-    const A4<float4> LX{reinterpret_cast<float4*>(a.lam + a.fstride + so)};
-    const A4<float4> LY{reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so)};
-    const A4<float4> LZ{reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so)};
-    const float4 r0 = Rr[q4];
-    const float4 xa = ld4(Rr, q4 - 1, g >= 1), xc = ld4(Rr, q4 + 1, g + 1 < d.GX);
-    const float rs[12] = {xa.x, xa.y, xa.z, xa.w, r0.x, r0.y, r0.z, r0.w, xc.x, xc.y, xc.z, xc.w};
-    const float4 ym1 = ld4(Rr, q4 - sy4, y >= 1), yp1 = Rr[q4 + sy4];
-    const float4 yp2 = ld4(Rr, q4 + 2 * sy4, y + 2 < d.NY);
-    const float4 zm1 = ld4(Rr, q4 - sz4, z >= 1), zp1 = Rr[q4 + sz4];
-    const float4 zp2 = ld4(Rr, q4 + 2 * sz4, z + 2 < d.NZ);
-    float4 lx = LX[q4], ly = LY[q4], lz = LZ[q4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      s3(lx, e, g3(lx, e) + d4(rs[5 + e], rs[4 + e], rs[6 + e], rs[3 + e]));
-      s3(ly, e, g3(ly, e) + d4(g3(yp1, e), g3(r0, e), g3(yp2, e), g3(ym1, e)));
-      s3(lz, e, g3(lz, e) + d4(g3(zp1, e), g3(r0, e), g3(zp2, e), g3(zm1, e)));
-    }
-    LX[q4] = lx;
-    LY[q4] = ly;
-    LZ[q4] = lz;
   }
 }
 
@@ -964,27 +1018,6 @@ __global__ __launch_bounds__(MT3) void fwd3_march_kernel(March3Args a) {
   }
 }
 
-// adjoint sources of step `it`: thread (shot, receiver cell) adds the g of
-// every receiver on that cell, in receiver order
-__global__ void inject3_kernel(long long plane, int S, int nt, int it, int R, int ncell,
-                               const int* ucell, const int* ustart, const int* urcv,
-                               const float* gp, const float* gvx, const float* gvy,
-                               const float* gvz, float* lam, long long fstride) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)S * ncell) return;
-  const int s = (int)(i / ncell), u = (int)(i % ncell);
-  const long long base = ((long long)s * nt + it) * R;
-  const long long q = (long long)s * plane + ucell[u];
-  const float* gs[4] = {gp, gvx, gvy, gvz};
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    if (!gs[f]) continue;
-    float v = lam[f * fstride + q];
-    for (int j = ustart[u]; j < ustart[u + 1]; ++j) v = v + gs[f][base + urcv[j]];
-    lam[f * fstride + q] = v;
-  }
-}
-
 // shared wavelet: grad[it] = sum over shots of the shot's source term, in shot order
 __global__ void wavelet_sum3_kernel(const float* gws, int S, float* gw, int it) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -1045,7 +1078,7 @@ struct Plan3 {
   long long units;
   size_t off_coef, off_state, off_E, off_src, off_rcell;
   size_t off_lam, off_q, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
-      off_urcv;
+      off_urcv, off_rowstart;
   size_t total;
 };
 
@@ -1127,6 +1160,7 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
     pl.off_ucell = take(R1 * sizeof(int));
     pl.off_ustart = take((R1 + 1) * sizeof(int));
     pl.off_urcv = take(R1 * sizeof(int));
+    pl.off_rowstart = take(((size_t)d.NZ * d.NY + 1) * sizeof(int));
   }
   pl.total = o;
   return 0;
@@ -1333,7 +1367,10 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
     ustart.push_back(pl.R);
     if (ucell.empty()) { ucell.push_back(0); urcv.push_back(0); }
   }
-  const int ncell = pl.R > 0 ? (int)ustart.size() - 1 : 0;
+  // per (z, y) row of the plane, the range of its receiver cells in ucell
+  std::vector<int> rowstart((size_t)d.NZ * d.NY + 1, 0);
+  for (int u = 0; u + 1 < (int)ustart.size(); ++u) ++rowstart[ucell[u] / d.PX + 1];
+  for (size_t i = 1; i < rowstart.size(); ++i) rowstart[i] += rowstart[i - 1];
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_src), src.data(), src.size() * sizeof(int),
                          hipMemcpyHostToDevice, st));
@@ -1343,6 +1380,8 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
                          ustart.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_urcv), urcv.data(), urcv.size() * sizeof(int),
                          hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at3<int>(workspace, pl.off_rowstart), rowstart.data(),
+                         rowstart.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));  // the host vectors go out of scope
   float* coef = at3<float>(workspace, pl.off_coef);
   coef3_kernel<<<(unsigned)((d.plane + 255) / 256), 256, 0, st>>>(d, in->vp, in->rho, in->damp,
@@ -1381,7 +1420,14 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   b.gw = (adj->grad_src_v && !shared) ? adj->grad_src_v : nullptr;
   b.gws = (adj->grad_src_v && shared) ? at3<float>(workspace, pl.off_gws) : nullptr;
   const dim3 grid(step_blocks3(d), pl.groups);
-  const bool inject = pl.R > 0 && (adj->g_p || adj->g_vx || adj->g_vy || adj->g_vz);
+  if (pl.R > 0 && (adj->g_p || adj->g_vx || adj->g_vy || adj->g_vz)) {
+    b.rowstart = at3<int>(workspace, pl.off_rowstart);
+    b.ucell = at3<int>(workspace, pl.off_ucell);
+    b.ustart = at3<int>(workspace, pl.off_ustart);
+    b.urcv = at3<int>(workspace, pl.off_urcv);
+    b.R = pl.R;
+    b.g_p = adj->g_p; b.g_vx = adj->g_vx; b.g_vy = adj->g_vy; b.g_vz = adj->g_vz;
+  }
   for (int c = pl.nseg - 1; c >= 0; --c) {
     const int t0 = c * pl.k, t1 = std::min(pl.nt, t0 + pl.k);
     HIP_TRY(hipMemcpyAsync(state,
@@ -1403,19 +1449,13 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
       }
     }
     for (int it = t1 - 1; it >= t0; --it) {
-      if (inject) {
-        inject3_kernel<<<(unsigned)(((long long)pl.S * ncell + 255) / 256), 256, 0, st>>>(
-            d.plane, pl.S, pl.nt, it, pl.R, ncell, at3<int>(workspace, pl.off_ucell),
-            at3<int>(workspace, pl.off_ustart), at3<int>(workspace, pl.off_urcv), adj->g_p,
-            adj->g_vx, adj->g_vy, adj->g_vz, lam, fstride);
-        LAUNCH_CHECK();
-      }
       b.it = it;
+      b.vpart = it + 1 < pl.nt;  // r of step it+1 is in the buffer
       b.dstep = D + (size_t)(it - t0) * fstride;
       b.pstep = P1 ? P1 + (size_t)(it - t0) * fstride : nullptr;
       {
-        LaunchTimer timer("adj3_q_kernel", st);
-        adj3_q_kernel<<<grid, T3, 0, st>>>(b);
+        LaunchTimer timer("adj3_vq_kernel", st);
+        adj3_vq_kernel<<<grid, T3, 0, st>>>(b);
       }
       LAUNCH_CHECK();
       {
@@ -1427,11 +1467,6 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
         wavelet_sum3_kernel<<<1, 64, 0, st>>>(b.gws, pl.S, adj->grad_src_v, it);
         LAUNCH_CHECK();
       }
-      {
-        LaunchTimer timer("adj3_v_kernel", st);
-        adj3_v_kernel<<<grid, T3, 0, st>>>(b);
-      }
-      LAUNCH_CHECK();
     }
   }
   if (adj->grad_vp || adj->grad_rho) {
