@@ -60,22 +60,24 @@ __device__ __forceinline__ float d4(float a, float b, float c, float d) {
   return t1 + t2;
 }
 
-// workgroup index -> work index: consecutive workgroups go to different XCDs
-// (round-robin over 8), so give each XCD a contiguous range
-__device__ __forceinline__ long long xcd_block3() {
-  const long long b = blockIdx.x;
-  return (b & 7) * (long long)(gridDim.x >> 3) + (b >> 3);
-}
-
-// step kernels: thread -> float4 group of the plane, z, y, x order
+// Step kernels: XCD k (workgroups are dealt to the 8 XCDs round-robin) owns
+// the y-slab [k NY / 8, (k + 1) NY / 8) of every plane and walks it in z, y,
+// x order, 256 float4 groups per workgroup.  The workgroups resident on an
+// XCD at one time then cover a few consecutive slab planes, so the planes
+// above and below that the z-stencils read are fetched into its L2 by
+// neighbouring workgroups at about the same time.  (Measured on a 240^3 grid
+// with 8 shots: 6% faster than contiguous whole planes per XCD; cutting each
+// slab into 2, 4 or 8 sub-slabs walked one after the other is slower.)
 __device__ __forceinline__ bool lin_cell3(const Dims3& d, int& g, int& y, int& z, unsigned& q4) {
-  const long long gq = xcd_block3() * T3 + threadIdx.x;
-  if (gq >= d.groups) return false;
-  q4 = (unsigned)gq;
-  g = (int)(q4 % (unsigned)d.GX);
-  const unsigned zy = q4 / (unsigned)d.GX;
-  y = (int)(zy % (unsigned)d.NY);
-  z = (int)(zy / (unsigned)d.NY);
+  const int k = (int)(blockIdx.x & 7);
+  const int y0 = k * d.NY / 8, rows = (k + 1) * d.NY / 8 - y0;
+  const long long l = (long long)(blockIdx.x >> 3) * T3 + threadIdx.x;
+  if (rows <= 0 || l >= (long long)d.NZ * rows * d.GX) return false;
+  g = (int)(l % d.GX);
+  const long long t = l / d.GX;
+  y = y0 + (int)(t % rows);
+  z = (int)(t / rows);
+  q4 = ((unsigned)z * (unsigned)d.NY + (unsigned)y) * (unsigned)d.GX + (unsigned)g;
   return true;
 }
 
@@ -1203,7 +1205,11 @@ int geometry3(const Plan3& pl, const gbw_acoustic3d_inputs* in, std::vector<int>
   return 0;
 }
 
-unsigned step_blocks3(const Dims3& d) { return (unsigned)(((d.groups + T3 - 1) / T3 + 7) / 8 * 8); }
+// workgroups of a step kernel: 8 x those of the largest y-slab
+unsigned step_blocks3(const Dims3& d) {
+  const long long slab = (long long)d.NZ * ((d.NY + 7) / 8) * d.GX;
+  return (unsigned)(8 * ((slab + T3 - 1) / T3));
+}
 
 // one forward step of every shot (p, then v)
 int forward_step3(Step3Args& a, const dim3& grid, hipStream_t st) {
