@@ -1789,6 +1789,11 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     const int scell = sSrc[s - s_begin];
     const int ssz = scell / d.PX, ssx = scell - ssz * d.PX;
     const bool src_tile = ssz >= z0 && ssz < z0 + TZ && ssx >= x0 && ssx < x0 + TX;
+    // Re-derive the region tests from fl each shot: left loop-invariant, the
+    // compiler keeps each as a 64-bit lane mask in SGPRs across the loop and
+    // spills them to VGPR lanes (about 90 readlanes per shot; 1.6% slower).
+#pragma unroll
+    for (int n = 0; n < NS; ++n) asm volatile("" : "+v"(fl[n]));
     // adjoint state of this slot (registers)
     float4 lp[NS], lx[NS], lz[NS];
     // D (and brackets) of step it for the tile slots (the others read a valid
