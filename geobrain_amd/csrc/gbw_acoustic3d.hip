@@ -375,8 +375,10 @@ __global__ void fill3_kernel(float* __restrict__ p, long long n, float v) {
 //   v part   lam_v += -B^T r
 // Two launches per reverse step: adj3_vq_kernel runs the v part of step it+1
 // and the q part of step `it` in one pass over lam_v (both are pointwise in
-// lam_v once r of step it+1 is written), adj3_p_kernel the p part.  The v part
-// of step 0 changes no gradient and is not run.
+// lam_v once r of step it+1 is written), adj3_p_kernel the p part.  lam_v is
+// stored before the q part's scaling: adj3_p_kernel forms q = alpha2 lam_v on
+// its stencil as it reads it, and the next adj3_vq_kernel applies the scaling,
+// so q is never stored.  The v part of step 0 changes no gradient and is not run.
 // ---------------------------------------------------------------------------
 
 struct Adj3Args {
@@ -386,7 +388,6 @@ struct Adj3Args {
   int vpart;               // adj3_vq_kernel: run the v part of step it+1 first
   float* lam;              // [field p|vx|vy|vz][shot][plane]
   long long fstride;       // S * plane
-  float* q;                // [qx|qy|qz][shot][plane]
   float* r;                // [shot][plane]
   const float* coef;
   const float* dstep;      // p-update bracket of step `it`, [shot][plane]
@@ -506,7 +507,15 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
     const A4<float4> LY{reinterpret_cast<float4*>(a.lam + 2 * a.fstride + so)};
     const A4<float4> LZ{reinterpret_cast<float4*>(a.lam + 3 * a.fstride + so)};
     const A4<const float4> Rr{reinterpret_cast<const float4*>(a.r + so)};
+    // lam_v as the q part of step it+1 left it, before its scaling
+    // lam_v *= 1 - kappa on the v regions, which is applied here
     float4 lx = LX[q4], ly = LY[q4], lz = above ? make_float4(0.f, 0.f, 0.f, 0.f) : LZ[q4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if ((mz >> e) & 1u) s3(lz, e, g3(omk4, e) * g3(lz, e));
+      if ((my >> e) & 1u) s3(ly, e, g3(omk3, e) * g3(ly, e));
+      if ((mx >> e) & 1u) s3(lx, e, g3(omk2, e) * g3(lx, e));
+    }
     if (vin) {  // lam_v += c1 (r[i+1] - r[i]) + c2 (r[i+2] - r[i-1]) along each axis
       const float4 r0 = Rr[q4];
       const float4 xa = ld4(Rr, q4 - 1, g >= 1), xc = ld4(Rr, q4 + 1, g + 1 < d.GX);
@@ -522,6 +531,7 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
         s3(lz, e, g3(lz, e) + d4(g3(zp1, e), g3(r0, e), g3(zp2, e), g3(zm1, e)));
       }
     }
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     const long long base = ((long long)s * a.nt + a.it) * a.R;
     if (rcv) {
       if (a.g_p) {
@@ -536,7 +546,7 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
     }
     if (fold) {
       // lam_vz of plane f-1 after the v part and its sources (it starts the step at 0)
-      float4 up = make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 up = zero;
       if (vin_up) {
         const unsigned qm = q4 - sz4;
         const int zm = z - 1;
@@ -548,10 +558,9 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
       if (w1 > w0 && a.g_vz) add_sources(a, a.g_vz, base, w0, w1, c0 - (int)(sz4 << 2), up);
       lz = make_float4(lz.x + up.x, lz.y + up.y, lz.z + up.z, lz.w + up.w);
     }
-    float4 qx = make_float4(0.f, 0.f, 0.f, 0.f), qy = qx, qz = qx;
-    float pxs[12];
-    float4 pb = qx, pyu = qx, pyd1 = qx, pyd2 = qx, pzu = qx, pzd1 = qx, pzd2 = qx;
-    if (rho) {  // brackets of the v updates, from p after the step
+    if (rho) {  // d alpha2 -= lam_v bracket(p'), the brackets of the v updates from p after the step
+      float pxs[12];
+      float4 pb = zero, pyu = zero, pyd1 = zero, pyd2 = zero, pzu = zero, pzd1 = zero, pzd2 = zero;
       const A4<const float4> P1{reinterpret_cast<const float4*>(a.pstep + so)};
       pb = P1[q4];
       if (mx) {
@@ -571,37 +580,18 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
         // the forward read the mirrored plane f-1 as -p[f+1]
         pzu = fold ? make_float4(-pzd1.x, -pzd1.y, -pzd1.z, -pzd1.w) : P1[q4 - sz4];
       }
-    }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      float acc = g3(acc2, e);
-      if ((mz >> e) & 1u) {
-        const float l = g3(lz, e);
-        s3(qz, e, g3(a2, e) * l);
-        if (rho) acc = acc - l * d4(g3(pzd1, e), g3(pb, e), g3(pzd2, e), g3(pzu, e));
-        s3(lz, e, g3(omk4, e) * l);
+      for (int e = 0; e < 4; ++e) {
+        float acc = g3(acc2, e);
+        if ((mz >> e) & 1u) acc = acc - g3(lz, e) * d4(g3(pzd1, e), g3(pb, e), g3(pzd2, e), g3(pzu, e));
+        if ((my >> e) & 1u) acc = acc - g3(ly, e) * d4(g3(pyd1, e), g3(pb, e), g3(pyd2, e), g3(pyu, e));
+        if ((mx >> e) & 1u) acc = acc - g3(lx, e) * d4(pxs[5 + e], pxs[4 + e], pxs[6 + e], pxs[3 + e]);
+        s3(acc2, e, acc);
       }
-      if ((my >> e) & 1u) {
-        const float l = g3(ly, e);
-        s3(qy, e, g3(a2, e) * l);
-        if (rho) acc = acc - l * d4(g3(pyd1, e), g3(pb, e), g3(pyd2, e), g3(pyu, e));
-        s3(ly, e, g3(omk3, e) * l);
-      }
-      if ((mx >> e) & 1u) {
-        const float l = g3(lx, e);
-        s3(qx, e, g3(a2, e) * l);
-        if (rho) acc = acc - l * d4(pxs[5 + e], pxs[4 + e], pxs[6 + e], pxs[3 + e]);
-        s3(lx, e, g3(omk2, e) * l);
-      }
-      s3(acc2, e, acc);
     }
-    const A4<float4> QX{reinterpret_cast<float4*>(a.q + so)};
-    const A4<float4> QY{reinterpret_cast<float4*>(a.q + a.fstride + so)};
-    const A4<float4> QZ{reinterpret_cast<float4*>(a.q + 2 * a.fstride + so)};
+    // stored before the q part's scaling: adj3_p_kernel forms q = alpha2 lam_v
+    // from it, and the next launch scales it
     const bool upd = vin || rcv;
-    if (mx) QX[q4] = qx;
-    if (my) QY[q4] = qy;
-    if (mz) QZ[q4] = qz;
     if (mx || upd) LX[q4] = lx;
     if (my || upd) LY[q4] = ly;
     if ((mz || fold || upd) && !above) LZ[q4] = lz;
@@ -613,18 +603,76 @@ __global__ __launch_bounds__(T3) void adj3_vq_kernel(Adj3Args a) {
   }
 }
 
-// -D^T q at the cell group (z, y, g): c1 (q[i] - q[i-1]) + c2 (q[i+1] - q[i-2]) per axis,
-// added z, then y, then x; q outside the grid is zero
-__device__ __forceinline__ float4 gather_q(const Dims3& d, const A4<const float4>& QX,
-                                           const A4<const float4>& QY,
-                                           const A4<const float4>& QZ, int z, int y, int g) {
+// The update-region masks of the q = alpha2 lam_v that the lam_p gather of one
+// cell group reads, the same for every shot (q is zero off each velocity's
+// region and outside the grid)
+struct QStencil {
+  unsigned mz[4], my[4], mx[3];  // element masks: planes z-2 .. z+1; rows y-2 .. y+1; groups g-1 .. g+1
+};
+
+__device__ __forceinline__ QStencil q_stencil(const Dims3& d, int z, int y, int g) {
+  QStencil t;
+  unsigned xyz = 0;  // x in [1, NX-2]: the vy and vz regions
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (4 * g + e >= 1 && 4 * g + e <= d.NX - 2) xyz |= 1u << e;
+  const bool yin = y >= 1 && y <= d.NY - 2, zx = z >= d.f && z <= d.NZ - 2;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int zz = z - 2 + i, yy = y - 2 + i;
+    t.mz[i] = (zz >= d.f && zz <= d.NZ - 3 && yin) ? xyz : 0u;
+    t.my[i] = (zx && yy >= 1 && yy <= d.NY - 3) ? xyz : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int h = g - 1 + i;
+    unsigned m = 0;
+    if (zx && yin && h >= 0 && h < d.GX)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (4 * h + e >= 1 && 4 * h + e <= d.NX - 3) m |= 1u << e;
+    t.mx[i] = m;
+  }
+  return t;
+}
+
+// q = alpha2 lam_v on the elements of m, else 0.  Every q of the gather is
+// loaded unconditionally (from the group itself where m is empty, so no
+// address leaves the volume) and masked afterwards: loads under a branch
+// would each wait for their own round trip.  alpha2 is re-read for every
+// shot, which measured faster than holding the gather's nine alpha2 groups
+// in registers across the shot loop.
+__device__ __forceinline__ float4 q_of(const A4<const float4>& A2, const A4<const float4>& L,
+                                       unsigned i, unsigned m, unsigned own) {
+  const unsigned j = m ? i : own;
+  const float4 c = A2[j], l = L[j];
+  float4 o;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) s3(o, e, ((m >> e) & 1u) ? g3(c, e) * g3(l, e) : 0.f);
+  return o;
+}
+
+// -D^T q at the cell group: c1 (q[i] - q[i-1]) + c2 (q[i+1] - q[i-2]) per axis,
+// added z, then y, then x; lam_v is read as adj3_vq_kernel left it.  Also
+// returns q_z of plane z-1 (the lam_p gather of plane z-2 needs it).
+__device__ __forceinline__ float4 gather_q(const Dims3& d, const QStencil& t,
+                                           const A4<const float4>& A2,
+                                           const A4<const float4>& LX,
+                                           const A4<const float4>& LY,
+                                           const A4<const float4>& LZ, unsigned q4,
+                                           float4& qz_up) {
   const unsigned sy4 = (unsigned)d.GX, sz4 = (unsigned)d.NY * (unsigned)d.GX;
-  const unsigned q4 = ((unsigned)z * (unsigned)d.NY + (unsigned)y) * (unsigned)d.GX + (unsigned)g;
-  const float4 z0 = QZ[q4], zm1 = ld4(QZ, q4 - sz4, z >= 1), zm2 = ld4(QZ, q4 - 2 * sz4, z >= 2);
-  const float4 zp1 = ld4(QZ, q4 + sz4, z + 1 < d.NZ);
-  const float4 y0 = QY[q4], ym1 = ld4(QY, q4 - sy4, y >= 1), ym2 = ld4(QY, q4 - 2 * sy4, y >= 2);
-  const float4 yp1 = ld4(QY, q4 + sy4, y + 1 < d.NY);
-  const float4 xa = ld4(QX, q4 - 1, g >= 1), xb = QX[q4], xc = ld4(QX, q4 + 1, g + 1 < d.GX);
+  const float4 zm2 = q_of(A2, LZ, q4 - 2 * sz4, t.mz[0], q4);
+  const float4 zm1 = q_of(A2, LZ, q4 - sz4, t.mz[1], q4);
+  const float4 z0 = q_of(A2, LZ, q4, t.mz[2], q4);
+  const float4 zp1 = q_of(A2, LZ, q4 + sz4, t.mz[3], q4);
+  const float4 ym2 = q_of(A2, LY, q4 - 2 * sy4, t.my[0], q4);
+  const float4 ym1 = q_of(A2, LY, q4 - sy4, t.my[1], q4);
+  const float4 y0 = q_of(A2, LY, q4, t.my[2], q4);
+  const float4 yp1 = q_of(A2, LY, q4 + sy4, t.my[3], q4);
+  const float4 xa = q_of(A2, LX, q4 - 1, t.mx[0], q4);
+  const float4 xb = q_of(A2, LX, q4, t.mx[1], q4);
+  const float4 xc = q_of(A2, LX, q4 + 1, t.mx[2], q4);
   const float xs[12] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w, xc.x, xc.y, xc.z, xc.w};
   float4 out;
 #pragma unroll
@@ -635,6 +683,7 @@ __device__ __forceinline__ float4 gather_q(const Dims3& d, const A4<const float4
     acc = acc + d4(xs[4 + e], xs[3 + e], xs[5 + e], xs[2 + e]);
     s3(out, e, acc);
   }
+  qz_up = zm1;
   return out;
 }
 
@@ -657,11 +706,23 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
   const long long p4 = d.plane >> 2;
   const float4 omk1 = ld4u(C, q4), a1 = ld4u(C + p4, q4);
   const unsigned sz4 = (unsigned)d.NY * (unsigned)d.GX;
+  const A4<const float4> A2{C + 5 * p4};
+  const QStencil qs = q_stencil(d, z, y, g);
   const int s_begin = blockIdx.y * a.spb, s_end = min(a.S, s_begin + a.spb);
   float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
+  // source cells one shot ahead: a load at the top of the shot would be
+  // waited for before anything else of the shot is issued
+  int sc_next = a.src_cell[s_begin];
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const int sc = a.src_cell[s];
+    const int sc = sc_next;
+    {
+      // a per-lane index: a uniform one makes the value scalar, read (and
+      // so waited for) as soon as it is loaded
+      int sn = min(s + 1, s_end - 1);
+      asm volatile("" : "+v"(sn));
+      sc_next = a.src_cell[sn];
+    }
     const bool src_here = (unsigned)(sc >> 2) == q4;
     if (mirrored) {  // the source term on the mirrored plane is overwritten: no gradient
       if (src_here) {
@@ -671,15 +732,26 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
       continue;
     }
     const A4<float4> LP{reinterpret_cast<float4*>(a.lam + so)};
-    const A4<const float4> QX{reinterpret_cast<const float4*>(a.q + so)};
-    const A4<const float4> QY{reinterpret_cast<const float4*>(a.q + a.fstride + so)};
-    const A4<const float4> QZ{reinterpret_cast<const float4*>(a.q + 2 * a.fstride + so)};
+    const A4<const float4> LX{reinterpret_cast<const float4*>(a.lam + a.fstride + so)};
+    const A4<const float4> LY{reinterpret_cast<const float4*>(a.lam + 2 * a.fstride + so)};
+    const A4<const float4> LZ{reinterpret_cast<const float4*>(a.lam + 3 * a.fstride + so)};
     float4 lp = LP[q4];
-    const float4 gq = gather_q(d, QX, QY, QZ, z, y, g);
+    float4 qf;  // q_z of plane z-1
+    const float4 gq = gather_q(d, qs, A2, LX, LY, LZ, q4, qf);
     lp = make_float4(lp.x + gq.x, lp.y + gq.y, lp.z + gq.z, lp.w + gq.w);
     if (fold) {
       float4 lm = LP[q4 - 2 * sz4];
-      const float4 gm = gather_q(d, QX, QY, QZ, z - 2, y, g);
+      // the gather at plane f-1 (= z-2): of the q it reads only q_z of plane f
+      // (= z-1) lies on a velocity region; the zeros are added as the gather does
+      float4 gm;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float acc = 0.f;
+        acc = acc + d4(0.f, 0.f, g3(qf, e), 0.f);
+        acc = acc + d4(0.f, 0.f, 0.f, 0.f);
+        acc = acc + d4(0.f, 0.f, 0.f, 0.f);
+        s3(gm, e, acc);
+      }
       lm = make_float4(lm.x + gm.x, lm.y + gm.y, lm.z + gm.z, lm.w + gm.w);
       lp = make_float4(lp.x - lm.x, lp.y - lm.y, lp.z - lm.z, lp.w - lm.w);
       LP[q4 - 2 * sz4] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1079,7 +1151,7 @@ struct Plan3 {
   int tiles_x, tiles_y, zc;         // z-marching decomposition
   long long units;
   size_t off_coef, off_state, off_E, off_src, off_rcell;
-  size_t off_lam, off_q, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
+  size_t off_lam, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
       off_urcv, off_rowstart;
   size_t total;
 };
@@ -1152,7 +1224,6 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
     pl.off_E = take((size_t)pl.groups * 4 * inner * sizeof(float));
   } else {
     pl.off_lam = take(4 * fld);
-    pl.off_q = take(3 * fld);
     pl.off_r = take(fld);
     pl.off_D = take((size_t)pl.k * fld);
     pl.off_P1 = pass == 2 ? take((size_t)pl.k * fld) : 0;
@@ -1396,14 +1467,12 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   const long long fstride = (long long)pl.S * d.plane;
   float* state = at3<float>(workspace, pl.off_state);
   float* lam = at3<float>(workspace, pl.off_lam);
-  float* q = at3<float>(workspace, pl.off_q);
   float* r = at3<float>(workspace, pl.off_r);
   float* D = at3<float>(workspace, pl.off_D);
   float* P1 = pass == 2 ? at3<float>(workspace, pl.off_P1) : nullptr;
   float* gA1 = at3<float>(workspace, pl.off_gA1);
   float* gA2 = pass == 2 ? at3<float>(workspace, pl.off_gA2) : nullptr;
   if ((rc = fill3(lam, 4 * fstride, 0.f, st))) return rc;
-  if ((rc = fill3(q, 3 * fstride, 0.f, st))) return rc;
   if ((rc = fill3(r, fstride, 0.f, st))) return rc;
   if ((rc = fill3(gA1, (long long)pl.groups * d.plane, 0.f, st))) return rc;
   if (gA2 && (rc = fill3(gA2, (long long)pl.groups * d.plane, 0.f, st))) return rc;
@@ -1421,7 +1490,7 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   m.src_cell = a.src_cell; m.wav = a.wav; m.wav_stride = a.wav_stride;
   Adj3Args b{};
   b.d = d; b.S = pl.S; b.spb = pl.spb; b.nt = pl.nt; b.dt = prob->dt;
-  b.lam = lam; b.fstride = fstride; b.q = q; b.r = r; b.coef = coef;
+  b.lam = lam; b.fstride = fstride; b.r = r; b.coef = coef;
   b.gA1 = gA1; b.gA2 = gA2; b.src_cell = a.src_cell;
   b.gw = (adj->grad_src_v && !shared) ? adj->grad_src_v : nullptr;
   b.gws = (adj->grad_src_v && shared) ? at3<float>(workspace, pl.off_gws) : nullptr;
