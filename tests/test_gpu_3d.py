@@ -113,6 +113,7 @@ CASES = [
     (9, 12, 56, 4, 24, 2, True, 3, False),     # NX = 64: no padding columns
     (12, 7, 61, 5, 26, 4, False, 4, False),    # NX = 71: a second 64-column block
     (6, 6, 6, 1, 20, 1, False, 5, False),      # smallest absorbing layer
+    (7, 4, 9, 1, 20, 2, True, 6, False),       # NY = 6: some XCD y-slabs are empty
 ]
 
 
@@ -229,6 +230,7 @@ GRAD_CASES = [
     (14, 11, 17, 4, 30, 3, True, 42, True, 4, 0, 0),
     (12, 9, 61, 5, 26, 4, True, 43, False, 3, 2, 0),   # NX = 71, two shot groups
     (7, 7, 7, 1, 20, 2, False, 44, False, 2, 1, 0),    # smallest absorbing layer, one shot per group
+    (8, 4, 9, 1, 24, 2, True, 47, False, 2, 0, 0),     # NY = 6: some XCD y-slabs are empty
     (14, 11, 17, 4, 30, 3, True, 45, False, 3, 4, 1),  # z-marching step
     (12, 9, 61, 5, 26, 4, True, 46, True, 2, 2, 1),    # z-marching step, two shot groups
 ]
