@@ -3,7 +3,9 @@
 // Replaces the time stepping of GeoBrain's acoustic_forward_kernel_3d
 // (geobrain/physics/wave/kernels/acoustic_3d.py:20-476): the 4th-order
 // staggered-grid pressure / velocity scheme with split PML damping, point
-// sources, receiver sampling and energy maps, forward only.
+// sources, receiver sampling and energy maps, and the hand-written adjoint
+// that stands in for autograd through its checkpointed steps (vp, rho and
+// wavelet gradients).
 //
 // Layout in HBM: each field (p, vx, vy, vz) is [shot][NZ][NY][PX] float32 with
 // PX = NX rounded up to a multiple of 64, so every (z, y) row starts on a
@@ -21,7 +23,8 @@
 // them through the shots of its shot group, with the coefficients in
 // registers.  Arithmetic rounds once per operation in the reference's order
 // (-ffp-contract=off), so the seismograms match the reference's FP32 PyTorch
-// path bit for bit.
+// path bit for bit.  The opt-in z-marching kernel (fwd3_march_kernel) fuses
+// the p and v launches; the adjoint kernels follow the forward ones below.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
