@@ -146,6 +146,42 @@ def test_batched_shots_equal_single_shots():
             np.testing.assert_array_equal(np_(one[k])[0], np_(full[k])[s])
 
 
+def test_batched_gradient_equals_sum_of_single_shot_gradients():
+    """The config-3 batch at reduced size (8 shots along the surface of the
+    350 x 1700 model, 1000 steps, long enough for the shots' wavefields to
+    overlap): the batched vp gradient against the sum of the 8 single-shot
+    gradients (FP32 sums in another order, so relative L2), and each shot's
+    wavelet gradient bit for bit."""
+    import bench
+    print("[case] batched gradient vs single shots, 8 shots 350x1700 x 1000 steps", flush=True)
+    w = bench.workload(n_shots=8, nt=1000)
+    S, nt = w["wavelet"].shape
+    R = w["rcv_x"].size
+    g_p = np.random.default_rng(3).standard_normal((S, nt, R)).astype(np.float32)
+
+    def grad(shots):
+        vp = _t(w["vp"], dtype=torch.float32).requires_grad_(True)
+        wv = _t(w["wavelet"][shots], dtype=torch.float32).requires_grad_(True)
+        out = wave.acoustic_forward_kernel(
+            nx=w["nx"], nz=w["nz"], dx=w["dx"], dz=w["dx"], nt=nt, dt=w["dt"], nabc=w["nabc"],
+            free_surface=False, src_x=w["src_x"][shots], src_z=w["src_z"][shots],
+            src_n=len(shots), src_v=wv, rcv_x=w["rcv_x"], rcv_z=w["rcv_z"], rcv_n=R,
+            damp=_t(w["damp"], dtype=torch.float32), vp=vp,
+            rho=_t(w["rho"], dtype=torch.float32), checkpoint_segments=8)
+        (_t(g_p[shots]) * out["p"]).sum().backward()
+        return np_(vp.grad).astype(np.float64), np_(wv.grad)
+
+    g_all, w_all = grad(np.arange(S))
+    g_sum = np.zeros_like(g_all)
+    for s in range(S):
+        g_one, w_one = grad(np.array([s]))
+        g_sum += g_one
+        np.testing.assert_array_equal(w_one[0], w_all[s], err_msg=f"wavelet gradient, shot {s}")
+    e = rel_l2(g_all, g_sum)
+    print(f"  vp gradient: rel-L2 batched vs sum of single shots {e:.2e}")
+    assert np.abs(g_all).max() > 0 and e <= 1e-6
+
+
 @pytest.mark.parametrize("fs", [False, True])
 def test_propagator_end_to_end(fs):
     """AcousticPropagator as examples 07/09 drive it (free surface, Gardner
