@@ -59,10 +59,14 @@ def _small_problem(seed=0, nz=12, nx=15, nabc=4, fs=False, S=2, R=7, nt=60):
     vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
     rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
     damp = oracle.bc_pml(nx, nz, 10.0, 10.0, nabc, float(vp.max()), False).astype(np.float32)
+    src_z, src_x = rng.integers(0, nz, S), rng.integers(0, nx, S)
+    src_v = rng.standard_normal((S, nt)).astype(np.float32)
+    rcv_z, rcv_x = rng.integers(0, nz, R), rng.integers(0, nx, R)
+    # receivers on the padded rows nabc - 1 and nabc: with a free surface the
+    # mirrored row f-1 and the surface f, whose transposes fold into f and f+1
+    rcv_z[0], rcv_z[1] = -1, 0
     return dict(nz=nz, nx=nx, nabc=nabc, free_surface=fs, dt=1e-3, dz=10.0,
-                src_z=rng.integers(0, nz, S), src_x=rng.integers(0, nx, S),
-                src_v=rng.standard_normal((S, nt)).astype(np.float32),
-                rcv_z=rng.integers(0, nz, R), rcv_x=rng.integers(0, nx, R),
+                src_z=src_z, src_x=src_x, src_v=src_v, rcv_z=rcv_z, rcv_x=rcv_x,
                 vp=vp, rho=rho, damp=damp)
 
 
@@ -184,6 +188,8 @@ def test_oracle_energy_equals_sum_of_squared_records():
     for k in ("p", "vx", "vz"):
         e = out["forward_wavefield_" + k]
         for r in range(len(args["rcv_z"])):
+            if args["rcv_z"][r] < 0:  # in the absorbing layer: no energy-map cell
+                continue
             want = float(np.sum(out[k][:, :, r] ** 2))
             assert abs(e[args["rcv_z"][r], args["rcv_x"][r]] - want) <= 1e-12 * max(want, 1e-300)
 

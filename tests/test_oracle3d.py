@@ -144,11 +144,15 @@ def _grad_case(seed, fs, nt=40, S=2, R=6, nz=9, ny=8, nx=10, nabc=3):
     vp = rng.uniform(1600, 3000, (nz, ny, nx)).astype(np.float32)
     rho = rng.uniform(1800, 2400, (nz, ny, nx)).astype(np.float32)
     damp = oracle.bc_pml_3d(nx, ny, nz, 10.0, 10.0, 10.0, nabc, float(vp.max()), False)
+    rz = rng.integers(-1 if fs else 0, nz, R)
+    # receivers on the padded planes nabc - 1 and nabc: with a free surface the
+    # mirrored plane f-1 and the surface f, whose transposes fold into f and f+1
+    rz[0], rz[1] = -1, 0
     return dict(nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=fs, dt=1e-3, dz=10.0,
                 src_z=rng.integers(0, nz, S), src_y=rng.integers(0, ny, S),
                 src_x=rng.integers(0, nx, S),
                 src_v=rng.standard_normal((S, nt)).astype(np.float32),
-                rcv_z=rng.integers(-1 if fs else 0, nz, R), rcv_y=rng.integers(0, ny, R),
+                rcv_z=rz, rcv_y=rng.integers(0, ny, R),
                 rcv_x=rng.integers(0, nx, R), vp=vp, rho=rho, damp=damp.astype(np.float32))
 
 
