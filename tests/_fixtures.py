@@ -52,31 +52,32 @@ OUT3D_KEYS = ("p", "vx", "vy", "vz", "forward_wavefield_p", "forward_wavefield_v
               "forward_wavefield_vy", "forward_wavefield_vz")
 
 
-def fd_vp_c1(cells=16, step=1e-2, seed=16):
-    """Central finite differences of L = sum(g . seismograms) w.r.t. vp on the
-    config-1 problem (100 x 200, one shot, 1000 steps; the c1_grad fixture's
-    inputs) with the FP64 oracle: ``cells`` seeded cells, vp +- ``step`` m/s
-    (rounded to the float32 model values the operator reads, and divided by
-    the difference of those).  Returns (fixture, meta, adjoint sources (g_p,
-    g_vx, g_vz), cell rows, cell columns, finite differences)."""
+def fd_c1(name="vp", cells=16, step=1e-2, seed=16):
+    """Central finite differences of L = sum(g . seismograms) w.r.t. the model
+    parameter ``name`` ("vp" or "rho") on the config-1 problem (100 x 200, one
+    shot, 1000 steps; the c1_grad fixture's inputs) with the FP64 oracle:
+    ``cells`` seeded cells, parameter +- ``step`` (rounded to the float32 model
+    values the operator reads, and divided by the difference of those).
+    Returns (fixture, meta, adjoint sources (g_p, g_vx, g_vz), cell rows, cell
+    columns, finite differences)."""
     import oracle
     z, m = load("c1_grad")
     a = oracle_args(z, m)
     rng = np.random.default_rng(seed)
     R, nt = len(z["in_rcv_z"]), m["nt"]
     gs = [rng.standard_normal((1, nt, R)) for _ in range(3)]
-    nz, nx = a["vp"].shape
+    nz, nx = a[name].shape
     ii, jj = rng.integers(0, nz, cells), rng.integers(0, nx, cells)
 
-    def loss(vp):
-        out = oracle.forward(**dict(a, vp=vp), precision="f64")
+    def loss(model):
+        out = oracle.forward(**dict(a, **{name: model}), precision="f64")
         return sum(float(np.sum(out[k] * g)) for k, g in zip(("p", "vx", "vz"), gs))
 
     fd = np.empty(cells)
     for c, (i, j) in enumerate(zip(ii, jj)):
-        up, dn = a["vp"].copy(), a["vp"].copy()
-        up[i, j] = np.float32(a["vp"][i, j] + step)
-        dn[i, j] = np.float32(a["vp"][i, j] - step)
+        up, dn = a[name].copy(), a[name].copy()
+        up[i, j] = np.float32(a[name][i, j] + step)
+        dn[i, j] = np.float32(a[name][i, j] - step)
         fd[c] = (loss(up) - loss(dn)) / (float(up[i, j]) - float(dn[i, j]))
     return z, m, gs, ii, jj, fd
 

@@ -13,7 +13,7 @@ import oracle
 from geobrain_amd import _native
 from geobrain_amd.physics import wave
 from geobrain_amd.physics.wave import kernels as K
-from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len, fd_vp_c1,
+from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len, fd_c1,
                              load, oracle_args, rel_l2)
 
 pytestmark = pytest.mark.gpu
@@ -236,15 +236,16 @@ def test_dot_product_fp32():
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
-def test_gradient_matches_finite_differences_config1():
-    """The HIP FP32 vp gradient of the config-1 problem (100 x 200, 1000 steps)
-    against FP64 central differences of the oracle: vp +- 1e-2 m/s on 16 seeded
-    cells (the FP32 forward cannot resolve so small a step itself)."""
-    print("[case] config-1 gradient vs finite differences, 16 cells", flush=True)
-    z, m, gs, ii, jj, fd = fd_vp_c1()
-    out, ts = run(z, m, grad=("vp",))
+@pytest.mark.parametrize("name", ["vp", "rho"])
+def test_gradient_matches_finite_differences_config1(name):
+    """The HIP FP32 vp (rho) gradient of the config-1 problem (100 x 200, 1000
+    steps) against FP64 central differences of the oracle: +- 1e-2 m/s (kg/m^3)
+    on 16 seeded cells (the FP32 forward cannot resolve so small a step itself)."""
+    print(f"[case] config-1 {name} gradient vs finite differences, 16 cells", flush=True)
+    z, m, gs, ii, jj, fd = fd_c1(name)
+    out, ts = run(z, m, grad=(name,))
     sum((_t(g.astype(np.float32)) * out[k]).sum() for g, k in zip(gs, ("p", "vx", "vz"))).backward()
-    g = np_(ts["vp"].grad)
+    g = np_(ts[name].grad)
     err = np.abs(g[ii, jj] - fd) / np.abs(g).max()
     print(f"  max |HIP - FD| / max |grad| = {err.max():.2e}")
     assert err.max() <= 1e-5

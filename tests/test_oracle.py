@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len,
-                             fd_vp_c1, load, oracle_args, rel_l2)
+                             fd_c1, load, oracle_args, rel_l2)
 
 
 @pytest.mark.parametrize("name", FWD_CASES + GRAD_CASES)
@@ -115,16 +115,19 @@ def test_oracle_gradient_finite_differences(fs):
             assert abs(fd - grad[a, b]) <= 1e-5 * abs(grad).max(), (name, a, b, fd, grad[a, b])
 
 
-def test_oracle_gradient_finite_differences_config1():
+@pytest.mark.parametrize("name", ["vp", "rho"])
+def test_oracle_gradient_finite_differences_config1(name):
     """The config-1 problem (100 x 200, 1000 steps): FP64 central differences,
-    vp +- 1e-2 m/s on 16 seeded cells, against the FP64 oracle's gradient."""
-    z, m, gs, ii, jj, fd = fd_vp_c1()
-    gv, _, _ = oracle.gradient(**oracle_args(z, m), g_p=gs[0], g_vx=gs[1], g_vz=gs[2],
-                               chunk=chunk_len(m["nt"], m["segments"]), precision="f64")
-    scale = np.abs(gv).max()
+    vp +- 1e-2 m/s (rho +- 1e-2 kg/m^3) on 16 seeded cells, against the FP64
+    oracle's gradient."""
+    z, m, gs, ii, jj, fd = fd_c1(name)
+    gv, gr, _ = oracle.gradient(**oracle_args(z, m), g_p=gs[0], g_vx=gs[1], g_vz=gs[2],
+                                chunk=chunk_len(m["nt"], m["segments"]), precision="f64")
+    g = gv if name == "vp" else gr
+    scale = np.abs(g).max()
     assert scale > 0
-    err = np.abs(fd - gv[ii, jj]) / scale
-    assert err.max() <= 1e-8, (err.max(), list(zip(ii, jj, fd, gv[ii, jj])))
+    err = np.abs(fd - g[ii, jj]) / scale
+    assert err.max() <= 1e-8, (err.max(), list(zip(ii, jj, fd, g[ii, jj])))
 
 
 def test_oracle_linear_and_zero_wavelet():
