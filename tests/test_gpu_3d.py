@@ -160,6 +160,34 @@ def test_3d_linear_in_wavelet():
     assert np.abs(o1["p"]).max() > 0
 
 
+@pytest.mark.parametrize("case", ["nt1", "nt3_fs", "no_receivers"])
+def test_3d_edge_cases_vs_oracle(case):
+    """One and three time steps (the latter with a free surface) and no
+    receivers at all, against the FP32 oracle: seismograms and energy maps bit
+    for bit; with receivers, the vp, rho and wavelet gradients bit for bit too
+    (one shot group)."""
+    print(f"[case] 3d edge case {case}", flush=True)
+    nt, fs = {"nt1": (1, False), "nt3_fs": (3, True), "no_receivers": (12, True)}[case]
+    a = random_case(10, 8, 12, 3, nt, 2, fs, 81)
+    if case == "no_receivers":
+        for k in ("rcv_z", "rcv_y", "rcv_x"):
+            a[k] = np.zeros(0, np.int64)
+    want = oracle.forward3d(**a, chunk=nt, precision="f32")
+    got = run3(a)
+    for k in REC:
+        assert got[k].shape == want[k].shape, (k, got[k].shape, want[k].shape)
+    _check(got, want)
+    if case != "no_receivers":
+        R = len(a["rcv_z"])
+        rng = np.random.default_rng(82)
+        gs = [rng.standard_normal((2, nt, R)).astype(np.float32) for _ in range(4)]
+        _, g = grad3(a, gs)
+        ref = oracle.gradient3d(**a, g_p=gs[0], g_vx=gs[1], g_vy=gs[2], g_vz=gs[3], chunk=nt,
+                                precision="f32")
+        for key, i in (("vp", 0), ("rho", 1), ("w", 2)):
+            np.testing.assert_array_equal(g[key], ref[i], err_msg=key)
+
+
 def test_3d_index_errors():
     print("[case] 3d errors", flush=True)
     a = random_case(10, 9, 11, 3, 5, 1, False, 31)
