@@ -270,3 +270,16 @@ def test_product_path_3d_refuses_cpu_tensors():
             src_n=len(z["in_src_x"]), src_v=T(z["in_wavelet"]), rcv_x=T(z["in_rcv_x"]),
             rcv_y=T(z["in_rcv_y"]), rcv_z=T(z["in_rcv_z"]), rcv_n=len(z["in_rcv_x"]),
             damp=T(z["in_damp"]), vp=T(z["in_vp"]), rho=T(z["in_rho"]))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """This build runs on one GPU: bench.py --gpus 2 exits non-zero with a
+    message, before anything touches a GPU."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120, cwd=root)
+    assert res.returncode != 0
+    assert "one GPU" in (res.stdout + res.stderr)
+
