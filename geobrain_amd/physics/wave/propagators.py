@@ -61,7 +61,8 @@ class AcousticPropagator(AbstractPropagator):
     returns a ``WaveResult`` whose ``p``/``vx``/``vz`` are ``(n_shots, nt, n_rcv)``.
 
     ``checkpoint_dtype`` (attribute, default ``torch.float32``) selects the
-    storage of the gradient's forward checkpoints (``torch.bfloat16`` halves it).
+    storage of the gradient's forward checkpoints (``torch.bfloat16`` halves it;
+    2-D only: a 3-D model with another dtype raises ``NotImplementedError``).
     """
 
     checkpoint_dtype: torch.dtype = torch.float32
@@ -114,6 +115,8 @@ class AcousticPropagator(AbstractPropagator):
 
     def forward(self, model: Optional[AcousticModel] = None, shot_index: Optional[int] = None,
                 checkpoint_segments: int = 1, snapshot_every: int = 0) -> WaveResult:
+        if self.model.ndim == 3 and self.checkpoint_dtype != torch.float32:
+            raise NotImplementedError("3-D gradients store float32 checkpoints only")
         model = model if model is not None else self.model
         model.forward()
         if checkpoint_segments > 1 and not any(p.requires_grad for p in model.parameters()):

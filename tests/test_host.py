@@ -145,6 +145,25 @@ def test_acoustic_model_gardner_and_clip():
     assert not m.rho.requires_grad and m.vp.requires_grad
 
 
+def test_3d_propagator_refuses_bf16_checkpoints():
+    """BF16 checkpoint storage is a 2-D option: a 3-D model asking for it gets
+    NotImplementedError before anything runs (the model is left as it was)."""
+    nz, ny, nx, nt = 4, 5, 6, 10
+    vp = np.full((nz, ny, nx), 2000.0, np.float32)
+    grid = wave.GridConfig(nx=nx, nz=nz, dx=10.0, dz=10.0, ny=ny, dy=10.0)
+    model = wave.AcousticModel(grid, wave.BoundaryConfig(type="pml", n_layers=3), vp=vp,
+                               rho=np.ones_like(vp), vp_grad=True)
+    src = wave.Source(nt=nt, dt=1e-3, f0=20.0)
+    src.add_source(x=2, y=2, z=1, wavelet=np.ones(nt, np.float32))
+    rcv = wave.Receiver(nt=nt, dt=1e-3)
+    rcv.add_receivers(x=np.arange(3), y=np.ones(3, int), z=np.full(3, 2), rcv_type="pr")
+    prop = wave.AcousticPropagator(model, wave.Survey(src, rcv))
+    prop.checkpoint_dtype = torch.bfloat16
+    with pytest.raises(NotImplementedError, match="float32 checkpoints"):
+        prop.forward(checkpoint_segments=2)
+    np.testing.assert_array_equal(model.rho.detach().numpy(), np.ones_like(vp))
+
+
 def test_ricker_closed_form():
     w, t = wave.RickerWavelet()(f0=15.0, dt=5e-4)
     tt = t.numpy().astype(np.float64)
