@@ -94,8 +94,10 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     (the keyword arguments of ``acoustic_forward_kernel``; ``nt`` and ``src_v``
     are cut to ``nt_trial`` steps for the trials).  The forward call (with
     checkpoint stores) and the gradient call (recompute + adjoint, tiles or
-    z-marching recompute) are timed and chosen separately.  Pins the result
-    with ``kernels.set_launch_choice`` and returns it."""
+    z-marching recompute) are timed and chosen separately.  Registers the
+    result for this problem shape (``kernels.register_tuned``: later calls of
+    the same shape use it unless a choice is pinned with
+    ``kernels.set_launch_choice``) and returns it."""
     nz, nx, nabc = args["nz"], args["nx"], args["nabc"]
     src_v = args["src_v"]
     key = _key(nz, nx, nabc, args["src_n"], args["rcv_n"], args["free_surface"], src_v.dim() == 1)
@@ -103,9 +105,11 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
         if not _cache:
             _load_file()
         hit = _cache.get(key)
+    skey = K.shape_key(nz, nx, nabc, args["src_n"], args["rcv_n"], args["free_surface"],
+                       src_v.dim() == 1)
     if hit is not None and len(hit) == 6:
         choice = K.LaunchChoice(*hit)
-        K.set_launch_choice(choice)
+        K.register_tuned(skey, choice)
         return choice
     nt = min(nt_trial, args["nt"])
     trial = dict(args, nt=nt, src_v=src_v[..., :nt].contiguous(),
@@ -117,6 +121,25 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
     def log(msg):
         if verbose:
             print("  autotune " + msg, file=sys.stderr, flush=True)
+
+    # trials pin each candidate; whatever happens, the caller's pinned choice
+    # (if any) is restored afterwards
+    prev = K.get_launch_choice()
+    try:
+        best = _tune_trials(args, trial, vp, cands, log, repeats)
+    finally:
+        K.set_launch_choice(prev)
+    with _lock:
+        _cache[key] = best
+        _save_file()
+    choice = K.LaunchChoice(*best)
+    K.register_tuned(skey, choice)
+    return choice
+
+
+def _tune_trials(args, trial, vp, cands, log, repeats):
+    """Times the forward candidates, then the gradient candidates with the
+    fastest forward; returns the LaunchChoice fields of the winner."""
 
     def fwd(pre):
         return K.acoustic_forward_kernel(**trial)["p"]
@@ -152,13 +175,7 @@ def tune(args: dict, nt_trial: int = 60, candidates: Optional[Iterable[Tuple[int
             log(f"gradient tile_rows={tr} shots_per_block={spb} march={march}: {t:.2f} ms")
             if t < best_bt:
                 best_b, best_bt = (tr, spb, march), t
-    best = (best_f[0], best_f[1], best_b[2], best_b[0], best_b[1], best_f[2])
-    with _lock:
-        _cache[key] = best
-        _save_file()
-    choice = K.LaunchChoice(*best)
-    K.set_launch_choice(choice)
-    return choice
+    return (best_f[0], best_f[1], best_b[2], best_b[0], best_b[1], best_f[2])
 
 
 def batch_candidates() -> List[Tuple[int, int, int]]:

@@ -70,8 +70,9 @@ class BatchedForward:
             sx, sz = K._host_index(a["src_x"], "src_x"), K._host_index(a["src_z"], "src_z")
             rx, rz = K._host_index(a["rcv_x"], "rcv_x"), K._host_index(a["rcv_z"], "rcv_z")
             S, nt, R = int(a["src_n"]), int(a["nt"]), int(a["rcv_n"])
-            if src_v.dim() == 2 and tuple(src_v.shape) != (S, nt):
-                raise ValueError(f"problem {i}: src_v must be (src_n, nt) or (nt,)")
+            # every shape check acoustic_forward_kernel makes, before any pointer is taken
+            K.validate_problem(int(a["nx"]), int(a["nz"]), nt, int(a["nabc"]), S, R, sx, sz,
+                               rx, rz, vp, rho, damp, src_v, where=f"problem {i}: ")
             p = self._probs[i]
             p.nx, p.nz = int(a["nx"]), int(a["nz"])
             p.dx, p.dz = float(a["dx"]), float(a["dz"])

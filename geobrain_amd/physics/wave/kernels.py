@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 from dataclasses import dataclass, field
 from typing import Dict, Optional
 
@@ -75,12 +76,36 @@ class LaunchChoice:
 
 
 _launch_override: Optional[LaunchChoice] = None
+# choices the autotuner found, per problem shape (see ``shape_key``)
+_tuned: Dict[tuple, LaunchChoice] = {}
 
 
 def set_launch_choice(choice: Optional[LaunchChoice]) -> None:
-    """Pin the launch parameters for subsequent calls (``None`` = library default)."""
+    """Pin the launch parameters for subsequent calls (``None`` = library default).
+    A pinned choice applies to every problem shape and takes precedence over
+    the tuned choices registered with ``register_tuned``."""
     global _launch_override
     _launch_override = choice
+
+
+def get_launch_choice() -> Optional[LaunchChoice]:
+    """The pinned launch choice (``None`` when none is pinned)."""
+    return _launch_override
+
+
+def shape_key(nz: int, nx: int, nabc: int, src_n: int, rcv_n: int, free_surface: bool,
+              shared: bool) -> tuple:
+    """The problem shape a tuned launch choice is valid for."""
+    return (int(nz) + 2 * int(nabc), int(nx) + 2 * int(nabc), int(src_n), int(rcv_n),
+            bool(free_surface), bool(shared))
+
+
+def register_tuned(key: tuple, choice: Optional[LaunchChoice]) -> None:
+    """Use ``choice`` for calls whose ``shape_key`` is ``key`` (``None`` forgets it)."""
+    if choice is None:
+        _tuned.pop(key, None)
+    else:
+        _tuned[key] = choice
 
 
 @dataclass
@@ -194,15 +219,60 @@ class AcousticFunction(torch.autograd.Function):
         return grad_vp, grad_rho, grad_w, None, None
 
 
+# host copies of device position tensors, keyed by the tensor (weakly) and
+# checked against its version counter and storage, so a caller that passes the
+# same device positions every call (the propagator, an FWI loop) pays one
+# device-to-host copy, not one per call
+_index_cache: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
+
+
 def _host_index(t, name: str) -> np.ndarray:
     if torch.is_tensor(t):
-        a = t.detach().to("cpu").numpy()
+        if t.device.type != "cpu":
+            stamp = (t._version, t.data_ptr(), tuple(t.shape), tuple(t.stride()), t.dtype)
+            hit = _index_cache.get(t)
+            if hit is not None and hit[0] == stamp:
+                return hit[1]
+            a = t.detach().to("cpu").numpy()
+        else:
+            a = t.detach().numpy()
     else:
         a = np.asarray(t)
     if a.ndim != 1:
         raise NotImplementedError(
             f"{name}: only one point per shot (1-D positions) is supported; got shape {a.shape}")
-    return np.ascontiguousarray(a, dtype=np.int64)
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    if torch.is_tensor(t) and t.device.type != "cpu":
+        a.setflags(write=False)
+        _index_cache[t] = (stamp, a)
+    return a
+
+
+def validate_problem(nx, nz, nt, nabc, src_n, rcv_n, sx, sz, rx, rz, vp, rho, damp,
+                     src_v, where: str = "") -> int:
+    """Shape and length checks of one 2D problem, made on the host before any
+    device pointer is handed to the library (a short wavelet or an undersized
+    model would otherwise be read past its end on the device).  ``sx .. rz``
+    are the host int64 position arrays.  Returns ``wavelet_shared`` (1 for a
+    ``(nt,)`` wavelet, 0 for ``(src_n, nt)``).  Raises ``ValueError`` or
+    ``NotImplementedError``; ``where`` prefixes the message."""
+    if sx.shape != sz.shape or sx.shape[0] != src_n:
+        raise ValueError(f"{where}src_x/src_z must both have src_n={src_n} entries")
+    if rx.shape != rz.shape or rx.shape[0] != rcv_n:
+        raise ValueError(f"{where}rcv_x/rcv_z must both have rcv_n={rcv_n} entries")
+    if tuple(vp.shape) != (nz, nx) or tuple(rho.shape) != (nz, nx):
+        raise ValueError(f"{where}vp and rho must have shape ({nz}, {nx})")
+    if tuple(damp.shape) != (nz + 2 * nabc, nx + 2 * nabc):
+        raise ValueError(f"{where}damp must have shape ({nz + 2 * nabc}, {nx + 2 * nabc})")
+    if src_v.dim() == 1:
+        if src_v.shape[0] != nt:
+            raise ValueError(f"{where}src_v has {src_v.shape[0]} samples, nt={nt}")
+        return 1
+    if src_v.dim() == 2:
+        if tuple(src_v.shape) != (src_n, nt):
+            raise ValueError(f"{where}src_v must have shape ({src_n}, {nt}) or ({nt},)")
+        return 0
+    raise NotImplementedError(f"{where}src_v must be (src_n, nt) or (nt,)")
 
 
 def _on_gpu(t: Tensor, name: str, device: torch.device) -> Tensor:
@@ -266,24 +336,8 @@ def acoustic_forward_kernel(
         raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
     sx, sz = _host_index(src_x, "src_x"), _host_index(src_z, "src_z")
     rx, rz = _host_index(rcv_x, "rcv_x"), _host_index(rcv_z, "rcv_z")
-    if sx.shape != sz.shape or sx.shape[0] != src_n:
-        raise ValueError(f"src_x/src_z must both have src_n={src_n} entries")
-    if rx.shape != rz.shape or rx.shape[0] != rcv_n:
-        raise ValueError(f"rcv_x/rcv_z must both have rcv_n={rcv_n} entries")
-    if tuple(vp.shape) != (nz, nx) or tuple(rho.shape) != (nz, nx):
-        raise ValueError(f"vp and rho must have shape ({nz}, {nx})")
-    if tuple(damp.shape) != (nz + 2 * nabc, nx + 2 * nabc):
-        raise ValueError(f"damp must have shape ({nz + 2 * nabc}, {nx + 2 * nabc})")
-    if src_v.dim() == 1:
-        shared = 1
-        if src_v.shape[0] != nt:
-            raise ValueError(f"src_v has {src_v.shape[0]} samples, nt={nt}")
-    elif src_v.dim() == 2:
-        shared = 0
-        if tuple(src_v.shape) != (src_n, nt):
-            raise ValueError(f"src_v must have shape ({src_n}, {nt}) or ({nt},)")
-    else:
-        raise NotImplementedError("src_v must be (src_n, nt) or (nt,)")
+    shared = validate_problem(nx, nz, nt, nabc, src_n, rcv_n, sx, sz, rx, rz, vp, rho, damp,
+                              src_v)
 
     prob = _native.Problem()
     prob.nx, prob.nz = int(nx), int(nz)
@@ -306,6 +360,8 @@ def acoustic_forward_kernel(
         prob.checkpoint_every = auto_checkpoint_interval(nt)
     prob.checkpoint_bf16 = int(checkpoint_dtype == torch.bfloat16)
     choice = _launch_override
+    if choice is None and _tuned:
+        choice = _tuned.get(shape_key(nz, nx, nabc, src_n, rcv_n, free_surface, shared))
     prob.tile_rows = choice.tile_rows if choice else 0
     prob.shots_per_block = choice.shots_per_block if choice else 0
     prob.march = choice.march if choice else 0

@@ -18,12 +18,11 @@ from __future__ import annotations
 
 from typing import Optional
 
-import numpy as np
 import torch
 import torch.nn as nn
 
 from .boundary import create_boundary
-from .kernels import acoustic_forward_kernel, acoustic_forward_kernel_3d
+from .kernels import _host_index, acoustic_forward_kernel, acoustic_forward_kernel_3d
 from .models import AbstractModel, AcousticModel
 from .result import WaveResult
 from .survey import Survey
@@ -92,17 +91,18 @@ class AcousticPropagator(AbstractPropagator):
         self.src_z = numpy2tensor(src_loc[:, zc], torch.long).to(device)
         self.rcv_x = numpy2tensor(rcv_loc[:, 0], torch.long).to(device)
         self.rcv_z = numpy2tensor(rcv_loc[:, zc], torch.long).to(device)
-        # host copies of the geometry: the library reads positions on the host
-        self._src_x_h = np.asarray(src_loc[:, 0], np.int64)
-        self._src_z_h = np.asarray(src_loc[:, zc], np.int64)
-        self._rcv_x_h = np.asarray(rcv_loc[:, 0], np.int64)
-        self._rcv_z_h = np.asarray(rcv_loc[:, zc], np.int64)
         if model.ndim == 3:
             self.src_y = numpy2tensor(src_loc[:, 1], torch.long).to(device)
             self.rcv_y = numpy2tensor(rcv_loc[:, 1], torch.long).to(device)
-            self._src_y_h = np.asarray(src_loc[:, 1], np.int64)
-            self._rcv_y_h = np.asarray(rcv_loc[:, 1], np.int64)
         self.receiver_masks = survey.receiver_masks
+
+    def _positions(self, *names: str):
+        """Host int64 copies of the position attributes, read from the current
+        ``src_x`` / ``rcv_x`` / ... tensors on every call (a caller may move
+        shots between calls, as the reference's forward re-reads them); the
+        copies are cached per tensor and version, so an unchanged device
+        tensor is copied to the host once."""
+        return [_host_index(getattr(self, n), n) for n in names]
 
     def _init_boundary(self, vmax: Optional[float] = None) -> None:
         if vmax is None:
@@ -122,15 +122,17 @@ class AcousticPropagator(AbstractPropagator):
         if checkpoint_segments > 1 and not any(p.requires_grad for p in model.parameters()):
             checkpoint_segments = 1
         sl = slice(shot_index, shot_index + 1) if shot_index is not None else slice(None)
-        src_x, src_z = self._src_x_h[sl], self._src_z_h[sl]
+        sx, sz, rx, rz = self._positions("src_x", "src_z", "rcv_x", "rcv_z")
+        src_x, src_z = sx[sl], sz[sl]
         wavelet = self.wavelet[sl]
         src_n = 1 if shot_index is not None else self.src_n
         if self.model.ndim == 3:
+            sy, ry = self._positions("src_y", "rcv_y")
             result = acoustic_forward_kernel_3d(
                 nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz,
                 nt=self.nt, dt=self.dt, nabc=self.nabc, free_surface=self.free_surface,
-                src_x=src_x, src_y=self._src_y_h[sl], src_z=src_z, src_n=src_n,
-                src_v=wavelet, rcv_x=self._rcv_x_h, rcv_y=self._rcv_y_h, rcv_z=self._rcv_z_h,
+                src_x=src_x, src_y=sy[sl], src_z=src_z, src_n=src_n,
+                src_v=wavelet, rcv_x=rx, rcv_y=ry, rcv_z=rz,
                 rcv_n=self.rcv_n, damp=self.damp, vp=model.vp, rho=model.rho,
                 checkpoint_segments=checkpoint_segments, device=torch.device(self.device),
                 dtype=self.dtype)
@@ -139,7 +141,7 @@ class AcousticPropagator(AbstractPropagator):
             nx=self.nx, nz=self.nz, dx=self.dx, dz=self.dz, nt=self.nt, dt=self.dt,
             nabc=self.nabc, free_surface=self.free_surface,
             src_x=src_x, src_z=src_z, src_n=src_n, src_v=wavelet,
-            rcv_x=self._rcv_x_h, rcv_z=self._rcv_z_h, rcv_n=self.rcv_n,
+            rcv_x=rx, rcv_z=rz, rcv_n=self.rcv_n,
             damp=self.damp, vp=model.vp, rho=model.rho,
             checkpoint_segments=checkpoint_segments, device=torch.device(self.device),
             dtype=self.dtype, snapshot_every=snapshot_every,

@@ -47,9 +47,12 @@ def build(force: bool = False) -> str:
 def _load():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        # GBW_ORACLE_LIB selects another build of the same sources (the
+        # sanitizer build of ``make asan``)
+        path = os.environ.get("GBW_ORACLE_LIB") or _LIB_PATH
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        lib = ctypes.CDLL(_LIB_PATH)
+        lib = ctypes.CDLL(path)
         P = ctypes.c_void_p
         I = ctypes.c_int
         D = ctypes.c_double

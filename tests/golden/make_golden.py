@@ -119,6 +119,48 @@ def run_case_3d(name, *, nz, ny, nx, nabc, fs, dx, dt, nt, vp, rho, src_z, src_y
     print(f"{name}: done in {time.time() - t0:.1f}s")
 
 
+def _load_wavelets():
+    """``wavelets/ricker.py`` imports its base class from its sibling
+    (``from .base import WaveletGenerator``): load both as members of a bare
+    package whose path is the reference's wavelets directory (the package's
+    own ``__init__`` is not run)."""
+    import types
+    pkg = types.ModuleType("_gb_ref_wavelets")
+    pkg.__path__ = [os.path.join(REF, "geobrain/physics/wave/wavelets")]
+    sys.modules["_gb_ref_wavelets"] = pkg
+    for leaf in ("base", "ricker"):
+        spec = importlib.util.spec_from_file_location(
+            "_gb_ref_wavelets." + leaf,
+            os.path.join(REF, "geobrain/physics/wave/wavelets", leaf + ".py"))
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[spec.name] = mod
+        spec.loader.exec_module(mod)
+    return sys.modules["_gb_ref_wavelets.ricker"]
+
+
+# (f0, dt, length_factor): examples 07 and 09 (5 Hz, 3 ms), the configs'
+# wavelets (10, 15, 20 Hz), odd sampling, a longer wavelet
+RICKER_CASES = [(5.0, 3e-3, 2.2), (10.0, 1e-3, 2.2), (15.0, 5e-4, 2.2), (20.0, 8e-4, 2.2),
+                (8.0, 5e-4, 2.2), (12.5, 7e-4, 2.2), (25.0, 1e-3, 3.0)]
+
+
+def ricker_case():
+    """FP32 ``RickerWavelet(length_factor)(f0, dt)`` outputs (wavelet and time
+    axis) of the reference for ``RICKER_CASES``."""
+    rk = _load_wavelets()
+    out = {}
+    for i, (f0, dt, lf) in enumerate(RICKER_CASES):
+        w, t = rk.RickerWavelet(length_factor=lf)(f0=f0, dt=dt)
+        out[f"args{i}"] = np.array([f0, dt, lf])
+        out[f"wavelet{i}"] = w.detach().numpy()
+        out[f"t{i}"] = t.detach().numpy()
+    out["meta"] = np.array(json.dumps(dict(name="ricker", n=len(RICKER_CASES),
+                                           torch=torch.__version__,
+                                           source="geobrain/physics/wave/wavelets/ricker.py:55-95")))
+    np.savez_compressed(os.path.join(HERE, "ricker.npz"), **out)
+    print("ricker: done")
+
+
 def cases_3d():
     rng = np.random.default_rng(4321)
     cases = []
@@ -374,6 +416,8 @@ def main():
         if args.only and name not in args.only:
             continue
         run_case_3d(name, **kw)
+    if not args.only or "ricker" in args.only:
+        ricker_case()
     if not args.only or "marmousi" in args.only:
         marmousi_case()
     if not args.only or "pml" in args.only:

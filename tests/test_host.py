@@ -3,6 +3,7 @@ declared symbol, host-side mirrors of the reference's types behave as the
 reference's do, and the product path refuses to run without a GPU."""
 
 import ctypes
+import json
 import os
 import re
 
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle
 from geobrain_amd import _native, synthetic
 from geobrain_amd.physics import wave
 from tests._fixtures import load
@@ -166,10 +168,29 @@ def test_3d_propagator_refuses_bf16_checkpoints():
 
 def test_ricker_closed_form():
     w, t = wave.RickerWavelet()(f0=15.0, dt=5e-4)
-    tt = t.numpy().astype(np.float64)
-    ref = (1 - 2 * (np.pi * 15.0 * tt) ** 2) * np.exp(-(np.pi * 15.0 * tt) ** 2)
-    np.testing.assert_allclose(w.numpy(), ref, atol=2e-6)
+    np.testing.assert_allclose(w.numpy(), oracle.ricker(15.0, 5e-4), rtol=0, atol=2e-6)
     assert len(w) % 2 == 1 and float(w[len(w) // 2]) == 1.0
+
+
+def test_ricker_bitwise_reference():
+    """RickerWavelet (wavelet and time axis) equals the reference's FP32
+    outputs bit for bit (tests/golden/ricker.npz, from wavelets/ricker.py:55-95)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ricker.npz"))
+    for i in range(json.loads(str(z["meta"]))["n"]):
+        f0, dt, lf = (float(v) for v in z[f"args{i}"])
+        w, t = wave.RickerWavelet(length_factor=lf)(f0=f0, dt=dt)
+        np.testing.assert_array_equal(w.numpy(), z[f"wavelet{i}"])
+        np.testing.assert_array_equal(t.numpy(), z[f"t{i}"])
+
+
+def test_gardner_matches_oracle():
+    vp = np.random.default_rng(2).uniform(1500, 4700, (13, 17)).astype(np.float32)
+    np.testing.assert_array_equal(synthetic.gardner(vp), oracle.gardner(vp))
+    grid = wave.GridConfig(nx=17, nz=13, dx=5.0, dz=5.0)
+    m = wave.AcousticModel(grid=grid, boundary=wave.BoundaryConfig(type="pml", n_layers=4),
+                           vp=vp, rho=np.ones_like(vp), auto_update_rho=True)
+    m.forward()
+    np.testing.assert_array_equal(m.rho.detach().numpy(), oracle.gardner(vp))
 
 
 def test_synthetic_generators_are_seeded():
@@ -235,10 +256,9 @@ def test_pad_model_3d_tensor_form():
     v = torch.arange(24, dtype=torch.float32).reshape(2, 3, 4)
     p = wave.pad_model_3d(v, 2, 2, 3, 4)
     assert p.shape == (6, 7, 8)
-    i = np.clip(np.arange(6) - 2, 0, 1)
-    j = np.clip(np.arange(7) - 2, 0, 2)
-    k = np.clip(np.arange(8) - 2, 0, 3)
-    np.testing.assert_array_equal(p.numpy(), v.numpy()[np.ix_(i, j, k)])
+    np.testing.assert_array_equal(p.numpy(), oracle.pad_model_3d(v.numpy(), 2))
+    v2 = torch.arange(35, dtype=torch.float32).reshape(5, 7)
+    np.testing.assert_array_equal(wave.pad_model(v2, 3, 5, 7).numpy(), oracle.pad_model(v2.numpy(), 3))
 
 
 def _problem3(**kw):
@@ -302,3 +322,42 @@ def test_bench_refuses_more_than_one_gpu():
     assert res.returncode != 0
     assert "one GPU" in (res.stdout + res.stderr)
 
+
+
+def test_validate_problem_rejects_bad_shapes():
+    """The checks acoustic_forward_kernel and BatchedForward both make on the
+    host before any device pointer reaches the library: each wrong size raises
+    ValueError (a short wavelet or an undersized model would otherwise be read
+    past its end on the device)."""
+    from geobrain_amd.physics.wave import kernels as K
+    nz, nx, nabc, nt, S, R = 6, 9, 3, 20, 2, 4
+    i64 = lambda n: np.zeros(n, np.int64)  # noqa: E731
+    good = dict(nx=nx, nz=nz, nt=nt, nabc=nabc, src_n=S, rcv_n=R, sx=i64(S), sz=i64(S),
+                rx=i64(R), rz=i64(R), vp=torch.zeros(nz, nx), rho=torch.zeros(nz, nx),
+                damp=torch.zeros(nz + 2 * nabc, nx + 2 * nabc), src_v=torch.zeros(S, nt))
+    assert K.validate_problem(**good) == 0
+    assert K.validate_problem(**dict(good, src_v=torch.zeros(nt))) == 1
+    bad = [dict(src_v=torch.zeros(nt - 1)), dict(src_v=torch.zeros(S, nt - 1)),
+           dict(src_v=torch.zeros(S + 1, nt)), dict(vp=torch.zeros(nz - 1, nx)),
+           dict(rho=torch.zeros(nz, nx + 1)), dict(damp=torch.zeros(nz, nx)),
+           dict(sx=i64(S - 1)), dict(sz=i64(S + 1)), dict(rx=i64(R - 1)), dict(rz=i64(R + 2))]
+    for b in bad:
+        with pytest.raises(ValueError, match="problem 3: "):
+            K.validate_problem(**dict(good, **b), where="problem 3: ")
+    with pytest.raises(NotImplementedError):
+        K.validate_problem(**dict(good, src_v=torch.zeros(1, S, nt)))
+    with pytest.raises(NotImplementedError):
+        K.validate_problem(**dict(good, src_v=torch.zeros(())))
+
+
+def test_host_index_cache_follows_tensor_changes():
+    """Positions given as tensors are read fresh when the tensor changes: the
+    cached host copy is keyed by the tensor's version (CPU tensors are read
+    directly, device tensors through the cache)."""
+    from geobrain_amd.physics.wave import kernels as K
+    t = torch.tensor([1, 2, 3])
+    np.testing.assert_array_equal(K._host_index(t, "src_x"), [1, 2, 3])
+    t[0] = 7
+    np.testing.assert_array_equal(K._host_index(t, "src_x"), [7, 2, 3])
+    with pytest.raises(NotImplementedError, match="one point per shot"):
+        K._host_index(torch.zeros(2, 2, dtype=torch.long), "src_x")

@@ -3,6 +3,8 @@ what the mathematics fixes independently of the reference (dot-product test,
 finite differences in FP64, linearity, translation and mirror symmetry,
 the energy-map / seismogram identity).  CPU only."""
 
+import json
+
 import numpy as np
 import pytest
 
@@ -216,12 +218,62 @@ def test_oracle_pml_closed_form():
     np.testing.assert_array_equal(d, d[:, ::-1])             # left/right symmetric
 
 
-def test_oracle_pad_model_and_ricker():
-    v = np.arange(12, dtype=np.float32).reshape(3, 4)
-    p = oracle.pad_model(v, 2)
-    assert p.shape == (7, 8)
-    assert p[0, 0] == v[0, 0] and p[-1, -1] == v[-1, -1] and p[3, 0] == v[1, 0]
-    np.testing.assert_array_equal(p[2:5, 2:6], v)
-    w = oracle.ricker(10.0, 1e-3)
+def test_oracle_pad_model_is_edge_padding():
+    """pad_model / pad_model_3d against numpy's edge-mode padding, a library
+    routine that replicates edges and corners as the reference's chained
+    copies do (acoustic_2d.py:18-66, :69-120), on shapes with every side
+    shorter and longer than the pad width."""
+    rng = np.random.default_rng(3)
+    for shape, n in (((3, 4), 2), ((1, 5), 3), ((6, 2), 4), ((7, 9), 1)):
+        v = rng.standard_normal(shape).astype(np.float32)
+        np.testing.assert_array_equal(oracle.pad_model(v, n), np.pad(v, n, mode="edge"))
+    for shape, n in (((2, 3, 4), 2), ((1, 1, 3), 3), ((5, 4, 2), 1)):
+        v = rng.standard_normal(shape).astype(np.float32)
+        np.testing.assert_array_equal(oracle.pad_model_3d(v, n), np.pad(v, n, mode="edge"))
+    # an x <-> z transposed restatement would fail this non-square case
+    v = np.arange(6, dtype=np.float32).reshape(2, 3)
+    p = oracle.pad_model(v, 1)
+    assert p.shape == (4, 5) and p[0, 4] == 2 and p[3, 0] == 3
+
+
+def test_oracle_ricker_known_values():
+    """Closed-form facts of (1 - 2a^2) exp(-a^2), a = pi f0 t
+    (wavelets/ricker.py:82-92): length 2 int(2.2/(f0 dt)/2) + 1, peak 1 at
+    t = 0, zeros at t = +-1/(pi f0 sqrt 2), value -1/e at t = +-1/(pi f0).
+    A dropped pi, a (1 - a^2) factor or exp(-2a^2) each fails one of them."""
+    f0 = 10.0
+    w = oracle.ricker(f0, 1e-3)
     assert len(w) == 221 and w[110] == 1.0
-    np.testing.assert_allclose(w, w[::-1])
+    np.testing.assert_allclose(w, w[::-1], rtol=0, atol=0)
+    t_zero = 1.0 / (np.pi * f0 * np.sqrt(2.0))
+    w = oracle.ricker(f0, t_zero / 5)           # the zeros fall on samples +-5
+    c = len(w) // 2
+    assert abs(w[c + 5]) < 1e-14 and abs(w[c - 5]) < 1e-14
+    assert w[c + 4] > 0 and w[c + 6] < 0
+    w = oracle.ricker(f0, 1.0 / (np.pi * f0) / 4)   # a = 1 at samples +-4
+    c = len(w) // 2
+    np.testing.assert_allclose([w[c + 4], w[c - 4]], -np.exp(-1.0), rtol=1e-14)
+    assert len(oracle.ricker(25.0, 1e-3, length_factor=3.0)) == 2 * int(120 / 2) + 1
+
+
+def test_oracle_ricker_matches_reference_fp32():
+    """The reference's own FP32 RickerWavelet outputs (tests/golden/ricker.npz,
+    written by make_golden.py from wavelets/ricker.py): the float64 oracle is
+    within FP32 rounding of every sample and has the same length."""
+    z = np.load("tests/golden/ricker.npz")
+    for i in range(json.loads(str(z["meta"]))["n"]):
+        f0, dt, lf = z[f"args{i}"]
+        w = oracle.ricker(f0, dt, length_factor=lf)
+        assert w.shape == z[f"wavelet{i}"].shape
+        np.testing.assert_allclose(w, z[f"wavelet{i}"], rtol=0, atol=2e-6)
+
+
+def test_oracle_gardner_known_values():
+    """310 vp^0.25 (models/acoustic.py:169) at fourth powers: 7^4 = 2401 m/s ->
+    2170, 6^4 = 1296 -> 1860, 8^4 = 4096 -> 2480, 10^4 -> 3100 (exact in
+    float32); a wrong exponent or constant fails."""
+    got = oracle.gardner(np.array([2401.0, 1296.0, 4096.0, 10000.0], np.float32))
+    assert got.dtype == np.float32
+    np.testing.assert_array_equal(got, np.array([2170.0, 1860.0, 2480.0, 3100.0], np.float32))
+    vp = np.linspace(1500, 4700, 33, dtype=np.float32)
+    assert np.all(np.diff(oracle.gardner(vp)) > 0)
