@@ -223,16 +223,19 @@ class AcousticFunction(torch.autograd.Function):
 # checked against its version counter and storage, so a caller that passes the
 # same device positions every call (the propagator, an FWI loop) pays one
 # device-to-host copy, not one per call
-_index_cache: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
+# (tensors compare elementwise, so the cache is keyed by id() and each entry
+# holds a weak reference that both confirms identity and drops the entry when
+# the tensor is freed)
+_index_cache: Dict[int, tuple] = {}
 
 
 def _host_index(t, name: str) -> np.ndarray:
     if torch.is_tensor(t):
         if t.device.type != "cpu":
             stamp = (t._version, t.data_ptr(), tuple(t.shape), tuple(t.stride()), t.dtype)
-            hit = _index_cache.get(t)
-            if hit is not None and hit[0] == stamp:
-                return hit[1]
+            hit = _index_cache.get(id(t))
+            if hit is not None and hit[0]() is t and hit[1] == stamp:
+                return hit[2]
             a = t.detach().to("cpu").numpy()
         else:
             a = t.detach().numpy()
@@ -244,7 +247,8 @@ def _host_index(t, name: str) -> np.ndarray:
     a = np.ascontiguousarray(a, dtype=np.int64)
     if torch.is_tensor(t) and t.device.type != "cpu":
         a.setflags(write=False)
-        _index_cache[t] = (stamp, a)
+        key = id(t)
+        _index_cache[key] = (weakref.ref(t, lambda _r, k=key: _index_cache.pop(k, None)), stamp, a)
     return a
 
 

@@ -96,3 +96,24 @@ def test_batch_invalid_geometry_raises():
                     rcv_z=np.array([0, 0], np.int64), rcv_n=2)
     with pytest.raises(IndexError):
         B.BatchedForward(probs)
+
+
+@pytest.mark.parametrize("bad", ["short_wavelet_1d", "short_wavelet_2d", "small_vp", "small_rho",
+                                 "small_damp", "short_src_x", "short_rcv_z"])
+def test_batch_invalid_shapes_raise_before_launch(bad):
+    """Each wrong size raises ValueError in BatchedForward's constructor, before
+    the library sees a pointer (the checks acoustic_forward_kernel makes)."""
+    probs = _problems()
+    a = dict(probs[1])
+    T = lambda x: torch.as_tensor(x, device=DEV)  # noqa: E731
+    nz, nx, nt, S = a["nz"], a["nx"], a["nt"], a["src_n"]
+    a.update({"short_wavelet_1d": dict(src_v=T(np.zeros(nt - 1, np.float32))),
+              "short_wavelet_2d": dict(src_v=T(np.zeros((S, nt - 3), np.float32))),
+              "small_vp": dict(vp=T(np.ones((nz - 1, nx), np.float32))),
+              "small_rho": dict(rho=T(np.ones((nz, nx - 2), np.float32))),
+              "small_damp": dict(damp=T(np.zeros((nz, nx), np.float32))),
+              "short_src_x": dict(src_x=a["src_x"][:0]),
+              "short_rcv_z": dict(rcv_z=a["rcv_z"][:-1])}[bad])
+    probs[1] = a
+    with pytest.raises(ValueError, match="problem 1: "):
+        B.BatchedForward(probs)

@@ -221,6 +221,77 @@ def test_propagator_end_to_end(fs):
     np.testing.assert_array_equal(np_(snap.p)[0], np_(res.p)[1])
 
 
+def test_propagator_reads_moved_positions():
+    """Moving the shots by reassigning ``prop.src_x`` (device tensor, as the
+    reference's propagator holds it) changes the next forward: the result
+    equals a propagator built with the new positions; moving them back
+    restores the first result bit for bit."""
+    print("[case] propagator with reassigned source positions", flush=True)
+    nz, nx, nt, dx, dt = 30, 80, 300, 10.0, 1e-3
+    vp = np.asarray(np.linspace(1500, 2500, nz)[:, None] * np.ones((1, nx)), np.float32)
+    grid = wave.GridConfig(nx=nx, nz=nz, dx=dx, dz=dx)
+    bnd = wave.BoundaryConfig(type="pml", n_layers=10, free_surface=False)
+    w, _ = wave.RickerWavelet()(f0=15.0, dt=dt)
+    w = np.pad(w.numpy(), (0, nt - len(w)))
+
+    def make(xs):
+        model = wave.AcousticModel(grid, bnd, vp=vp, rho=np.ones_like(vp), device="cuda")
+        src = wave.Source(nt=nt, dt=dt, f0=15.0)
+        for x in xs:
+            src.add_source(x=x, z=2, wavelet=w)
+        rcv = wave.Receiver(nt=nt, dt=dt)
+        rcv.add_receivers(x=np.arange(nx), z=np.full(nx, 2), rcv_type="pr")
+        return wave.AcousticPropagator(model, wave.Survey(src, rcv), device="cuda")
+
+    prop = make([10, 40])
+    first = np_(prop.forward().p)
+    prop.src_x = torch.tensor([25, 70], device="cuda")
+    moved = np_(prop.forward().p)
+    np.testing.assert_array_equal(moved, np_(make([25, 70]).forward().p))
+    assert not np.array_equal(moved, first)
+    prop.src_x[:] = torch.tensor([10, 40], device="cuda")  # in place: a new version
+    np.testing.assert_array_equal(np_(prop.forward().p), first)
+
+
+@pytest.mark.parametrize("where", ["interior", "absorbing"])
+def test_nan_velocity_propagates_as_the_oracle(where):
+    """A NaN in vp reaches the seismograms and energy maps exactly where the
+    reference's arithmetic puts it: the HIP outputs equal the FP32 oracle's
+    bit for bit, NaN positions included (the reference itself does not check
+    its inputs for NaN)."""
+    print(f"[case] NaN in vp ({where})", flush=True)
+    z, m = load("c1_fwd")
+    vp = z["in_vp"].copy()
+    if where == "interior":
+        vp[50, 120] = np.nan
+    else:
+        vp[0, 0] = np.nan  # edge cell: pad_model copies it into the absorbing corner
+    out, _ = run(z, m, vp=vp)
+    ref = oracle.forward(**dict(oracle_args(z, m), vp=vp), chunk=chunk_len(m["nt"], m["segments"]),
+                         precision="f32")
+    for k in ("p", "vx", "vz", "forward_wavefield_p", "forward_wavefield_vx",
+              "forward_wavefield_vz"):
+        got = np_(out[k])
+        nan = int(np.isnan(got).sum())
+        np.testing.assert_array_equal(got, ref[k], err_msg=k)
+        print(f"  {k}: {nan} NaN of {got.size}, bit-equal to the oracle")
+    assert np.isnan(np_(out["p"])).any()
+
+
+def test_zero_wavelet_gives_exact_zeros():
+    """A zero wavelet leaves every output exactly zero, and the gradient of any
+    seismogram loss is exactly zero too (every term carries a zero field)."""
+    print("[case] zero wavelet", flush=True)
+    z, m = load("tiny1_grad")
+    out, ts = run(z, m, grad=("vp", "rho"), wavelet=np.zeros_like(z["in_wavelet"]))
+    for k in ("p", "vx", "vz", "forward_wavefield_p", "forward_wavefield_vx",
+              "forward_wavefield_vz"):
+        assert not np_(out[k]).any(), k
+    g = adjoint_sources(z, m)
+    sum((_t(gk) * out[k]).sum() for gk, k in zip(g, ("p", "vx", "vz"))).backward()
+    assert not np_(ts["vp"].grad).any() and not np_(ts["rho"].grad).any()
+
+
 def test_dot_product_fp32():
     """<F w, g> vs <w, F^T g> through the HIP forward and adjoint."""
     print("[case] dot product", flush=True)
