@@ -13,9 +13,11 @@ Workload (one "step" = one full gradient evaluation):
 Metric: Gcell-updates/s = shots * padded cells * nt / step time.
 
 Also reported (one JSON line): the roofline of the dominant kernel measured
-live with events on the launch stream, the job-level HBM roofline fraction at
-the algorithmic 80 + 24/k bytes per cell-update, and the CPU oracle (FP32, the
-host's cores) timed on a bounded sample of the same workload.
+live with events on the launch stream (in one instrumented step after the
+timed ones), the job-level HBM roofline fraction at the algorithmic 80 + 24/k
+bytes per cell-update, the reference CPU path (the PyTorch restatement of the
+reference operator with autograd, on the host's cores) timed on a bounded
+sample of the same workload, and beside it the FP32 C++ oracle.
 """
 
 from __future__ import annotations
@@ -131,8 +133,96 @@ def eager_forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z
     return out
 
 
-def cpu_baseline(w, steps=400, shots=1):
-    """FP32 oracle gradient on `shots` shot(s) x `steps` time steps of the same grid."""
+def host_cpu_info():
+    """Host CPU facts for the baseline record: model name, logical CPUs, physical
+    cores (distinct (package, core) pairs in /proc/cpuinfo) and the CPUs this
+    process may run on (its affinity mask, which a container's CPU share sets)."""
+    model, cores = None, set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "physical_cores": len(cores) or None, "allowed_cpus": allowed}
+
+
+def cpu_threads(info):
+    """Threads for the CPU baselines: one per physical core this process may
+    use, capped by OMP_NUM_THREADS when the environment sets it."""
+    n = min(v for v in (info["physical_cores"], info["allowed_cpus"]) if v)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(w, shots=8, steps=100, k=50):
+    """The reference CPU path: ``eager_forward`` (the PyTorch restatement of
+    acoustic_2d.py:123-501, one ATen op per slice operation) on the host CPU
+    with autograd through reentrant checkpoint chunks of k steps, as
+    acoustic_2d.py:448-460 runs it, for a bounded sample of the benchmark
+    workload (the first `shots` shots x the first `steps` steps); the rate is
+    Gcell-updates/s of forward + gradient, which the sample measures directly
+    (the whole 64-shot x 4000-step job at that rate is stated beside it)."""
+    import torch
+    info = host_cpu_info()
+    n = cpu_threads(info)
+    prev = torch.get_num_threads()
+    torch.set_num_threads(n)
+    try:
+        T = lambda a, **kw: torch.as_tensor(a, **kw)  # noqa: E731
+        d_obs = None
+        args = dict(nz=w["nz"], nx=w["nx"], nabc=w["nabc"], free_surface=False, dt=w["dt"],
+                    dz=w["dx"], src_z=T(w["src_z"][:shots]), src_x=T(w["src_x"][:shots]),
+                    src_v=T(w["wavelet"][:shots, :steps]), rcv_z=T(w["rcv_z"]),
+                    rcv_x=T(w["rcv_x"]), damp=T(w["damp"]), rho=T(w["rho"]),
+                    checkpoint_segments=max(1, steps // k))
+        with torch.no_grad():
+            d_obs = eager_forward(vp=T(w["vp_obs"]), **args)["p"]
+        vp = T(w["vp"]).requires_grad_(True)
+        t0 = time.perf_counter()
+        out = eager_forward(vp=vp, **args)
+        ((out["p"] - d_obs) ** 2).sum().backward()
+        t = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(prev)
+    NZ, NX = w["nz"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]
+    rate = shots * NZ * NX * steps / t
+    full = w["wavelet"].shape[0] * NZ * NX * w["wavelet"].shape[1]
+    return {"value": round(rate / 1e9, 5), "unit": "Gcell-updates/s", "cores": n,
+            "kind": "reference",
+            "sample": f"PyTorch eager restatement of the reference operator (acoustic_2d.py) on "
+                      f"the host CPU, forward + autograd gradient through checkpoint chunks of "
+                      f"{k} steps, first {shots} of {w['wavelet'].shape[0]} shots x first "
+                      f"{steps} of {w['wavelet'].shape[1]} steps of the benchmark workload, "
+                      f"{t:.1f} s wall; the full job would take {full / rate / 3600:.1f} h "
+                      f"at this rate (extrapolated)",
+            "cpu": info, "torch_threads": n}
+
+
+def cpu_oracle_baseline(w, steps=2500, shots=1):
+    """The FP32 C++ oracle (OpenMP) gradient on `shots` shot(s) x `steps` time
+    steps of the same grid: the build's own CPU port, reported beside the
+    reference path, never as it."""
     import oracle
     oracle.build()
     sl = slice(0, shots)
@@ -321,7 +411,7 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
                      "bytes_per_time_step": b_step},
         "kernels": kernels,
         "check": {"p_max": float(out["p"].abs().max())},
-        "cpu_baseline": None,
+        "cpu_oracle": None,
     }
     if grad:
         # gradient of a seismogram misfit w.r.t. vp: forward with checkpoints,
@@ -362,7 +452,7 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
                          rho=w["rho"], damp=w["damp"], precision="f32")
         t = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rec["cpu_baseline"] = {
+        rec["cpu_oracle"] = {
             "value": round(pad[0] * pad[1] * pad[2] * cpu_steps / t / 1e9, 5),
             "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"FP32 C++ 3D oracle (OpenMP), forward, 1 shot x {cpu_steps} steps of "
@@ -463,8 +553,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2500)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baselines (reference path and C++ oracle)")
+    ap.add_argument("--cpu-shots", type=int, default=8,
+                    help="shots in the reference-path CPU baseline's sample")
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="time steps in the reference-path CPU baseline's sample")
+    ap.add_argument("--oracle-steps", type=int, default=2500,
+                    help="time steps of the C++ oracle timing (one shot)")
     ap.add_argument("--no-tune", action="store_true", help="library default launch choice")
     ap.add_argument("--no-eager", action="store_true", help="skip the per-shot PyTorch GPU baseline")
     ap.add_argument("--launch", type=str, default=None,
@@ -560,19 +656,26 @@ def main():
     torch.cuda.synchronize()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     torch.cuda.synchronize()
-    # the library brackets each time-step kernel launch with HIP events on the
-    # stream it launches on (torch's current stream) during the timed steps
-    from geobrain_amd import _native
-    _native.set_kernel_timing(True)
+    # timed steps: no instrumentation (the events above are four per step, on
+    # the launch stream, outside the library)
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(evs[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ktimes = _native.kernel_times()
-    _native.set_kernel_timing(False)
     t_fwd = np.mean([e[0].elapsed_time(e[1]) for e in evs]) / 1e3
     t_bwd = np.mean([e[2].elapsed_time(e[3]) for e in evs]) / 1e3
+    # one more step, instrumented: the library brackets each time-step kernel
+    # launch with HIP events on the stream it launches on (torch's current
+    # stream); per-kernel times and the roofline come from it
+    from geobrain_amd import _native
+    _native.set_kernel_timing(True)
+    ti = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    wall_instr = time.perf_counter() - ti
+    ktimes = _native.kernel_times()
+    _native.set_kernel_timing(False)
     cells = S * NZ * NX * nt
     step_s = wall / args.steps
     value = cells / step_s / 1e9
@@ -627,6 +730,7 @@ def main():
                      "avg_launch_us": kernels[dom]["avg_launch_us"],
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"]},
         "kernels": kernels,
+        "instrumented_step_s": round(wall_instr, 4),
         "phases": {"forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4),
                    "forward_gcells": round(cells / t_fwd / 1e9, 3),
                    "backward_gcells": round(cells / t_bwd / 1e9, 3),
@@ -639,7 +743,9 @@ def main():
         rec["per_shot_gpu"] = base
         rec["speedup_vs_per_shot_gpu"] = round(value / base["value"], 1)
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(w, steps=args.cpu_steps)
+        rec["cpu_baseline"] = cpu_baseline(w, shots=args.cpu_shots, steps=args.cpu_steps)
+        rec["cpu_oracle"] = cpu_oracle_baseline(w, steps=args.oracle_steps)
+        rec["speedup_vs_cpu_baseline"] = round(value / rec["cpu_baseline"]["value"], 1)
     if not args.no_uneven:
         del d_obs, vp, common
         torch.cuda.empty_cache()
