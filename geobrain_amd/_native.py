@@ -19,6 +19,8 @@ c_f32 = ctypes.c_float
 c_vp = ctypes.c_void_p
 
 ABI_VERSION = 3
+# size of the device array behind gbw_acoustic2d_forward_noalloc (GBW_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 EXPORTS = (
     "gbw_abi_version",
@@ -26,6 +28,7 @@ EXPORTS = (
     "gbw_acoustic2d_workspace_bytes",
     "gbw_acoustic2d_checkpoint_bytes",
     "gbw_acoustic2d_forward",
+    "gbw_acoustic2d_forward_noalloc",
     "gbw_acoustic2d_backward",
     "gbw_set_kernel_timing",
     "gbw_kernel_times",
@@ -152,6 +155,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_acoustic2d_forward.argtypes = [
             ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
             c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_acoustic2d_forward_noalloc.restype = ctypes.c_int
+        lib.gbw_acoustic2d_forward_noalloc.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs), c_vp]
         lib.gbw_acoustic2d_backward.restype = ctypes.c_int
         lib.gbw_acoustic2d_backward.argtypes = [
             ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Adjoint),

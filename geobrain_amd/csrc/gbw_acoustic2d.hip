@@ -2636,6 +2636,43 @@ int make_batch(const gbw_acoustic2d_problem* probs, int n, BatchLayout& L) {
 
 }  // namespace
 
+// Workspace of gbw_acoustic2d_forward_noalloc: one array per GPU, in the
+// library's code object.  In host code `scratch` names a host-side stand-in,
+// not device memory: its device address is looked up per device with
+// hipGetSymbolAddress and only that address is passed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[GBW_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // device ordinal -> address of `scratch` there
+
+// address of the current device's `scratch`, looked up once per device with
+// that device current (each GPU holds its own copy of the array)
+int scratch_address(void** ptr) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(GBW_ERR_HIP, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
+
+// suppresses the timing hook's event creation for the calls in its scope
+struct NoTimingScope {
+  bool prev;
+  NoTimingScope() : prev(t_no_timing) { t_no_timing = true; }
+  ~NoTimingScope() { t_no_timing = prev; }
+};
+
+}  // namespace
+
 extern "C" {
 
 int gbw_abi_version(void) { return GBW_ABI_VERSION; }
@@ -2739,6 +2776,26 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
     }
   }
   return flush_timing(st);
+}
+
+int gbw_acoustic2d_forward_noalloc(const gbw_acoustic2d_problem* prob,
+                                   const gbw_acoustic2d_inputs* in,
+                                   const gbw_acoustic2d_outputs* out, void* stream) {
+  Plan pl;
+  int rc = make_plan(prob, 0, pl);
+  if (rc) return rc;
+  if (pl.total > GBW_NOALLOC_SCRATCH_BYTES)
+    return fail(GBW_ERR_WORKSPACE,
+                "problem too large for the no-allocation forward: needs " +
+                    std::to_string(pl.total) + " bytes of workspace, its scratch array holds " +
+                    std::to_string((size_t)GBW_NOALLOC_SCRATCH_BYTES));
+  void* ws = nullptr;
+  if ((rc = scratch_address(&ws))) return rc;
+  // the forward clears or writes every part of the workspace it reads (state
+  // fills, receiver map, coefficients, energy partials) before reading it, so
+  // what an earlier call left in the array does not matter
+  NoTimingScope no_timing;
+  return gbw_acoustic2d_forward(prob, in, out, ws, GBW_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoustic2d_inputs* in,

@@ -52,6 +52,8 @@ inline bool g_timing = false;
 inline std::map<std::string, std::pair<long long, double>> g_times;  // launches, ms
 inline std::vector<hipEvent_t> g_event_pool;
 inline thread_local std::vector<TimedLaunch> t_pending;
+// set while an entry point that must not create runtime objects runs
+inline thread_local bool t_no_timing = false;
 
 inline hipEvent_t take_event() {
   std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -73,7 +75,7 @@ struct LaunchTimer {
     bool on;
     {
       std::lock_guard<std::mutex> lk(g_timing_mu);
-      on = g_timing;
+      on = g_timing && !t_no_timing;
     }
     if (on) {  // never inside a graph capture: the call could not wait for its events
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

@@ -141,6 +141,28 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob,
                             void* workspace, size_t workspace_bytes,
                             void* stream /* hipStream_t */);
 
+/* Size of the device array behind gbw_acoustic2d_forward_noalloc (per GPU). */
+#define GBW_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* Forward modelling that allocates nothing.  No counterpart in the reference:
+ * it is gbw_acoustic2d_forward with the workspace taken from a fixed device
+ * array of GBW_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it
+ * runs on until the process ends, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The caller still passes inputs and outputs (and the
+ * checkpoint buffer when checkpoint_every > 0).  A problem whose forward
+ * workspace (gbw_acoustic2d_workspace_bytes(prob, 0)) exceeds the array is
+ * rejected with GBW_ERR_WORKSPACE before anything is enqueued.  The call makes
+ * no hipMalloc / hipFree and no other allocation on the device, and never
+ * records timing events.  Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (two host threads, or two streams whose work
+ * overlaps) race; calls in order on one stream do not.  Results equal
+ * gbw_acoustic2d_forward's bit for bit. */
+int gbw_acoustic2d_forward_noalloc(const gbw_acoustic2d_problem* prob,
+                                   const gbw_acoustic2d_inputs* in,
+                                   const gbw_acoustic2d_outputs* out,
+                                   void* stream /* hipStream_t */);
+
 /* Batched forward modelling of n independent problems (uneven grids, step
  * counts and shot counts) as one launch sequence: one launch per two time steps
  * covers every problem still running.  No counterpart in the reference, which

@@ -1,0 +1,97 @@
+"""gbw_acoustic2d_forward_noalloc (the forward whose workspace is the library's
+fixed per-GPU device array) called through the C ABI against
+gbw_acoustic2d_forward with a caller workspace: bit-equal outputs on two calls
+in a row, the second with a smaller problem (what the first call left in the
+array must not leak into it), and a problem too large for the array rejected
+before anything is enqueued."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from geobrain_amd import _native
+from tests._fixtures import load
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _case(name, checkpoint_every=0):
+    """C-ABI problem, inputs and caller-owned output tensors for a fixture."""
+    z, m = load(name)
+    S, R, nt = len(z["in_src_x"]), len(z["in_rcv_x"]), m["nt"]
+    p = _native.Problem()
+    p.nx, p.nz, p.dx, p.dz = m["nx"], m["nz"], m["dx"], m["dx"]
+    p.nt, p.dt, p.nabc, p.free_surface = nt, m["dt"], m["nabc"], int(m["free_surface"])
+    p.src_n, p.rcv_n = S, R
+    w = np.asarray(z["in_wavelet"], np.float32)
+    p.wavelet_shared = int(w.ndim == 1)
+    p.energy_chunk = -(-nt // max(1, m["segments"]))
+    p.checkpoint_every = checkpoint_every
+    host = {k: np.ascontiguousarray(z["in_" + k], np.int64)
+            for k in ("src_x", "src_z", "rcv_x", "rcv_z")}
+    dev = {k: torch.as_tensor(np.asarray(z["in_" + k], np.float32), device=DEV)
+           for k in ("vp", "rho", "damp")}
+    dev["src_v"] = torch.as_tensor(w, device=DEV)
+    return p, host, dev, (S, nt, R, m["nz"], m["nx"])
+
+
+def _run(lib, p, host, dev, dims, noalloc):
+    S, nt, R, nz, nx = dims
+    ins = _native.Inputs()
+    ins.src_x, ins.src_z = host["src_x"].ctypes.data, host["src_z"].ctypes.data
+    ins.rcv_x, ins.rcv_z = host["rcv_x"].ctypes.data, host["rcv_z"].ctypes.data
+    ins.src_v, ins.damp = dev["src_v"].data_ptr(), dev["damp"].data_ptr()
+    ins.vp, ins.rho = dev["vp"].data_ptr(), dev["rho"].data_ptr()
+    f32 = dict(dtype=torch.float32, device=DEV)
+    out = {k: torch.full((S, nt, R), 7.0, **f32) for k in ("p", "vx", "vz")}
+    out.update({k: torch.full((nz, nx), 7.0, **f32) for k in ("ep", "evx", "evz")})
+    ck_bytes = lib.gbw_acoustic2d_checkpoint_bytes(ctypes.byref(p))
+    out["ck"] = torch.full((max(ck_bytes, 1),), 7, dtype=torch.uint8, device=DEV)
+    o = _native.Outputs()
+    o.rec_p, o.rec_vx, o.rec_vz = (out[k].data_ptr() for k in ("p", "vx", "vz"))
+    o.energy_p, o.energy_vx, o.energy_vz = (out[k].data_ptr() for k in ("ep", "evx", "evz"))
+    o.checkpoints = out["ck"].data_ptr() if ck_bytes else None
+    stream = torch.cuda.current_stream(DEV).cuda_stream
+    if noalloc:
+        rc = lib.gbw_acoustic2d_forward_noalloc(ctypes.byref(p), ctypes.byref(ins),
+                                                ctypes.byref(o), stream)
+    else:
+        nbytes = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 0)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=DEV)
+        rc = lib.gbw_acoustic2d_forward(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(o),
+                                        ws.data_ptr(), nbytes, stream)
+    _native.check(rc)
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def test_noalloc_forward_equals_forward_on_two_calls():
+    lib = _native.load()
+    # first call: the config-1 problem (100 x 200, 1000 steps) with checkpoints
+    # every 50 steps; second call: a 2-shot 8 x 12 grid, far smaller, so stale
+    # state from the first would show if any of it were read
+    for name, k in (("c1_fwd", 50), ("tiny0_fwd", 0)):
+        print(f"[case] no-allocation forward: {name}, checkpoint_every={k}", flush=True)
+        p, host, dev, dims = _case(name, checkpoint_every=k)
+        nbytes = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 0)
+        assert 0 < nbytes <= _native.NOALLOC_SCRATCH_BYTES
+        want = _run(lib, p, host, dev, dims, noalloc=False)
+        got = _run(lib, p, host, dev, dims, noalloc=True)
+        for key in want:
+            np.testing.assert_array_equal(got[key], want[key], err_msg=f"{name} {key}")
+        print(f"  {name}: every output bit-equal ({nbytes} bytes of scratch used)")
+
+
+def test_noalloc_forward_rejects_oversized_problem():
+    lib = _native.load()
+    p, host, dev, dims = _case("c1_fwd")
+    p.src_n = 128  # 128 shots of the config-1 grid need about twice what the array holds
+    nbytes = lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p), 0)
+    assert nbytes > _native.NOALLOC_SCRATCH_BYTES
+    ins, o = _native.Inputs(), _native.Outputs()
+    rc = lib.gbw_acoustic2d_forward_noalloc(ctypes.byref(p), ctypes.byref(ins), ctypes.byref(o),
+                                            torch.cuda.current_stream(DEV).cuda_stream)
+    assert rc == -3 and "no-allocation forward" in _native.last_error()
