@@ -331,6 +331,68 @@ def uneven_batch(steps: int, warmup: int, tune: bool, launch=None) -> dict:
                              "avg_launch_us": round(kt[1] / max(kt[0], 1) * 1e3, 2)}}
 
 
+def uneven_gradient(steps: int, warmup: int, tune: bool, launch=None) -> dict:
+    """The uneven-batch workload as a gradient: the same 96 problems from seed
+    7, misfit sum((p - d_obs)^2) against data modelled in 2%-slower velocity
+    models, forward with checkpoints + adjoint sources + recompute + adjoint
+    of all problems captured in one HIP graph (BatchedGradient), launch choice
+    from the autotuner.  Returns forward+adjoint throughput, per-problem
+    latency and the batch kernels' live per-launch times."""
+    import torch
+    from geobrain_amd import _native, autotune, synthetic
+    from geobrain_amd import batch as B
+    dev = torch.device("cuda:0")
+    specs = synthetic.uneven_batch(96, seed=7)
+    problems = [B.problem_inputs(sp, dev) for sp in specs]
+    obs = B.BatchedForward([dict(a, vp=a["vp"] * 0.98) for a in problems])
+    d_obs = [o["p"].clone() for o in obs.run()]
+    torch.cuda.synchronize()
+    del obs
+    torch.cuda.empty_cache()
+    if launch is None:
+        launch = autotune.tune_batch_gradient(problems, verbose=True) if tune else B.BatchLaunch()
+    bg = B.BatchedGradient(problems, launch)
+    # live per-launch times of the batch kernels (an eager step; the graph's
+    # launches are the same)
+    _native.set_kernel_timing(True)
+    bg.misfit_step(d_obs)
+    torch.cuda.synchronize()
+    kt = _native.kernel_times()
+    _native.set_kernel_timing(False)
+    bg.capture(lambda: bg.misfit_step(d_obs))
+    for _ in range(warmup):
+        bg.replay()
+    torch.cuda.synchronize()
+    times, lats = [], []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bg.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+        lats.append(bg.latencies_ms())
+    t = float(np.median(times))
+    lat = np.median(np.stack(lats), axis=0)
+    nts = np.array([sp["nt"] for sp in specs])
+    return {"metric": "Gcell-updates/s, forward + adjoint gradient, uneven batch in one HIP graph",
+            "value": round(bg.cells / t / 1e9, 3), "unit": "Gcell-updates/s",
+            "graph_replay_s": round(t, 4), "replays": steps,
+            "workload": "96 problems from seed 7: nz 200-600, nx 200-1200, nt 1000-4000, "
+                        "1-8 shots, receivers on every column; gradient of the seismogram "
+                        "misfit against data from 2%-slower models",
+            "cell_updates": bg.cells, "shots": int(sum(sp["n_shots"] for sp in specs)),
+            "launch": {"shots_per_block": launch.shots_per_block, "tile_rows": launch.tile_rows},
+            "workspace_gb": round(bg.ws.numel() / 1e9, 1),
+            "latency_ms": {"min": round(float(lat.min()), 2),
+                           "median": round(float(np.median(lat)), 2),
+                           "p90": round(float(np.percentile(lat, 90)), 2),
+                           "max": round(float(lat.max()), 2),
+                           "per_step_us_median": round(float(np.median(lat / nts * 1e3)), 2)},
+            "kernels": {k: {"launches": n, "avg_launch_us": round(ms / n * 1e3, 2)}
+                        for k, (n, ms) in kt.items()}}
+
+
 def workload_3d(n_shots=8, n=200, nt=1000, nabc=20, dx=10.0, dt=8e-4, f0=10.0):
     """3D forward: n^3 grid (+nabc per side), depth-linear 1500-4500 m/s plus a
     3D Gaussian random field (lh 200 m, lv 50 m, std 200, seed 0) clipped to
@@ -755,6 +817,8 @@ def main():
             blaunch = B.BatchLaunch(*[int(v) for v in args.batch_launch.split(",")])
         rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune,
                                            launch=blaunch)
+        torch.cuda.empty_cache()
+        rec["uneven_gradient"] = uneven_gradient(steps=2, warmup=1, tune=not args.no_tune)
     if not args.no_3d:
         torch.cuda.empty_cache()
         rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d,

@@ -36,6 +36,10 @@ EXPORTS = (
     "gbw_acoustic2d_batch_setup",
     "gbw_acoustic2d_batch_forward",
     "gbw_device_wall_clock_khz",
+    "gbw_acoustic2d_batch_gradient_workspace_bytes",
+    "gbw_acoustic2d_batch_gradient_setup",
+    "gbw_acoustic2d_batch_gradient_forward",
+    "gbw_acoustic2d_batch_gradient_backward",
     "gbw_acoustic3d_workspace_bytes",
     "gbw_acoustic3d_checkpoint_bytes",
     "gbw_acoustic3d_forward",
@@ -175,6 +179,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_acoustic2d_batch_forward.restype = ctypes.c_int
         lib.gbw_acoustic2d_batch_forward.argtypes = [
             ctypes.POINTER(Problem), ctypes.c_int, c_vp, ctypes.c_size_t, c_vp, c_vp]
+        lib.gbw_acoustic2d_batch_gradient_workspace_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic2d_batch_gradient_workspace_bytes.argtypes = [
+            ctypes.POINTER(Problem), ctypes.c_int]
+        lib.gbw_acoustic2d_batch_gradient_setup.restype = ctypes.c_int
+        lib.gbw_acoustic2d_batch_gradient_setup.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
+            ctypes.POINTER(Adjoint), ctypes.c_int, c_vp, ctypes.c_size_t, c_vp]
+        for name in ("gbw_acoustic2d_batch_gradient_forward", "gbw_acoustic2d_batch_gradient_backward"):
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, c_vp, ctypes.c_size_t, c_vp, c_vp]
         lib.gbw_device_wall_clock_khz.restype = ctypes.c_int
         lib.gbw_device_wall_clock_khz.argtypes = []
         lib.gbw_acoustic3d_workspace_bytes.restype = ctypes.c_size_t

@@ -227,3 +227,55 @@ def tune_batch(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
         _cache[key] = best
         _save_file()
     return B.BatchLaunch(*best)
+
+
+def batch_gradient_candidates() -> List[Tuple[int, int]]:
+    """(shots_per_block, tile_rows) tried for a batched gradient."""
+    return [(0, 0), (2, 0), (4, 0), (0, 24), (2, 24), (4, 24)]
+
+
+def tune_batch_gradient(problems, nt_trial: int = 80, candidates=None, repeats: int = 2,
+                        verbose: bool = False):
+    """Pick the batched-gradient launch choice (shots per workgroup, tile shape)
+    for ``problems`` by timing a misfit-gradient step of each candidate on the
+    same problems cut to ``nt_trial`` steps.  Returns a ``batch.BatchLaunch``;
+    results are cached per batch shape like ``tune``."""
+    from . import batch as B
+    shapes = [(a["nz"] + 2 * a["nabc"], a["nx"] + 2 * a["nabc"], int(a["src_n"]), int(a["rcv_n"]),
+               bool(a["free_surface"]), a["src_v"].dim() == 1) for a in problems]
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "none"
+    key = json.dumps(["batch_gradient", shapes, dev])
+    with _lock:
+        if not _cache:
+            _load_file()
+        hit = _cache.get(key)
+    if hit is not None and len(hit) == 2:
+        return B.BatchLaunch(shots_per_block=hit[0], tile_rows=hit[1])
+    trial = []
+    for a in problems:
+        nt = min(nt_trial, int(a["nt"]))
+        trial.append(dict(a, nt=nt, src_v=a["src_v"][..., :nt].contiguous()))
+    best, best_t = None, float("inf")
+    for spb, rows in (candidates or batch_gradient_candidates()):
+        bg = B.BatchedGradient(trial, B.BatchLaunch(shots_per_block=spb, tile_rows=rows))
+        d_obs = [torch.zeros_like(o["p"]) for o in bg.outputs]
+        times = []
+        for _ in range(repeats + 1):
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record()
+            bg.misfit_step(d_obs)
+            end.record()
+            torch.cuda.synchronize()
+            times.append(start.elapsed_time(end))
+        t = min(times[1:])
+        if verbose:
+            print(f"  autotune batch gradient shots_per_block={spb} tile_rows={rows}: {t:.2f} ms",
+                  file=sys.stderr, flush=True)
+        if t < best_t:
+            best, best_t = (spb, rows), t
+        del bg, d_obs
+    with _lock:
+        _cache[key] = best
+        _save_file()
+    return B.BatchLaunch(shots_per_block=best[0], tile_rows=best[1])

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -152,6 +153,8 @@ struct FwdArgs {
   float* gxbuf; float* gzbuf;    // [S][plane] vx / vz brackets of this step, or null
   long long bstride;             // two-step kernel: offset of step it+1's brackets
   int one;                       // two-step kernel: do step it only (the last of a problem)
+  int skip1;                     // two-step kernel: step it is before the problem's first
+                                 // step (it == -1): run step it+1 only, from the input state
 };
 
 // ---------------------------------------------------------------------------
@@ -517,7 +520,9 @@ constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surfa
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
 // one workgroup's tile `tile` and shot group `group` of a problem
-template <int TZ, int TXT, int NTH, bool RECOMP>
+// PHASED: the batched gradient's launches, in which a problem may start at the
+// second step (a.skip1); the other kernels compile without that test
+template <int TZ, int TXT, int NTH, bool RECOMP, bool PHASED = false>
 __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group) {
   // this kernel's tile width (columns) and the staging geometry that follows
   constexpr int TX = TXT, GW = TXT / 4, W2 = GW + 4, LW2 = GW + 6;
@@ -745,8 +750,8 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   for (int i = tid; i < s_end - s_begin; i += NTH) {
     const float* wv = a.wav + (long long)(s_begin + i) * a.wav_stride + a.it;
     sSrc[i] = a.src_cell[s_begin + i];
-    sW[0][i] = a.dt * wv[0];
-    sW[1][i] = a.one ? 0.f : a.dt * wv[1];  // step it+1 exists only without `one`
+    sW[0][i] = (PHASED && a.skip1) ? 0.f : a.dt * wv[0];  // step it exists only without `skip1`
+    sW[1][i] = a.one ? 0.f : a.dt * wv[1];    // step it+1 exists only without `one`
   }
   __syncthreads();
   for (int s = s_begin; s < s_end; ++s) {
@@ -773,6 +778,9 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
     }
     if (s + 1 < s_end) load(s + 1);  // next shot's loads overlap this shot
     __syncthreads();
+    // step it (uniform per launch: a problem that starts at step it+1 skips it;
+    // its staged input state then feeds step it+1 directly)
+    if (!(PHASED && a.skip1)) {
     // step it: p'
 #pragma unroll
     for (int n = 0; n < NS; ++n) {
@@ -852,6 +860,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
         }
       }
     }
+    }  // !skip1
     if (a.one) {
       // step it only: store p', v' of the tile (LDS holds both mirrors'
       // results for every tile row, row f-1 included)
@@ -1298,6 +1307,12 @@ struct AdjArgs {
   float* gA1;   // [group][plane]
   float* gA2;   // [group][plane] or null
   float* gw;    // [S][nt] or null
+  int one;      // two-step kernel: adjoint step it only (it == 0: the problem's first step)
+  // the batched gradient's launches (PHASED kernel): the per-problem arguments
+  // come from these descriptors, the rest of this struct is unused
+  const struct GradDesc* batch_descs;
+  const int* batch_units;  // n + 1 prefix sums of work units
+  int batch_n, batch_g, batch_parity, batch_c, batch_k;
 };
 
 // RHO: also accumulate the density gradient (needs the recompute's vx/vz brackets)
@@ -1638,6 +1653,40 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
   }
 }
 
+// Descriptor of one problem of the batched gradient (see the batched-gradient
+// kernels below).
+struct GradDesc {
+  FwdArgs f;               // forward / recompute: geometry, coefficients, receivers, outputs
+  AdjArgs b;               // adjoint: adjoint sources, gradient accumulators
+  float* A; float* B;      // state ping-pong
+  float* LA; float* LB;    // adjoint-state ping-pong
+  float* D;                // [k][S][plane] brackets of the current segment, by global step
+  char* slots;             // checkpoint slots of segments c0 .. nseg-1
+  float* G1;               // [groups][plane] imaging sums
+  float* GW;               // [S][nt] wavelet-gradient samples
+  const int* rcell;        // [R] padded cell of each receiver
+  const float* vp; const float* rho;   // the problem's model (chain rule)
+  float* grad_vp;          // (nz, nx) output or null
+  float* grad_w;           // shape of src_v, output or null
+  float dz;
+  int shared;              // wavelet shared by every shot
+  long long fstride;       // S * plane
+  long long slot_bytes;    // bytes per checkpoint slot (3 * fstride floats)
+  int off;                 // global step of the problem's first step
+  int c0;                  // segment holding that step
+  int groups;              // shot groups (units = ntiles * groups)
+  int pad;
+};
+
+__device__ __forceinline__ int batch_problem(const int* __restrict__ unit_start, int n_active, int u) {
+  int lo = 0, hi = n_active - 1;  // last problem whose first unit is <= u
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (unit_start[mid] <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------------------
 // Two-step adjoint kernel: adjoint steps it and it-1 in one launch, on the
 // staging geometry of the two-step forward kernel (the adjoint's stencils
@@ -1654,7 +1703,10 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
 // in the same order, and the two steps' gradient sums are added in step order,
 // so results equal two one-step launches bit for bit.
 // ---------------------------------------------------------------------------
-template <int TZ, int TXT, int NTH, bool RHO>
+// PHASED: a launch of the batched gradient (arguments a_in.batch_*): the
+// workgroup finds its problem, whose first step may be the launch's first
+// (a.one); the single-problem instantiation compiles without either
+template <int TZ, int TXT, int NTH, bool RHO, bool PHASED = false>
 __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
   // this kernel's tile width (columns) and the staging geometry that follows
   constexpr int TX = TXT, GW = TXT / 4, W2 = GW + 4, LW2 = GW + 6;
@@ -1669,15 +1721,44 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
   // imaging-condition sums of the tile, per step (each cell owned by one thread)
   __shared__ float4 sG[2][TZ][GW];
   __shared__ float4 sG2[RHO ? 2 : 1][RHO ? TZ : 1][RHO ? GW : 1];
+  int tile = 0, group = 0;
+  if constexpr (PHASED) {
+    // the kernel argument carries only the batch launch: replace it by the
+    // arguments of this workgroup's problem
+    const int u = xcd_tile();
+    const int* units = a.batch_units;
+    const int n_active = a.batch_n, parity = a.batch_parity, g = a.batch_g;
+    const int seg0 = a.batch_c * a.batch_k;
+    if (u >= units[n_active]) return;
+    const int p = batch_problem(units, n_active, u);
+    const GradDesc& bd = a.batch_descs[p];
+    a = bd.b;
+    const float* lin = parity ? bd.LB : bd.LA;
+    float* lout = parity ? bd.LA : bd.LB;
+    a.LP0 = lin; a.LX0 = lin + bd.fstride; a.LZ0 = lin + 2 * bd.fstride;
+    a.LP1 = lout; a.LX1 = lout + bd.fstride; a.LZ1 = lout + 2 * bd.fstride;
+    a.it = g - bd.off;
+    a.one = a.it == 0;
+    a.dbuf = bd.D + (long long)(g - seg0) * bd.fstride;
+    a.dbuf1 = a.one ? a.dbuf : a.dbuf - bd.fstride;
+    a.gxbuf = a.gzbuf = a.gxbuf1 = a.gzbuf1 = nullptr;
+    a.gA2 = nullptr;
+    const int local = u - units[p];
+    tile = local % a.ntiles;
+    group = local / a.ntiles;
+  }
   const Dims d = a.d;
   const int tid = threadIdx.x;
-  const int tile = xcd_tile();
-  if (tile >= a.ntiles) return;
+  if constexpr (!PHASED) {
+    tile = xcd_tile();
+    if (tile >= a.ntiles) return;
+    group = blockIdx.y;
+  }
   const int x0 = (tile % a.tiles_x) * TX;
   const int z0 = (tile / a.tiles_x) * TZ;
-  const int s_begin = blockIdx.y * a.spb;
+  const int s_begin = group * a.spb;
   const int s_end = min(a.S, s_begin + a.spb);
-  if (s_begin >= s_end) return;
+  if (s_begin >= s_end) return;  // uniform per workgroup
   const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
   const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
   const float4* __restrict__ omk2 = reinterpret_cast<const float4*>(a.coef + 2 * d.plane);
@@ -1857,6 +1938,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
       const int step = a.it - k;
       // the next shot's loads overlap this shot's second step
       if (k == 1 && s + 1 < s_end) load(s + 1);
+      if (PHASED && k == 1 && a.one) break;  // uniform: step it was the problem's first step
       const unsigned preg_s = k == 0 ? F_P1 : F_P2;
       const unsigned vreg_s = k == 0 ? F_V1 : F_T;
       float4 dcur[NS], gxc[RHO ? NS : 1], gzc[RHO ? NS : 1];
@@ -2088,7 +2170,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
     }
   }
   // the two steps' sums in step order: as two one-step launches add them
-  float4* g1 = reinterpret_cast<float4*>(a.gA1 + (long long)blockIdx.y * d.plane);
+  float4* g1 = reinterpret_cast<float4*>(a.gA1 + (long long)group * d.plane);
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
     if ((fl[n] & (F_T | F_OK)) == (F_T | F_OK)) {
@@ -2096,18 +2178,19 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
       const int tr = item / W2 - 6, h = item % W2 - 2;
       const float4 v = ld4u(g1, idx[n]), a0 = sG[0][tr][h], a1v = sG[1][tr][h];
       float4 w = make_float4(v.x + a0.x, v.y + a0.y, v.z + a0.z, v.w + a0.w);
-      w = make_float4(w.x + a1v.x, w.y + a1v.y, w.z + a1v.z, w.w + a1v.w);
+      if (!(PHASED && a.one)) w = make_float4(w.x + a1v.x, w.y + a1v.y, w.z + a1v.z, w.w + a1v.w);
       st4u(g1, idx[n], w);
       if constexpr (RHO) {
-        float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)blockIdx.y * d.plane);
+        float4* g2 = reinterpret_cast<float4*>(a.gA2 + (long long)group * d.plane);
         const float4 u = ld4u(g2, idx[n]), b0 = sG2[0][tr][h], b1 = sG2[1][tr][h];
         float4 t = make_float4(u.x + b0.x, u.y + b0.y, u.z + b0.z, u.w + b0.w);
-        t = make_float4(t.x + b1.x, t.y + b1.y, t.z + b1.z, t.w + b1.w);
+        if (!(PHASED && a.one)) t = make_float4(t.x + b1.x, t.y + b1.y, t.z + b1.z, t.w + b1.w);
         st4u(g2, idx[n], t);
       }
     }
   }
 }
+
 
 // lambda^{nt} = scatter of the records of the last step (zeroed buffer in)
 __global__ void adj_inject_kernel(Dims d, int S, int nt, int it, int R, const int* __restrict__ rcell,
@@ -2132,15 +2215,120 @@ __global__ void adj_inject_kernel(Dims d, int S, int nt, int it, int R, const in
   LZ[q] = LZ[q] + sz;
 }
 
+// ---------------------------------------------------------------------------
+// Batched gradient of uneven problems
+//
+// Problems are aligned at their last step: problem i's local step t runs at
+// global step g = t + off_i, off_i = T - nt_i, T the longest step count
+// rounded up to even.  Checkpoints sit at global multiples of k (even), so
+// every problem's checkpoint intervals ("segments") coincide and one launch per
+// two global steps covers every problem that has started: forward, segment
+// recompute and adjoint keep the single-problem launch structure.  Sorted by
+// step count, longest first, the problems running at a global step are a
+// prefix, and their work units are numbered consecutively as in the batched
+// forward.  A problem whose first step falls on the second step of a launch
+// skips the first one (skip1; its state is still zero); in reverse, a launch
+// whose second step would precede the problem's first runs one step (one).
+// ---------------------------------------------------------------------------
+// forward (RECOMP false, global steps g, g+1; launch j of the forward) or the
+// recompute of segment c (RECOMP true, launch j of the segment, g = c*k + 2j)
+template <int TZ, int TXT, bool RECOMP>
+__global__ __launch_bounds__(FWD2_THREADS) void grad_batch_step_kernel(
+    const GradDesc* __restrict__ descs, const int* __restrict__ unit_start, int n_active, int g,
+    int j, int c, int k) {
+  const int u = xcd_tile();
+  if (u >= unit_start[n_active]) return;
+  const int p = batch_problem(unit_start, n_active, u);
+  const GradDesc& bd = descs[p];
+  FwdArgs a = bd.f;
+  const float* in;
+  float* out;
+  if (RECOMP) {
+    out = (j & 1) ? bd.B : bd.A;
+    in = j == 0 ? reinterpret_cast<const float*>(bd.slots + (long long)(c - bd.c0) * bd.slot_bytes)
+                : ((j & 1) ? bd.A : bd.B);
+    a.dbuf = bd.D + (long long)(g - c * k) * bd.fstride;
+    a.bstride = bd.fstride;
+    a.gxbuf = a.gzbuf = nullptr;
+  } else {
+    in = (j & 1) ? bd.B : bd.A;
+    out = (j & 1) ? bd.A : bd.B;
+  }
+  a.P0 = in; a.VX0 = in + bd.fstride; a.VZ0 = in + 2 * bd.fstride;
+  a.P1 = out; a.VX1 = out + bd.fstride; a.VZ1 = out + 2 * bd.fstride;
+  a.it = g - bd.off;
+  a.skip1 = a.it == -1;
+  a.one = 0;
+  const int local = u - unit_start[p];
+  fwd2_tile<TZ, TXT, FWD2_THREADS, RECOMP, true>(a, local % a.ntiles, local / a.ntiles);
+}
+
+// per-problem fills and copies of the batched gradient (blockIdx.y = problem):
+//   0: forward start: zero A, B and the first checkpoint slot
+//   1: segment c start: zero A and B of the problems whose first step lies in c
+//   2: reverse start: zero the adjoint state (both sides), imaging and wavelet sums
+//   3: checkpoint c: copy the state in A (which = 0) or B (which = 1) to slot c
+__global__ void grad_batch_prep_kernel(const GradDesc* __restrict__ descs, int mode, int c,
+                                       int which) {
+  const GradDesc& bd = descs[blockIdx.y];
+  const long long st = 3 * bd.fstride;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mode == 0) {
+    float* s0 = reinterpret_cast<float*>(bd.slots);
+    for (long long i = i0; i < st; i += stride) { bd.A[i] = 0.f; bd.B[i] = 0.f; s0[i] = 0.f; }
+  } else if (mode == 1) {
+    if (bd.c0 != c) return;
+    for (long long i = i0; i < st; i += stride) { bd.A[i] = 0.f; bd.B[i] = 0.f; }
+  } else if (mode == 2) {
+    for (long long i = i0; i < st; i += stride) { bd.LA[i] = 0.f; bd.LB[i] = 0.f; }
+    const long long ng = (long long)bd.groups * bd.b.d.plane;
+    for (long long i = i0; i < ng; i += stride) bd.G1[i] = 0.f;
+    const long long nw = (long long)bd.b.S * bd.b.nt;
+    for (long long i = i0; i < nw; i += stride) bd.GW[i] = 0.f;
+  } else {
+    if (c <= bd.c0) return;  // slot c0 holds the zero state from the start
+    const float* src = which ? bd.B : bd.A;
+    float* dst = reinterpret_cast<float*>(bd.slots + (long long)(c - bd.c0) * bd.slot_bytes);
+    for (long long i = i0; i < st; i += stride) dst[i] = src[i];
+  }
+}
+
+// lambda at each problem's last step: scatter of that step's adjoint sources
+// into LA (zeroed), as adj_inject_kernel does for one problem
+__global__ void grad_batch_inject_kernel(const GradDesc* __restrict__ descs) {
+  const GradDesc& bd = descs[blockIdx.y];
+  const AdjArgs& b = bd.b;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= b.S * b.R) return;
+  const int s = idx / b.R, r = idx % b.R;
+  const int cell = bd.rcell[r];
+  if (b.rhead[cell] != r) return;  // only the list head sums its cell
+  float sp = 0.f, sx = 0.f, sz = 0.f;
+  for (int h = r; h >= 0; h = b.rnext[h]) {
+    const long long o = ((long long)s * b.nt + (b.nt - 1)) * b.R + h;
+    if (b.g_p) sp += b.g_p[o];
+    if (b.g_vx) sx += b.g_vx[o];
+    if (b.g_vz) sz += b.g_vz[o];
+  }
+  const long long q = (long long)s * b.d.plane + cell;
+  bd.LA[q] = bd.LA[q] + sp;
+  bd.LA[bd.fstride + q] = bd.LA[bd.fstride + q] + sx;
+  bd.LA[2 * bd.fstride + q] = bd.LA[2 * bd.fstride + q] + sz;
+}
+
+__global__ void stamp_kernel(unsigned long long* t) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *t = wall_clock64();
+}
+
 // chain rule alpha1 = den*c*c*dt/dz, alpha2 = dt/(den*dz) and the fold of
 // pad_model's edge replication (acoustic_2d.py:407-410, :18-66)
-__global__ void fold_kernel(Dims d, int groups, const float* __restrict__ gA1,
-                            const float* __restrict__ gA2, const float* __restrict__ vp,
-                            const float* __restrict__ rho, float dt, float dz,
-                            float* __restrict__ grad_vp, float* __restrict__ grad_rho) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = blockIdx.y;
-  if (b >= d.nx || a >= d.nz) return;
+__device__ __forceinline__ void fold_cell(const Dims& d, int groups, const float* __restrict__ gA1,
+                                          const float* __restrict__ gA2,
+                                          const float* __restrict__ vp,
+                                          const float* __restrict__ rho, float dt, float dz,
+                                          float* __restrict__ grad_vp,
+                                          float* __restrict__ grad_rho, int a, int b) {
   const int i0 = a == 0 ? 0 : a + d.nabc;
   const int i1 = a == d.nz - 1 ? d.NZ - 1 : a + d.nabc;
   const int j0 = b == 0 ? 0 : b + d.nabc;
@@ -2177,10 +2365,19 @@ __global__ void fold_kernel(Dims d, int groups, const float* __restrict__ gA1,
   if (grad_rho) grad_rho[(long long)a * d.nx + b] = sr;
 }
 
+__global__ void fold_kernel(Dims d, int groups, const float* __restrict__ gA1,
+                            const float* __restrict__ gA2, const float* __restrict__ vp,
+                            const float* __restrict__ rho, float dt, float dz,
+                            float* __restrict__ grad_vp, float* __restrict__ grad_rho) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (b >= d.nx || a >= d.nz) return;
+  fold_cell(d, groups, gA1, gA2, vp, rho, dt, dz, grad_vp, grad_rho, a, b);
+}
+
 // wavelet gradient: per shot, or summed over shots for a shared wavelet
-__global__ void wavelet_grad_kernel(const float* __restrict__ gw, int S, int nt, int shared,
-                                    float* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void wavelet_grad(const float* __restrict__ gw, int S, int nt,
+                                             int shared, float* __restrict__ out, int t) {
   if (shared) {
     if (t >= nt) return;
     float s = 0.f;
@@ -2190,6 +2387,28 @@ __global__ void wavelet_grad_kernel(const float* __restrict__ gw, int S, int nt,
     if (t >= S * nt) return;
     out[t] = gw[t];
   }
+}
+
+__global__ void wavelet_grad_kernel(const float* __restrict__ gw, int S, int nt, int shared,
+                                    float* __restrict__ out) {
+  wavelet_grad(gw, S, nt, shared, out, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// a problem of the batched gradient is done: chain rule and pad_model fold of
+// its imaging sums (blockIdx.y = model row), then its wavelet gradient
+__global__ void grad_batch_fold_kernel(const GradDesc* __restrict__ descs, int p) {
+  const GradDesc& bd = descs[p];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  const Dims& d = bd.f.d;
+  if (!bd.grad_vp || b >= d.nx || a >= d.nz) return;
+  fold_cell(d, bd.groups, bd.G1, nullptr, bd.vp, bd.rho, bd.f.dt, bd.dz, bd.grad_vp, nullptr, a, b);
+}
+
+__global__ void grad_batch_wgrad_kernel(const GradDesc* __restrict__ descs, int p) {
+  const GradDesc& bd = descs[p];
+  if (!bd.grad_w) return;
+  wavelet_grad(bd.GW, bd.b.S, bd.b.nt, bd.shared, bd.grad_w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2634,6 +2853,116 @@ int make_batch(const gbw_acoustic2d_problem* probs, int n, BatchLayout& L) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched gradient: layout and schedule (host)
+// ---------------------------------------------------------------------------
+struct GradLayout {
+  int n = 0, T = 0, k = 0, nseg = 0, TZ2 = 16;
+  std::vector<int> order;        // sorted position -> caller's problem index
+  std::vector<Plan> plans;       // per sorted position
+  std::vector<size_t> off_prob;  // sub-workspace of each problem
+  std::vector<size_t> off_D;     // its segment bracket store (k steps), within the workspace
+  std::vector<size_t> off_slots; // its checkpoint slots c0 .. nseg-1
+  std::vector<int> off, c0;      // global first step, first segment
+  std::vector<int> unit_start;   // n + 1 prefix sums of (two-step tiles x shot groups)
+  size_t off_desc = 0, off_units = 0, total = 0;
+  // problems running at global step g (sorted by first step, so a prefix)
+  int active(int g) const {
+    return (int)(std::upper_bound(off.begin(), off.end(), g) - off.begin());
+  }
+};
+
+// default interval: about sqrt(3 T) (checkpoints 12 B/cell each, T/k of them,
+// against the bracket store's 4 B/cell per step, k of them), even
+int grad_auto_interval(int T) {
+  int k = (int)std::lround(std::sqrt(3.0 * T) / 2.0) * 2;
+  return std::max(2, std::min(k, T));
+}
+
+int make_grad_batch(const gbw_acoustic2d_problem* probs, int n, GradLayout& L) {
+  if (!probs || n <= 0) return fail(GBW_ERR_INVALID, "batched gradient needs n > 0 problems");
+  L = GradLayout();
+  L.n = n;
+  L.order.resize(n);
+  for (int i = 0; i < n; ++i) L.order[i] = i;
+  std::stable_sort(L.order.begin(), L.order.end(),
+                   [&](int x, int y) { return probs[x].nt > probs[y].nt; });
+  int max_nt = 0;
+  for (int i = 0; i < n; ++i) {
+    if (probs[i].nt <= 0) return fail(GBW_ERR_INVALID, "nt must be positive");
+    max_nt = std::max(max_nt, probs[i].nt);
+  }
+  L.T = max_nt + (max_nt & 1);
+  const int kreq = probs[0].checkpoint_every;
+  for (int i = 1; i < n; ++i)
+    if (probs[i].checkpoint_every != kreq)
+      return fail(GBW_ERR_INVALID, "batched gradient: checkpoint_every must agree across problems");
+  if (kreq < 0 || (kreq & 1))
+    return fail(GBW_ERR_INVALID, "batched gradient: checkpoint_every must be even (0 = automatic)");
+  L.k = kreq > 0 ? std::min(kreq, L.T) : grad_auto_interval(L.T);
+  L.nseg = (L.T + L.k - 1) / L.k;
+  L.TZ2 = probs[0].tile_rows == 24 ? 24 : 16;
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o += align256(bytes); return at; };
+  L.off_desc = take((size_t)n * sizeof(GradDesc));
+  L.off_units = take((size_t)(n + 1) * sizeof(int));
+  L.unit_start.assign(n + 1, 0);
+  for (int j = 0; j < n; ++j) {
+    const gbw_acoustic2d_problem& p = probs[L.order[j]];
+    const std::string where = "batched gradient, problem " + std::to_string(L.order[j]) + ": ";
+    if (p.snapshot_every != 0 || p.checkpoint_bf16 != 0)
+      return fail(GBW_ERR_UNSUPPORTED, where + "snapshots and bf16 checkpoints are not supported");
+    if (p.steps_per_launch == 1)
+      return fail(GBW_ERR_UNSUPPORTED, where + "runs two time steps per launch (steps_per_launch 0 or 2)");
+    if ((p.tile_rows == 24 ? 24 : 16) != L.TZ2)
+      return fail(GBW_ERR_INVALID, where + "tile_rows must agree across problems");
+    // the plan of a single gradient call with this interval (the bracket store
+    // holds k steps even when the problem is shorter: it is indexed by global step)
+    // (default: all shots of a problem in one workgroup, the batch supplies the parallelism)
+    gbw_acoustic2d_problem q = p;
+    q.checkpoint_every = L.k;
+    q.nt = std::max(p.nt, L.k);
+    q.shots_per_block = p.shots_per_block > 0 ? p.shots_per_block : std::max(1, p.src_n);
+    Plan pl;
+    int rc = make_plan(&q, 1, pl);
+    if (rc) {
+      g_last_error = where + g_last_error;
+      return rc;
+    }
+    pl.nt = p.nt;
+    const int tiles2 = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
+    L.unit_start[j + 1] = L.unit_start[j] + tiles2 * pl.groups;
+    const int off = L.T - p.nt;
+    L.off.push_back(off);
+    L.c0.push_back(off / L.k);
+    L.off_prob.push_back(take(pl.total));
+    L.off_D.push_back(pl.off_D);  // make_plan sized it for k steps (nt >= k above)
+    L.off_slots.push_back(take((size_t)(L.nseg - off / L.k) * pl.slot_bytes));
+    L.plans.push_back(pl);
+  }
+  L.total = o;
+  return 0;
+}
+
+unsigned grad_prep_blocks(const GradLayout& L) {
+  long long most = 0;
+  for (const Plan& pl : L.plans) most = std::max(most, 3LL * pl.S * pl.d.plane);
+  return (unsigned)std::max<long long>(1, std::min<long long>((most + 255) / 256, 2048));
+}
+
+template <bool RECOMP>
+int grad_batch_step(const GradLayout& L, const GradDesc* descs, const int* units, int n_active,
+                    int g, int j, int c, hipStream_t st) {
+  const unsigned gx = (unsigned)((L.unit_start[n_active] + 7) / 8 * 8);
+  LaunchTimer timer(RECOMP ? "grad_batch_step_kernel<recompute>" : "grad_batch_step_kernel<forward>", st);
+  if (L.TZ2 == 24)
+    grad_batch_step_kernel<24, 96, RECOMP><<<gx, FWD2_THREADS, 0, st>>>(descs, units, n_active, g, j, c, L.k);
+  else
+    grad_batch_step_kernel<16, 128, RECOMP><<<gx, FWD2_THREADS, 0, st>>>(descs, units, n_active, g, j, c, L.k);
+  LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace
 
 // Workspace of gbw_acoustic2d_forward_noalloc: one array per GPU, in the
@@ -3017,6 +3346,176 @@ int gbw_acoustic2d_batch_forward(const gbw_acoustic2d_problem* probs, int n, voi
                                                              L.single, counters, done_ticks);
     LAUNCH_CHECK();
     parity ^= 1;
+  }
+  return flush_timing(st);
+}
+
+size_t gbw_acoustic2d_batch_gradient_workspace_bytes(const gbw_acoustic2d_problem* probs, int n) {
+  GradLayout L;
+  if (make_grad_batch(probs, n, L)) return 0;
+  return L.total;
+}
+
+int gbw_acoustic2d_batch_gradient_setup(const gbw_acoustic2d_problem* probs,
+                                        const gbw_acoustic2d_inputs* ins,
+                                        const gbw_acoustic2d_outputs* outs,
+                                        const gbw_acoustic2d_adjoint* adjs, int n,
+                                        void* workspace, size_t workspace_bytes, void* stream) {
+  GradLayout L;
+  int rc = make_grad_batch(probs, n, L);
+  if (rc) return rc;
+  if (!ins || !outs || !adjs) return fail(GBW_ERR_INVALID, "inputs / outputs / adjoints are NULL");
+  if (!workspace || workspace_bytes < L.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<GradDesc> descs(n);
+  for (int j = 0; j < n; ++j) {
+    const int o = L.order[j];
+    const Plan& pl = L.plans[j];
+    void* wsk = static_cast<char*>(workspace) + L.off_prob[j];
+    const gbw_acoustic2d_outputs& out = outs[o];
+    const gbw_acoustic2d_adjoint& adj = adjs[o];
+    const std::string where = "problem " + std::to_string(o) + ": ";
+    if (out.energy_p || out.energy_vx || out.energy_vz || out.snapshots)
+      return fail(GBW_ERR_UNSUPPORTED, "batched gradient returns seismograms only (" + where + ")");
+    const bool rec = out.rec_p || out.rec_vx || out.rec_vz;
+    if (rec && !(out.rec_p && out.rec_vx && out.rec_vz))
+      return fail(GBW_ERR_INVALID, where + "rec_p, rec_vx and rec_vz must be given together");
+    if (adj.grad_rho)
+      return fail(GBW_ERR_UNSUPPORTED, where + "the batched gradient computes vp and src_v gradients");
+    std::vector<int> rcell;
+    if ((rc = setup_geometry(pl, &ins[o], wsk, st, rcell))) {
+      g_last_error = where + g_last_error;
+      return rc;
+    }
+    if ((rc = build_coefs(pl, &probs[o], &ins[o], wsk, st))) return rc;
+    GradDesc& g = descs[j];
+    g = GradDesc{};
+    g.f = base_fwd_args(pl, &probs[o], &ins[o], wsk);
+    g.f.spb = pl.spb;
+    g.f.tiles_x = pl.tiles_x2;
+    g.f.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
+    g.f.rec_p = rec ? out.rec_p : nullptr;
+    g.f.rec_vx = rec ? out.rec_vx : nullptr;
+    g.f.rec_vz = rec ? out.rec_vz : nullptr;
+    g.f.energy = nullptr;
+    AdjArgs& b = g.b;
+    b.d = pl.d; b.S = pl.S; b.spb = pl.spb; b.tiles_x = g.f.tiles_x; b.ntiles = g.f.ntiles;
+    b.coef = g.f.coef; b.src_cell = g.f.src_cell; b.nt = pl.nt; b.dt = probs[o].dt;
+    b.rhead = g.f.rhead; b.rnext = g.f.rnext; b.rrow = g.f.rrow; b.R = pl.R;
+    b.g_p = adj.g_p; b.g_vx = adj.g_vx; b.g_vz = adj.g_vz;
+    g.A = at<float>(wsk, pl.off_A);
+    g.B = at<float>(wsk, pl.off_B);
+    g.LA = at<float>(wsk, pl.off_LA);
+    g.LB = at<float>(wsk, pl.off_LB);
+    g.D = at<float>(wsk, L.off_D[j]);
+    g.slots = static_cast<char*>(workspace) + L.off_slots[j];
+    g.G1 = at<float>(wsk, pl.off_G1);
+    g.GW = at<float>(wsk, pl.off_GW);
+    b.gA1 = g.G1; b.gA2 = nullptr; b.gw = g.GW;
+    g.rcell = at<int>(wsk, pl.off_rcell);
+    g.vp = ins[o].vp; g.rho = ins[o].rho;
+    g.grad_vp = adj.grad_vp; g.grad_w = adj.grad_src_v;
+    g.dz = probs[o].dz;
+    g.shared = probs[o].wavelet_shared;
+    g.fstride = (long long)pl.S * pl.d.plane;
+    g.slot_bytes = (long long)pl.slot_bytes;
+    g.off = L.off[j];
+    g.c0 = L.c0[j];
+    g.groups = pl.groups;
+  }
+  HIP_TRY(hipMemcpyAsync(at<GradDesc>(workspace, L.off_desc), descs.data(),
+                         descs.size() * sizeof(GradDesc), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(at<int>(workspace, L.off_units), L.unit_start.data(),
+                         L.unit_start.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host vectors go out of scope
+  return 0;
+}
+
+int gbw_acoustic2d_batch_gradient_forward(const gbw_acoustic2d_problem* probs, int n,
+                                          void* workspace, size_t workspace_bytes,
+                                          unsigned long long* done_ticks, void* stream) {
+  GradLayout L;
+  int rc = make_grad_batch(probs, n, L);
+  if (rc) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const GradDesc* descs = at<GradDesc>(workspace, L.off_desc);
+  const int* units = at<int>(workspace, L.off_units);
+  if (done_ticks) {
+    stamp_kernel<<<1, 64, 0, st>>>(done_ticks);
+    LAUNCH_CHECK();
+  }
+  const unsigned pb = grad_prep_blocks(L);
+  grad_batch_prep_kernel<<<dim3(pb, n), 256, 0, st>>>(descs, 0, 0, 0);
+  LAUNCH_CHECK();
+  for (int g = 0, j = 0; g < L.T; g += 2, ++j) {
+    const int na = L.active(g + 1);
+    if ((rc = grad_batch_step<false>(L, descs, units, na, g, j, 0, st))) return rc;
+    if ((g + 2) % L.k == 0 && g + 2 < L.T) {  // checkpoint: launch j wrote A (j odd) or B
+      grad_batch_prep_kernel<<<dim3(pb, na), 256, 0, st>>>(descs, 3, (g + 2) / L.k, (j & 1) ? 0 : 1);
+      LAUNCH_CHECK();
+    }
+  }
+  return flush_timing(st);
+}
+
+int gbw_acoustic2d_batch_gradient_backward(const gbw_acoustic2d_problem* probs, int n,
+                                           void* workspace, size_t workspace_bytes,
+                                           unsigned long long* done_ticks, void* stream) {
+  GradLayout L;
+  int rc = make_grad_batch(probs, n, L);
+  if (rc) return rc;
+  if (!workspace || workspace_bytes < L.total)
+    return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(L.total) + " bytes");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const GradDesc* descs = at<GradDesc>(workspace, L.off_desc);
+  const int* units = at<int>(workspace, L.off_units);
+  const unsigned pb = grad_prep_blocks(L);
+  grad_batch_prep_kernel<<<dim3(pb, n), 256, 0, st>>>(descs, 2, 0, 0);
+  LAUNCH_CHECK();
+  int most_sr = 1;
+  for (const Plan& pl : L.plans) most_sr = std::max(most_sr, pl.S * pl.R);
+  grad_batch_inject_kernel<<<dim3((most_sr + 255) / 256, n), 256, 0, st>>>(descs);
+  LAUNCH_CHECK();
+  int parity = 0;  // the adjoint state at the last step is in LA
+  for (int c = L.nseg - 1; c >= 0; --c) {
+    const int lo = c * L.k, hi = std::min(L.T, lo + L.k);
+    const int n_seg = L.active(hi - 1);
+    grad_batch_prep_kernel<<<dim3(pb, n_seg), 256, 0, st>>>(descs, 1, c, 0);
+    LAUNCH_CHECK();
+    for (int g = lo, j = 0; g < hi; g += 2, ++j)
+      if ((rc = grad_batch_step<true>(L, descs, units, L.active(g + 1), g, j, c, st))) return rc;
+    for (int g = hi - 1; g > lo; g -= 2) {
+      const int na = L.active(g);
+      {
+        const unsigned gx = (unsigned)((L.unit_start[na] + 7) / 8 * 8);
+        AdjArgs ba{};
+        ba.batch_descs = descs; ba.batch_units = units; ba.batch_n = na; ba.batch_g = g;
+        ba.batch_parity = parity; ba.batch_c = c; ba.batch_k = L.k;
+        LaunchTimer timer("adj2_step_kernel<batch>", st);
+        if (L.TZ2 == 24)
+          adj2_step_kernel<24, 96, FWD2_THREADS, false, true><<<gx, FWD2_THREADS, 0, st>>>(ba);
+        else
+          adj2_step_kernel<16, 128, FWD2_THREADS, false, true><<<gx, FWD2_THREADS, 0, st>>>(ba);
+        LAUNCH_CHECK();
+      }
+      parity ^= 1;
+      // problems whose first step was g or g-1 are done: chain rule, wavelet gradient, stamp
+      for (int j = L.active(g - 2); j < na; ++j) {
+        const Plan& pl = L.plans[j];
+        grad_batch_fold_kernel<<<dim3((pl.d.nx + 127) / 128, pl.d.nz), 128, 0, st>>>(descs, j);
+        LAUNCH_CHECK();
+        const int nw = pl.S * pl.nt;
+        grad_batch_wgrad_kernel<<<(nw + 255) / 256, 256, 0, st>>>(descs, j);
+        LAUNCH_CHECK();
+        if (done_ticks) {
+          stamp_kernel<<<1, 64, 0, st>>>(done_ticks + 1 + j);
+          LAUNCH_CHECK();
+        }
+      }
+    }
   }
   return flush_timing(st);
 }

@@ -190,6 +190,51 @@ int gbw_acoustic2d_batch_forward(const gbw_acoustic2d_problem* probs, int n, voi
                                  void* stream /* hipStream_t */);
 int gbw_device_wall_clock_khz(void);
 
+/* Batched gradient of n independent problems (uneven grids, step counts and
+ * shot counts): the forward with checkpoints, then the recompute and adjoint,
+ * of all n problems as one launch sequence each.  No single counterpart in the
+ * reference, which differentiates one acoustic_forward_kernel call at a time
+ * with PyTorch autograd through torch.utils.checkpoint
+ * (acoustic_2d.py:448-460); this runs n such gradients together.  Problems are
+ * aligned at their last time step: every problem's checkpoint intervals then
+ * coincide (checkpoint_every, equal for all problems and even, 0 = automatic),
+ * and each launch advances every problem that has started by two steps.
+ * Each problem's results equal its own gbw_acoustic2d_forward /
+ * gbw_acoustic2d_backward up to the order of FP32 sums (bit for bit for the
+ * seismograms).
+ *  - gbw_acoustic2d_batch_gradient_setup validates the geometry (IndexError
+ *    semantics), builds coefficients and receiver maps and stores the output
+ *    and adjoint pointers: outs[i] takes seismograms only (energy maps and
+ *    snapshots NULL); adjs[i] gives the adjoint sources g_p / g_vx / g_vz (any
+ *    may be NULL) and the outputs grad_vp and grad_src_v (either may be NULL;
+ *    grad_rho must be NULL; checkpoints is ignored: they live in the
+ *    workspace).  It synchronises `stream`.  Call it again after changing any
+ *    input or pointer.
+ *  - gbw_acoustic2d_batch_gradient_forward runs the forward pass (seismograms
+ *    into outs, checkpoints into the workspace); _backward reads the adjoint
+ *    sources and writes the gradients.  Both only enqueue kernels (no
+ *    allocation, copy or synchronisation), so forward, the caller's misfit and
+ *    backward can be captured in one HIP graph.  With done_ticks (DEVICE,
+ *    n + 1 entries) forward writes the device wall clock at its start into
+ *    done_ticks[0] and backward writes it into done_ticks[1 + i] when problem i's
+ *    gradient is complete, i its position in the list sorted by nt, longest
+ *    first (ties keep the caller's order). */
+size_t gbw_acoustic2d_batch_gradient_workspace_bytes(const gbw_acoustic2d_problem* probs, int n);
+int gbw_acoustic2d_batch_gradient_setup(const gbw_acoustic2d_problem* probs,
+                                        const gbw_acoustic2d_inputs* ins,
+                                        const gbw_acoustic2d_outputs* outs,
+                                        const gbw_acoustic2d_adjoint* adjs, int n,
+                                        void* workspace, size_t workspace_bytes,
+                                        void* stream /* hipStream_t */);
+int gbw_acoustic2d_batch_gradient_forward(const gbw_acoustic2d_problem* probs, int n,
+                                          void* workspace, size_t workspace_bytes,
+                                          unsigned long long* done_ticks,
+                                          void* stream /* hipStream_t */);
+int gbw_acoustic2d_batch_gradient_backward(const gbw_acoustic2d_problem* probs, int n,
+                                           void* workspace, size_t workspace_bytes,
+                                           unsigned long long* done_ticks,
+                                           void* stream /* hipStream_t */);
+
 /* Measurement hook, no counterpart in the reference (for benchmarks and
  * profilers that need per-kernel times from a live run).  While enabled, each
  * time-step kernel launch is bracketed by two events on the caller's stream

@@ -11,7 +11,9 @@ entry points the benchmark times.
   directional derivative <grad J, dm>.
 * 96 uneven problems from seed 7, forward, one HIP graph: every problem's
   seismograms bit-equal to its own acoustic_forward_kernel call (eager, and on
-  two graph replays), the two smallest also bit-equal to the FP32 oracle.
+  two graph replays), the two smallest also bit-equal to the FP32 oracle; and
+  their batched gradient against each problem's own gradient and, for the two
+  smallest, the FP32 oracle's.
 * 128 shots, 400 x 2000, 5000 steps: one gradient with FP32 and one with BF16
   checkpoint storage; forward outputs identical, BF16-vs-FP32 gradient within
   the bound the 8-bit significand implies.
@@ -227,6 +229,57 @@ def test_config4_uneven_batch_full_size():
             np.testing.assert_array_equal(_np(eager[i][k]), ref[k], err_msg=f"problem {i} {k}")
         print(f"  problem {i} ({specs[i]['nz']}x{specs[i]['nx']}, nt {specs[i]['nt']}, "
               f"{specs[i]['n_shots']} shots) bit-equal to the FP32 oracle")
+
+
+def test_config4_uneven_batch_gradient_full_size():
+    """The 96 problems from seed 7 in one BatchedGradient (forward with
+    checkpoints, recompute, adjoint), seeded random adjoint sources: every
+    problem's vp gradient equals autograd through its own
+    acoustic_forward_kernel call with the same adjoint sources (rel-L2 <= 1e-6;
+    bit for bit with the same shot grouping), and the two smallest problems'
+    gradients are within 1e-5 of the FP32 oracle's."""
+    from geobrain_amd.physics.wave import kernels as K
+    print("[case] config 4 full size: 96 uneven problems, batched gradient vs single calls",
+          flush=True)
+    specs = synthetic.uneven_batch(96, seed=7)
+    problems = [B.problem_inputs(sp, DEV) for sp in specs]
+    gen = torch.Generator(device=DEV).manual_seed(7)
+    t0 = time.perf_counter()
+    bg = B.BatchedGradient(problems)
+    bg.forward()
+    for adj in bg.adjoint:
+        adj["p"].copy_(torch.randn(adj["p"].shape, generator=gen, device=DEV))
+    bg.backward()
+    torch.cuda.synchronize()
+    print(f"  batched gradient {time.perf_counter() - t0:.2f} s ({bg.cells:.3e} cell-updates)")
+    worst, equal = 0.0, 0
+    for i, a in enumerate(problems):
+        K.set_launch_choice(K.LaunchChoice(bwd_shots_per_block=a["src_n"]))
+        try:
+            vp = a["vp"].detach().clone().requires_grad_(True)
+            out = wave.acoustic_forward_kernel(**dict(a, vp=vp))
+            assert torch.equal(out["p"], bg.outputs[i]["p"]), f"problem {i} seismogram"
+            (bg.adjoint[i]["p"] * out["p"]).sum().backward()
+        finally:
+            K.set_launch_choice(None)
+        e = rel_l2(_np(bg.grad_vp[i]), _np(vp.grad))
+        worst = max(worst, e)
+        equal += int(torch.equal(bg.grad_vp[i], vp.grad))
+        assert float(vp.grad.abs().max()) > 0 and e <= 1e-6, (i, e)
+        del out, vp
+    print(f"  every problem's gradient vs its single call: worst rel-L2 {worst:.1e}, "
+          f"{equal} of 96 bit-equal")
+    cost = [sp["n_shots"] * (sp["nz"] + 40) * (sp["nx"] + 40) * sp["nt"] for sp in specs]
+    for i in np.argsort(cost)[:2]:
+        a = problems[i]
+        gv, _, _ = oracle.gradient(nz=a["nz"], nx=a["nx"], nabc=a["nabc"], free_surface=False,
+                                   dt=a["dt"], dz=a["dz"], src_z=a["src_z"], src_x=a["src_x"],
+                                   src_v=_np(a["src_v"]), rcv_z=a["rcv_z"], rcv_x=a["rcv_x"],
+                                   vp=_np(a["vp"]), rho=_np(a["rho"]), damp=_np(a["damp"]),
+                                   g_p=_np(bg.adjoint[i]["p"]), chunk=50, precision="f32")
+        e = rel_l2(_np(bg.grad_vp[i]), gv)
+        print(f"  problem {i}: grad_vp rel-L2 vs the FP32 oracle {e:.1e}")
+        assert e <= 1e-5
 
 
 def test_config5_bf16_vs_fp32_checkpoints_full_size():
