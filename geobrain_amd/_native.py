@@ -18,7 +18,7 @@ c_i32 = ctypes.c_int32
 c_f32 = ctypes.c_float
 c_vp = ctypes.c_void_p
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 # size of the device array behind gbw_acoustic2d_forward_noalloc (GBW_NOALLOC_SCRATCH_BYTES)
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -29,6 +29,8 @@ EXPORTS = (
     "gbw_acoustic2d_checkpoint_bytes",
     "gbw_acoustic2d_forward",
     "gbw_acoustic2d_forward_noalloc",
+    "gbw_acoustic2d_geometry_bytes",
+    "gbw_acoustic2d_geometry_setup",
     "gbw_acoustic2d_backward",
     "gbw_set_kernel_timing",
     "gbw_kernel_times",
@@ -63,7 +65,7 @@ class Inputs(ctypes.Structure):
     """``gbw_acoustic2d_inputs``."""
     _fields_ = [
         ("src_x", c_vp), ("src_z", c_vp), ("rcv_x", c_vp), ("rcv_z", c_vp),
-        ("src_v", c_vp), ("damp", c_vp), ("vp", c_vp), ("rho", c_vp),
+        ("src_v", c_vp), ("damp", c_vp), ("vp", c_vp), ("rho", c_vp), ("geometry", c_vp),
     ]
 
 
@@ -159,6 +161,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         lib.gbw_acoustic2d_forward.argtypes = [
             ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs),
             c_vp, ctypes.c_size_t, c_vp]
+        lib.gbw_acoustic2d_geometry_bytes.restype = ctypes.c_size_t
+        lib.gbw_acoustic2d_geometry_bytes.argtypes = [ctypes.POINTER(Problem)]
+        lib.gbw_acoustic2d_geometry_setup.restype = ctypes.c_int
+        lib.gbw_acoustic2d_geometry_setup.argtypes = [
+            ctypes.POINTER(Problem), ctypes.POINTER(Inputs), c_vp, ctypes.c_size_t, c_vp]
         lib.gbw_acoustic2d_forward_noalloc.restype = ctypes.c_int
         lib.gbw_acoustic2d_forward_noalloc.argtypes = [
             ctypes.POINTER(Problem), ctypes.POINTER(Inputs), ctypes.POINTER(Outputs), c_vp]

@@ -2491,7 +2491,11 @@ struct Plan {
   size_t plane_bytes, slot_bytes, slot_scale_off;
   // byte offsets into the workspace
   size_t off_coef, off_A, off_B, off_LA, off_LB, off_D, off_GX, off_GZ, off_E, off_G1, off_G2,
-      off_GW, off_rhead, off_rnext, off_rrow, off_rcell, off_heads, off_src, off_MX, total;
+      off_GW, off_geo, off_MX, total;
+  // geometry region (receiver maps, source cells): offsets from its start; it
+  // lies in the workspace at off_geo, or in a caller's geometry buffer
+  // (gbw_acoustic2d_geometry_setup)
+  size_t off_rhead, off_rnext, off_rrow, off_rcell, off_heads, off_src, geo_total;
 };
 
 int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
@@ -2584,12 +2588,18 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
     pl.off_G1 = take((size_t)pl.groups * pl.plane_bytes);
     pl.off_GW = take((size_t)pl.S * pl.nt * sizeof(float));
   }
-  pl.off_rhead = take((size_t)d.plane * sizeof(int));
-  pl.off_rnext = take((size_t)std::max(1, pl.R) * sizeof(int));
-  pl.off_rcell = take((size_t)std::max(1, pl.R) * sizeof(int));
-  pl.off_heads = take(2 * (size_t)std::max(1, pl.R) * sizeof(int));
-  pl.off_rrow = take((size_t)d.NZ);
-  pl.off_src = take((size_t)pl.S * sizeof(int));
+  {
+    size_t g = 0;
+    auto gtake = [&](size_t bytes) { size_t at = g; g += align256(bytes); return at; };
+    pl.off_rhead = gtake((size_t)d.plane * sizeof(int));
+    pl.off_rnext = gtake((size_t)std::max(1, pl.R) * sizeof(int));
+    pl.off_rcell = gtake((size_t)std::max(1, pl.R) * sizeof(int));
+    pl.off_heads = gtake(2 * (size_t)std::max(1, pl.R) * sizeof(int));
+    pl.off_rrow = gtake((size_t)d.NZ);
+    pl.off_src = gtake((size_t)pl.S * sizeof(int));
+    pl.geo_total = g;
+  }
+  pl.off_geo = take(pl.geo_total);
   pl.off_MX = take(3 * (size_t)pl.S * sizeof(unsigned));
   pl.total = o;
   return 0;
@@ -2639,6 +2649,7 @@ int fill(float* p, long long n, float v, hipStream_t st) {
 // indexing) and uploads source cells and the receiver map.
 int setup_geometry(const Plan& pl, const gbw_acoustic2d_inputs* in, void* ws, hipStream_t st,
                    std::vector<int>& host_rcell) {
+  // ws: the start of the geometry region
   const Dims& d = pl.d;
   if (!in || !in->src_x || !in->src_z) return fail(GBW_ERR_INVALID, "source positions are NULL");
   if (pl.R > 0 && (!in->rcv_x || !in->rcv_z)) return fail(GBW_ERR_INVALID, "receiver positions are NULL");
@@ -2775,19 +2786,61 @@ void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
     adj_step_kernel<TZ, false><<<grid, NTHREADS, 0, st>>>(a);
 }
 
+// Prepared geometry buffers (gbw_acoustic2d_geometry_setup), by device
+// address, with the shape each was prepared for: a call handed one checks its
+// own shape against it before any kernel reads the buffer.
+struct GeoKey {
+  int NZ, NX, PX, nabc, fs, S, R;
+  bool operator==(const GeoKey& o) const {
+    return NZ == o.NZ && NX == o.NX && PX == o.PX && nabc == o.nabc && fs == o.fs && S == o.S &&
+           R == o.R;
+  }
+};
+std::mutex g_geo_mu;
+std::unordered_map<const void*, GeoKey> g_geo;
+
+GeoKey geo_key(const Plan& pl) {
+  return GeoKey{pl.d.NZ, pl.d.NX, pl.d.PX, pl.d.nabc, pl.d.fs, pl.S, pl.R};
+}
+
+// the geometry region of a call: the caller's prepared buffer, or the
+// workspace's, built now from the positions (which synchronises `st`)
+int resolve_geometry(const Plan& pl, const gbw_acoustic2d_inputs* in, void* ws, hipStream_t st,
+                     const void** geo) {
+  if (!in) return fail(GBW_ERR_INVALID, "inputs is NULL");
+  if (in->geometry) {
+    std::lock_guard<std::mutex> lk(g_geo_mu);
+    auto it = g_geo.find(in->geometry);
+    if (it == g_geo.end() || !(it->second == geo_key(pl)))
+      return fail(GBW_ERR_INVALID,
+                  "geometry buffer was not prepared by gbw_acoustic2d_geometry_setup for this "
+                  "problem's grid, shots and receivers");
+    *geo = in->geometry;
+    return 0;
+  }
+  void* g = static_cast<char*>(ws) + pl.off_geo;
+  std::vector<int> rcell;
+  int rc = setup_geometry(pl, in, g, st, rcell);
+  if (rc) return rc;
+  *geo = g;
+  return 0;
+}
+
+// geo: the geometry region (in the workspace or a caller's geometry buffer)
 FwdArgs base_fwd_args(const Plan& pl, const gbw_acoustic2d_problem* p,
-                      const gbw_acoustic2d_inputs* in, void* ws) {
+                      const gbw_acoustic2d_inputs* in, void* ws, const void* geo) {
+  void* g = const_cast<void*>(geo);
   FwdArgs a{};
   a.d = pl.d; a.S = pl.S; a.spb = pl.spb; a.tiles_x = pl.tiles_x;
   a.ntiles = pl.tiles_x * pl.tiles_z;
   a.coef = at<float>(ws, pl.off_coef);
-  a.src_cell = at<int>(ws, pl.off_src);
+  a.src_cell = at<int>(g, pl.off_src);
   a.wav = in->src_v;
   a.wav_stride = p->wavelet_shared ? 0 : p->nt;
   a.nt = p->nt; a.dt = p->dt;
-  a.rhead = at<int>(ws, pl.off_rhead);
-  a.rnext = at<int>(ws, pl.off_rnext);
-  a.rrow = at<unsigned char>(ws, pl.off_rrow);
+  a.rhead = at<int>(g, pl.off_rhead);
+  a.rnext = at<int>(g, pl.off_rnext);
+  a.rrow = at<unsigned char>(g, pl.off_rrow);
   a.R = pl.R;
   return a;
 }
@@ -3036,8 +3089,8 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   if (rec && !(out->rec_p && out->rec_vx && out->rec_vz))
     return fail(GBW_ERR_INVALID, "rec_p, rec_vx and rec_vz must be given together");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  std::vector<int> rcell;
-  if ((rc = setup_geometry(pl, in, workspace, st, rcell))) return rc;
+  const void* geo = nullptr;
+  if ((rc = resolve_geometry(pl, in, workspace, st, &geo))) return rc;
   if ((rc = build_coefs(pl, prob, in, workspace, st))) return rc;
   const Dims& d = pl.d;
   const long long fstride = (long long)pl.S * d.plane;
@@ -3061,7 +3114,7 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   float* cur = ck32 ? reinterpret_cast<float*>(ck) : A;
   if ((rc = fill(cur, slot, 0.f, st))) return rc;
   if (pl.k && pl.bf16 && (rc = quantize_state(pl, cur, ck, MX, st))) return rc;
-  FwdArgs a = base_fwd_args(pl, prob, in, workspace);
+  FwdArgs a = base_fwd_args(pl, prob, in, workspace, geo);
   a.rec_p = out->rec_p; a.rec_vx = out->rec_vx; a.rec_vz = out->rec_vz;
   if (!rec) a.rec_p = a.rec_vx = a.rec_vz = nullptr;
   a.energy = energy ? E : nullptr;
@@ -3107,6 +3160,33 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   return flush_timing(st);
 }
 
+size_t gbw_acoustic2d_geometry_bytes(const gbw_acoustic2d_problem* prob) {
+  Plan pl;
+  if (make_plan(prob, 0, pl)) return 0;
+  return pl.geo_total;
+}
+
+int gbw_acoustic2d_geometry_setup(const gbw_acoustic2d_problem* prob,
+                                  const gbw_acoustic2d_inputs* in, void* geometry,
+                                  size_t geometry_bytes, void* stream) {
+  Plan pl;
+  int rc = make_plan(prob, 0, pl);
+  if (rc) return rc;
+  if (!in) return fail(GBW_ERR_INVALID, "inputs is NULL");
+  if (!geometry || geometry_bytes < pl.geo_total)
+    return fail(GBW_ERR_WORKSPACE,
+                "geometry buffer too small: need " + std::to_string(pl.geo_total) + " bytes");
+  {  // a failed setup leaves no stale entry behind
+    std::lock_guard<std::mutex> lk(g_geo_mu);
+    g_geo.erase(geometry);
+  }
+  std::vector<int> rcell;
+  if ((rc = setup_geometry(pl, in, geometry, static_cast<hipStream_t>(stream), rcell))) return rc;
+  std::lock_guard<std::mutex> lk(g_geo_mu);
+  g_geo[geometry] = geo_key(pl);
+  return 0;
+}
+
 int gbw_acoustic2d_forward_noalloc(const gbw_acoustic2d_problem* prob,
                                    const gbw_acoustic2d_inputs* in,
                                    const gbw_acoustic2d_outputs* out, void* stream) {
@@ -3139,8 +3219,9 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
     return fail(GBW_ERR_WORKSPACE, "workspace too small: need " + std::to_string(pl.total) + " bytes");
   if (!adj->checkpoints) return fail(GBW_ERR_INVALID, "checkpoint buffer is NULL");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  std::vector<int> rcell;
-  if ((rc = setup_geometry(pl, in, workspace, st, rcell))) return rc;
+  const void* geo = nullptr;
+  if ((rc = resolve_geometry(pl, in, workspace, st, &geo))) return rc;
+  void* geom = const_cast<void*>(geo);
   if ((rc = build_coefs(pl, prob, in, workspace, st))) return rc;
   const Dims& d = pl.d;
   const long long fstride = (long long)pl.S * d.plane;
@@ -3163,12 +3244,12 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   const bool any_g = adj->g_p || adj->g_vx || adj->g_vz;
   if (any_g && pl.R > 0) {
     adj_inject_kernel<<<(pl.S * pl.R + 255) / 256, 256, 0, st>>>(
-        d, pl.S, pl.nt, pl.nt - 1, pl.R, at<int>(workspace, pl.off_rcell),
-        at<int>(workspace, pl.off_rhead), at<int>(workspace, pl.off_rnext), adj->g_p, adj->g_vx,
+        d, pl.S, pl.nt, pl.nt - 1, pl.R, at<int>(geom, pl.off_rcell),
+        at<int>(geom, pl.off_rhead), at<int>(geom, pl.off_rnext), adj->g_p, adj->g_vx,
         adj->g_vz, LA, LA + fstride, LA + 2 * fstride);
     LAUNCH_CHECK();
   }
-  FwdArgs f = base_fwd_args(pl, prob, in, workspace);
+  FwdArgs f = base_fwd_args(pl, prob, in, workspace, geo);
   AdjArgs b{};
   b.d = d; b.S = pl.S; b.spb = pl.spb; b.tiles_x = pl.tiles_x; b.ntiles = f.ntiles;
   b.coef = f.coef; b.src_cell = f.src_cell; b.nt = pl.nt; b.dt = prob->dt;
@@ -3283,14 +3364,15 @@ int gbw_acoustic2d_batch_setup(const gbw_acoustic2d_problem* probs,
     if (rec && !(out.rec_p && out.rec_vx && out.rec_vz))
       return fail(GBW_ERR_INVALID, "rec_p, rec_vx and rec_vz must be given together");
     std::vector<int> rcell;
-    if ((rc = setup_geometry(pl, &ins[o], wsk, st, rcell))) {
+    void* geo = static_cast<char*>(wsk) + pl.off_geo;
+    if ((rc = setup_geometry(pl, &ins[o], geo, st, rcell))) {
       g_last_error = "problem " + std::to_string(o) + ": " + g_last_error;
       return rc;
     }
     if ((rc = build_coefs(pl, &probs[o], &ins[o], wsk, st))) return rc;
     BatchDesc& b = descs[k];
     b = BatchDesc{};
-    b.a = base_fwd_args(pl, &probs[o], &ins[o], wsk);
+    b.a = base_fwd_args(pl, &probs[o], &ins[o], wsk, geo);
     b.a.spb = pl.spb;
     b.a.tiles_x = pl.tiles_x2;
     b.a.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
@@ -3384,14 +3466,15 @@ int gbw_acoustic2d_batch_gradient_setup(const gbw_acoustic2d_problem* probs,
     if (adj.grad_rho)
       return fail(GBW_ERR_UNSUPPORTED, where + "the batched gradient computes vp and src_v gradients");
     std::vector<int> rcell;
-    if ((rc = setup_geometry(pl, &ins[o], wsk, st, rcell))) {
+    void* geo = static_cast<char*>(wsk) + pl.off_geo;
+    if ((rc = setup_geometry(pl, &ins[o], geo, st, rcell))) {
       g_last_error = where + g_last_error;
       return rc;
     }
     if ((rc = build_coefs(pl, &probs[o], &ins[o], wsk, st))) return rc;
     GradDesc& g = descs[j];
     g = GradDesc{};
-    g.f = base_fwd_args(pl, &probs[o], &ins[o], wsk);
+    g.f = base_fwd_args(pl, &probs[o], &ins[o], wsk, geo);
     g.f.spb = pl.spb;
     g.f.tiles_x = pl.tiles_x2;
     g.f.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
@@ -3413,7 +3496,7 @@ int gbw_acoustic2d_batch_gradient_setup(const gbw_acoustic2d_problem* probs,
     g.G1 = at<float>(wsk, pl.off_G1);
     g.GW = at<float>(wsk, pl.off_GW);
     b.gA1 = g.G1; b.gA2 = nullptr; b.gw = g.GW;
-    g.rcell = at<int>(wsk, pl.off_rcell);
+    g.rcell = at<int>(geo, pl.off_rcell);
     g.vp = ins[o].vp; g.rho = ins[o].rho;
     g.grad_vp = adj.grad_vp; g.grad_w = adj.grad_src_v;
     g.dz = probs[o].dz;

@@ -23,6 +23,7 @@ missing library or a missing GPU raises ``RuntimeError``.
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import math
 import weakref
@@ -138,7 +139,46 @@ def _inputs(spec: _Spec, src_v: Tensor, damp: Tensor, vp: Tensor, rho: Tensor) -
     ins.damp = damp.data_ptr()
     ins.vp = vp.data_ptr()
     ins.rho = rho.data_ptr()
+    geo = spec.keep.get("geometry")
+    ins.geometry = geo.data_ptr() if geo is not None else None
     return ins
+
+
+# Prepared geometry buffers (source cells and receiver maps on the device,
+# gbw_acoustic2d_geometry_setup), keyed by the grid and the position values:
+# a call that repeats a geometry does no host work on it, so the forward and
+# gradient calls then only enqueue kernels and can be captured in a HIP graph.
+_GEOMETRY_CACHE_SIZE = 8
+_geometry_cache: "collections.OrderedDict[tuple, Tensor]" = collections.OrderedDict()
+
+
+def _geometry(prob: _native.Problem, sx, sz, rx, rz, device: torch.device) -> Tensor:
+    key = (str(device), prob.nz, prob.nx, prob.nabc, prob.free_surface, prob.src_n, prob.rcv_n,
+           sx.tobytes(), sz.tobytes(), rx.tobytes(), rz.tobytes())
+    hit = _geometry_cache.get(key)
+    if hit is not None:
+        _geometry_cache.move_to_end(key)
+        return hit
+    if torch.cuda.is_current_stream_capturing():
+        raise RuntimeError(
+            "acoustic_forward_kernel: this source/receiver geometry has not been prepared; "
+            "run the call once before capturing it in a graph")
+    lib = _native.load()
+    nbytes = lib.gbw_acoustic2d_geometry_bytes(ctypes.byref(prob))
+    if nbytes == 0:
+        _native.check(-1)
+    buf = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    ins = _native.Inputs()
+    ins.src_x, ins.src_z = sx.ctypes.data, sz.ctypes.data
+    ins.rcv_x = rx.ctypes.data if rx.size else None
+    ins.rcv_z = rz.ctypes.data if rz.size else None
+    _native.check(lib.gbw_acoustic2d_geometry_setup(ctypes.byref(prob), ctypes.byref(ins),
+                                                    buf.data_ptr(), nbytes,
+                                                    _stream_handle(device)))
+    _geometry_cache[key] = buf
+    while len(_geometry_cache) > _GEOMETRY_CACHE_SIZE:
+        _geometry_cache.popitem(last=False)
+    return buf
 
 
 class AcousticFunction(torch.autograd.Function):
@@ -227,6 +267,8 @@ class AcousticFunction(torch.autograd.Function):
 # holds a weak reference that both confirms identity and drops the entry when
 # the tensor is freed)
 _index_cache: Dict[int, tuple] = {}
+# device-to-host copies of position tensors made so far (the cache's misses)
+host_index_copies = 0
 
 
 def _host_index(t, name: str) -> np.ndarray:
@@ -236,6 +278,8 @@ def _host_index(t, name: str) -> np.ndarray:
             hit = _index_cache.get(id(t))
             if hit is not None and hit[0]() is t and hit[1] == stamp:
                 return hit[2]
+            global host_index_copies
+            host_index_copies += 1
             a = t.detach().to("cpu").numpy()
         else:
             a = t.detach().numpy()
@@ -375,6 +419,7 @@ def acoustic_forward_kernel(
         bwd_launch = (choice.bwd_tile_rows, choice.bwd_shots_per_block)
     spec = _Spec(prob=prob, src_x=sx, src_z=sz, rcv_x=rx, rcv_z=rz, n_snap=n_snap,
                  bwd_launch=bwd_launch)
+    spec.keep["geometry"] = _geometry(prob, sx, sz, rx, rz, dev)
 
     f32 = torch.float32
     p, vx, vz, ep, evx, evz, snaps = AcousticFunction.apply(

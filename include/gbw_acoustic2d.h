@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GBW_ABI_VERSION 3
+#define GBW_ABI_VERSION 4
 
 typedef enum {
   GBW_OK = 0,
@@ -75,7 +75,10 @@ typedef struct gbw_acoustic2d_problem {
 } gbw_acoustic2d_problem;
 
 /* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model
- * indices, as the reference's kernel receives them. */
+ * indices, as the reference's kernel receives them.  With `geometry` set (a
+ * buffer prepared by gbw_acoustic2d_geometry_setup), the positions are not
+ * read: the call uses the prepared source cells and receiver maps and does no
+ * host work, copy or synchronisation, so it can be captured in a HIP graph. */
 typedef struct gbw_acoustic2d_inputs {
   const int64_t* src_x;      /* HOST (src_n,)                                          */
   const int64_t* src_z;      /* HOST (src_n,)                                          */
@@ -85,6 +88,7 @@ typedef struct gbw_acoustic2d_inputs {
   const float* damp;         /* DEVICE (nz + 2 nabc, nx + 2 nabc)                      */
   const float* vp;           /* DEVICE (nz, nx)                                        */
   const float* rho;          /* DEVICE (nz, nx)                                        */
+  const void* geometry;      /* DEVICE prepared geometry buffer, or NULL               */
 } gbw_acoustic2d_inputs;
 
 /* Outputs of the forward pass (the dict of acoustic_2d.py:491-500).  Any
@@ -140,6 +144,22 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob,
                             const gbw_acoustic2d_adjoint* adj,
                             void* workspace, size_t workspace_bytes,
                             void* stream /* hipStream_t */);
+
+/* Source cells and receiver maps of a problem, prepared once for calls that
+ * repeat the same geometry (an FWI loop, graph capture): the part of
+ * acoustic_forward_kernel that indexes with the positions (acoustic_2d.py:
+ * 382-385, 228-237, 271-274; IndexError semantics as there).
+ * gbw_acoustic2d_geometry_bytes gives the buffer size (0 if the problem is
+ * invalid); gbw_acoustic2d_geometry_setup validates the positions in `in`,
+ * writes the buffer (DEVICE) and synchronises `stream`.  Calls then pass it as
+ * inputs.geometry; they check that their problem has the grid, absorbing
+ * width, free surface, shot and receiver counts it was prepared for.  The
+ * buffer must stay unchanged while calls use it; preparing it again (for new
+ * positions) replaces it. */
+size_t gbw_acoustic2d_geometry_bytes(const gbw_acoustic2d_problem* prob);
+int gbw_acoustic2d_geometry_setup(const gbw_acoustic2d_problem* prob,
+                                  const gbw_acoustic2d_inputs* in, void* geometry,
+                                  size_t geometry_bytes, void* stream /* hipStream_t */);
 
 /* Size of the device array behind gbw_acoustic2d_forward_noalloc (per GPU). */
 #define GBW_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)

@@ -32,7 +32,8 @@ def _case(name, checkpoint_every=0):
     p.checkpoint_every = checkpoint_every
     host = {k: np.ascontiguousarray(z["in_" + k], np.int64)
             for k in ("src_x", "src_z", "rcv_x", "rcv_z")}
-    dev = {k: torch.as_tensor(np.asarray(z["in_" + k], np.float32), device=DEV)
+    # row-major device copies (the fixtures store the damping array column-major)
+    dev = {k: torch.as_tensor(np.ascontiguousarray(z["in_" + k], np.float32), device=DEV)
            for k in ("vp", "rho", "damp")}
     dev["src_v"] = torch.as_tensor(w, device=DEV)
     return p, host, dev, (S, nt, R, m["nz"], m["nx"])
