@@ -198,11 +198,17 @@ def cpu_baseline(w, shots=8, steps=100, k=50):
                     checkpoint_segments=max(1, steps // k))
         with torch.no_grad():
             d_obs = eager_forward(vp=T(w["vp_obs"]), **args)["p"]
-        vp = T(w["vp"]).requires_grad_(True)
-        t0 = time.perf_counter()
-        out = eager_forward(vp=vp, **args)
-        ((out["p"] - d_obs) ** 2).sum().backward()
-        t = time.perf_counter() - t0
+
+        def once():
+            vp = T(w["vp"]).requires_grad_(True)
+            t0 = time.perf_counter()
+            out = eager_forward(vp=vp, **args)
+            ((out["p"] - d_obs) ** 2).sum().backward()
+            return time.perf_counter() - t0
+
+        once()  # warm-up: thread pool and first-touch allocation
+        runs = [once() for _ in range(3)]
+        t = float(np.median(runs))
     finally:
         torch.set_num_threads(prev)
     NZ, NX = w["nz"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]
@@ -214,8 +220,9 @@ def cpu_baseline(w, shots=8, steps=100, k=50):
                       f"the host CPU, forward + autograd gradient through checkpoint chunks of "
                       f"{k} steps, first {shots} of {w['wavelet'].shape[0]} shots x first "
                       f"{steps} of {w['wavelet'].shape[1]} steps of the benchmark workload, "
-                      f"{t:.1f} s wall; the full job would take {full / rate / 3600:.1f} h "
-                      f"at this rate (extrapolated)",
+                      f"median of 3 timed runs after a warm-up ({t:.2f} s each); the full job "
+                      f"would take {full / rate / 3600:.2f} h at this rate (extrapolated)",
+            "runs_s": [round(r, 3) for r in runs],
             "cpu": info, "torch_threads": n}
 
 
@@ -637,6 +644,9 @@ def main():
     ap.add_argument("--batch-launch", type=str, default=None,
                     help="pin the uneven batch's launch choice instead of autotuning: "
                          "shots_per_block,steps_per_launch,tile_rows")
+    ap.add_argument("--grad-batch-launch", type=str, default=None,
+                    help="pin the uneven batch gradient's launch choice instead of autotuning: "
+                         "shots_per_block,tile_rows")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward workload")
@@ -818,7 +828,13 @@ def main():
         rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune,
                                            launch=blaunch)
         torch.cuda.empty_cache()
-        rec["uneven_gradient"] = uneven_gradient(steps=2, warmup=1, tune=not args.no_tune)
+        glaunch = None
+        if args.grad_batch_launch:
+            from geobrain_amd import batch as B
+            spb, rows = (int(v) for v in args.grad_batch_launch.split(","))
+            glaunch = B.BatchLaunch(shots_per_block=spb, tile_rows=rows)
+        rec["uneven_gradient"] = uneven_gradient(steps=2, warmup=1, tune=not args.no_tune,
+                                                 launch=glaunch)
     if not args.no_3d:
         torch.cuda.empty_cache()
         rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d,

@@ -1,4 +1,4 @@
-"""Batched forward modelling of many independent problems.
+"""Batched forward modelling and gradients of many independent problems.
 
 ``BatchedForward`` runs the forward pass of ``n`` independent
 ``acoustic_forward_kernel`` problems -- different grids, step counts, shot
@@ -15,6 +15,15 @@ The launch sequence allocates, copies and synchronises nothing, so
 by ``setup()`` (coefficients, receiver maps); call it again after changing
 them.  Per-problem latency comes from device wall-clock stamps the last
 workgroup of each problem writes when the problem's final launch completes.
+
+``BatchedGradient`` does the same for the gradient: the forward with
+checkpoints, then the recompute and adjoint, of all problems as one launch
+sequence each.  Problems are aligned at their last step, so their checkpoint
+intervals coincide and the backward walks them together.  Each problem's
+seismograms, vp gradient and wavelet gradient equal those of its own
+``acoustic_forward_kernel`` call and autograd backward through
+``torch.utils.checkpoint`` (``acoustic_2d.py:448-460``) for the same adjoint
+sources.  Forward, the caller's misfit and backward capture into one graph.
 """
 
 from __future__ import annotations
