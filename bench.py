@@ -758,8 +758,10 @@ def main():
                      "adj_step_kernel": 28.0, "recomp_march_kernel": 28.0}
     kernels = {}
     for name, (launches, ms) in ktimes.items():
-        base = name.replace("fwd2_", "fwd_").replace("adj2_", "adj_")
-        steps_per = 2 if ("fwd2_" in name or "adj2_" in name) else 1
+        # adj2m_kernel: the z-marching two-step adjoint
+        base = name.replace("fwd2_", "fwd_").replace("adj2_", "adj_").replace(
+            "adj2m_kernel", "adj_step_kernel")
+        steps_per = 2 if ("fwd2_" in name or "adj2" in name) else 1
         b = per_cell_step[base] * steps_per * S * NZ * NX
         avg_s = ms / launches / 1e3
         kernels[name] = {"launches": launches, "total_s": round(ms / 1e3, 4),

@@ -2,8 +2,8 @@
 
 The library takes launch parameters per call (``tile_rows``: rows per
 workgroup tile, 16 or 32; ``shots_per_block``: shots one workgroup steps
-through, 0 = library default; ``march``: tiled or z-marching recompute in the
-gradient).  ``tune`` times short forward and gradient calls of the caller's
+through, 0 = library default; ``march``: tiled, or z-marching recompute or
+adjoint, in the gradient).  ``tune`` times short forward and gradient calls of the caller's
 problem (same grid, shots and geometry, ``nt_trial`` steps) for each candidate
 through the public API and keeps the fastest for each of the two calls.
 
@@ -160,16 +160,19 @@ def _tune_trials(args, trial, vp, cands, log, repeats):
 
     # the recompute and adjoint inside the gradient call follow the forward's
     # steps_per_launch and take their own tiles; the z-marching recompute
-    # exists for one-step launches only
+    # exists for one-step launches only, the z-marching adjoint (march 3, at
+    # most 8 shots per workgroup) for two-step launches
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
     spl = best_f[2]
     bwd_cands = cands if spl == 1 else [(rows, spb) for rows in (16, 24)
                                         for spb in (0, 4, 8, 16, 32)
                                         if spb == 0 or spb < args["src_n"]]
-    for march in ((1, 2) if spl == 1 else (1,)):
+    for march in ((1, 2) if spl == 1 else (1, 3)):
         for tr, spb in bwd_cands:
             if march == 2 and tr != cands[0][0]:
                 continue  # the marching recompute ignores tile_rows
+            if march == 3 and spb > 8:
+                continue  # the marching adjoint takes at most 8 shots per workgroup
             K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb, best_f[2]))
             t = _timed(bwd, repeats)
             log(f"gradient tile_rows={tr} shots_per_block={spb} march={march}: {t:.2f} ms")

@@ -38,6 +38,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -1308,6 +1309,7 @@ struct AdjArgs {
   float* gA2;   // [group][plane] or null
   float* gw;    // [S][nt] or null
   int one;      // two-step kernel: adjoint step it only (it == 0: the problem's first step)
+  int am_segs, am_seg_h;  // z-marching adjoint: z segments per strip, rows per segment
   // the batched gradient's launches (PHASED kernel): the per-problem arguments
   // come from these descriptors, the rest of this struct is unused
   const struct GradDesc* batch_descs;
@@ -2192,6 +2194,457 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
 }
 
 
+// ---------------------------------------------------------------------------
+// Two adjoint steps (it, it-1) by z-marching wavefronts (launch choice
+// march = 3; no free surface, no density gradient).  The per-cell arithmetic
+// is adj2_step_kernel's, in the same order, and the imaging sums are added in
+// the same order, so the results equal it bit for bit.
+//
+// A wavefront holds one shot on a 128-column window (2 cells per lane) and
+// walks a z-segment of rows top to bottom.  Iteration i finishes row i of
+// the second step and works on five rows in a pipeline:
+//   row i+6: the adjoint state lambda (step it+1), q0 = a2 * lambda_v;
+//   row i+5: lambda_p' of step it (gather of q0), r1 = a1 * lambda_p',
+//            lambda_p after step it, the imaging term of step it;
+//   row i+3: lambda_v after step it (gather of r1), q1 = a2 * lambda_v;
+//   row i+2: lambda_p' of step it-1, r2, lambda_p after it-1 (stored), the
+//            imaging term of step it-1;
+//   row i:   lambda_v after step it-1 (stored).
+// Each stage reads the rows above and below it from a four-row window kept in
+// registers, and the neighbouring columns from the adjacent lanes (DPP
+// wavefront shifts), so the stencils need no barrier.  The window is 6
+// columns wider than the 116 columns a wavefront owns on each side, and a
+// segment's pipeline starts 6 rows above its first row.  Rows a stage works on
+// outside what the segment's own rows need are computed and not used.
+// The NW wavefronts of a workgroup take the shots of one shot group on the
+// same window and segment: they share the coefficient rows, which the
+// workgroup stages in an LDS ring one 4-row block ahead.  Each wavefront loads
+// its shot's rows one iteration ahead.  Every 4 rows the wavefronts write their
+// imaging terms to LDS; after a barrier the workgroup sums them over its shots
+// in shot order and adds them into the group's gradient plane.
+// Blocks away from the grid's edge rows, the segment's first and last rows,
+// receivers and the shot's source run without any row test (G = false).
+// ---------------------------------------------------------------------------
+constexpr int AM_OWN = 116;   // columns a wavefront owns (64 lanes x 2 cells, less 6 per side)
+constexpr int AM_SLOTS = 8;   // LDS row slots of the imaging terms: two 4-row blocks
+constexpr int AM_CROWS = 16;  // LDS ring rows of the coefficients
+#ifndef GBW_AM_TARGET_WGS
+#define GBW_AM_TARGET_WGS 256
+#endif
+constexpr int AM_TARGET_WGS = GBW_AM_TARGET_WGS;  // workgroups to aim for (MI355X: 256 CUs)
+#ifndef GBW_AM_PF
+#define GBW_AM_PF 2
+#endif
+constexpr int AM_PF = GBW_AM_PF;  // iterations a wavefront loads its shot's rows ahead (1, 2, 4)
+static_assert(AM_PF == 1 || AM_PF == 2 || AM_PF == 4, "the 4-row unroll needs AM_PF | 4");
+
+// lane l-1's value (lane 0: 0) and lane l+1's value (lane 63: 0)
+__device__ __forceinline__ float lane_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float2 f2zero() { return make_float2(0.f, 0.f); }
+// float2 at byte offset `lane_off` of row `row` of a plane (uniform row base,
+// 32-bit lane offset)
+__device__ __forceinline__ float2 row_ld2(const float* plane, int row, int PX, unsigned lane_off) {
+  const char* rb = reinterpret_cast<const char*>(plane) + (long long)row * PX * 4;
+  return *reinterpret_cast<const float2*>(rb + lane_off);
+}
+__device__ __forceinline__ void row_st2(float* plane, int row, int PX, unsigned lane_off,
+                                        const float2& v) {
+  char* rb = reinterpret_cast<char*>(plane) + (long long)row * PX * 4;
+  *reinterpret_cast<float2*>(rb + lane_off) = v;
+}
+
+// CM: the window reaches a column outside the P, X or Z region (first and
+// last strips): per-cell column masks
+template <int NW, bool CM>
+__device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg, int strip,
+                                           float2 (*sT)[NW][AM_SLOTS][64],
+                                           float2 (*sC)[AM_CROWS][64]) {
+  const Dims d = a.d;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s_begin = group * a.spb;
+  const int ns = min(a.S, s_begin + a.spb) - s_begin;
+  const bool active = w < ns;  // uniform per wavefront
+  const int s = s_begin + (active ? w : 0);
+  const int zs = seg * a.am_seg_h, ze = min(d.NZ, zs + a.am_seg_h);
+  const int xs = strip * AM_OWN - 6;
+  const int x0 = xs + 2 * lane;
+  const bool lane_ok = x0 >= 0 && x0 <= d.PX - 2;
+  const unsigned xo4 = (unsigned)min(max(x0, 0), d.PX - 2) * 4u;  // lane byte offset in a row
+  const bool own_lane = lane >= 3 && lane < 3 + AM_OWN / 2 && x0 < d.NX;
+  // column tests of the lane's two cells
+  bool cin[2], cp[2], cx[2], cz[2], ce[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int x = x0 + e;
+    cin[e] = lane_ok && x < d.NX;
+    cp[e] = cin[e] && x >= 2 && x <= d.NX - 3;
+    cx[e] = cin[e] && x >= 1 && x <= d.NX - 3;
+    cz[e] = cin[e] && x >= 1 && x <= d.NX - 2;
+    ce[e] = cin[e] && x <= d.NX - 2;
+  }
+  const long long so = (long long)s * d.plane;
+  const float* LP0 = a.LP0 + so;
+  const float* LX0 = a.LX0 + so;
+  const float* LZ0 = a.LZ0 + so;
+  float* LP1 = a.LP1 + so;
+  float* LX1 = a.LX1 + so;
+  float* LZ1 = a.LZ1 + so;
+  const float* D0 = a.dbuf + so;
+  const float* D1 = a.dbuf1 + so;
+  const int scell = a.src_cell[s];
+  const int ssz = scell / d.PX, ssx = scell - ssz * d.PX;
+  const int it = a.it;
+  const float2* sCl = &sC[0][0][lane];  // this lane's column in the coefficient ring
+  auto row_in = [&](int r) { return r >= 0 && r < d.NZ; };
+  auto row_p = [&](int r) { return r >= d.f + 1 && r <= d.NZ - 3; };
+  auto row_x = [&](int r) { return r >= d.f && r <= d.NZ - 2; };
+  auto row_z = [&](int r) { return r >= d.f && r <= d.NZ - 3; };
+  // receivers' adjoint sources of record step `rec` at row r, added to the
+  // lane's cells of p and / or v (same list order as the tile kernels)
+  auto inject = [&](int r, int rec, float2* lp, float2* lx, float2* lz) {
+    if (!lane_ok) return;
+    const int2 rh = *reinterpret_cast<const int2*>(
+        reinterpret_cast<const char*>(a.rhead) + (long long)r * d.PX * 4 + xo4);
+    for (int hh = rh.x; hh >= 0; hh = a.rnext[hh]) {
+      const long long o = ((long long)s * a.nt + rec) * a.R + hh;
+      if (lp && a.g_p) lp->x = lp->x + a.g_p[o];
+      if (lx && a.g_vx) lx->x = lx->x + a.g_vx[o];
+      if (lz && a.g_vz) lz->x = lz->x + a.g_vz[o];
+    }
+    for (int hh = rh.y; hh >= 0; hh = a.rnext[hh]) {
+      const long long o = ((long long)s * a.nt + rec) * a.R + hh;
+      if (lp && a.g_p) lp->y = lp->y + a.g_p[o];
+      if (lx && a.g_vx) lx->y = lx->y + a.g_vx[o];
+      if (lz && a.g_vz) lz->y = lz->y + a.g_vz[o];
+    }
+  };
+  // q = a2 * lambda_v of row r (X / Z regions only); G: test the row
+  auto make_q = [&](auto G, int r, const float2* cr, const float2& lx, const float2& lz,
+                    float2& qx, float2& qz) {
+    const float2 c2 = cr[4 * AM_CROWS * 64];
+    qx = make_float2(c2.x * lx.x, c2.y * lx.y);
+    qz = make_float2(c2.x * lz.x, c2.y * lz.y);
+    if constexpr (decltype(G)::value) {
+      if (!row_x(r)) qx = f2zero();
+      if (!row_z(r)) qz = f2zero();
+    }
+    if constexpr (CM) {
+      if (!cx[0]) qx.x = 0.f;
+      if (!cx[1]) qx.y = 0.f;
+      if (!cz[0]) qz.x = 0.f;
+      if (!cz[1]) qz.y = 0.f;
+    }
+  };
+  // lambda_p' = lambda_p + gather of q (rows r-2..r+1 of qz, columns of qx),
+  // then r = a1 * lambda_p' on the P region (pr: cells in it) and lambda_p
+  // after the step, omk1 * lambda_p' on the P region
+  auto lam_p = [&](auto G, int r, const float2* cr, const float2& lp, const float2& qx,
+                   const float2& zm2, const float2& zm1, const float2& z00, const float2& zp1,
+                   float2& lpv, float2& rr, float2& lpo, bool* pr) {
+    const float ql0 = lane_prev(qx.x), ql1 = lane_prev(qx.y), qr0 = lane_next(qx.x);
+    {
+      const float az = kC1 * (z00.x - zm1.x) + kC2 * (zp1.x - zm2.x);
+      const float ax = kC1 * (qx.x - ql1) + kC2 * (qx.y - ql0);
+      float acc = az;
+      acc = acc + ax;
+      lpv.x = lp.x + acc;
+    }
+    {
+      const float az = kC1 * (z00.y - zm1.y) + kC2 * (zp1.y - zm2.y);
+      const float ax = kC1 * (qx.y - qx.x) + kC2 * (qr0 - ql1);
+      float acc = az;
+      acc = acc + ax;
+      lpv.y = lp.y + acc;
+    }
+    bool rp = true;
+    if constexpr (decltype(G)::value) {
+      rp = row_p(r);
+      if (!row_in(r)) lpv = f2zero();
+    }
+    if constexpr (CM) {
+      if (!lane_ok) lpv = f2zero();
+    }
+    const float2 c1 = cr[1 * AM_CROWS * 64], o1 = cr[0];
+    rr = make_float2(c1.x * lpv.x, c1.y * lpv.y);
+    lpo = make_float2(o1.x * lpv.x, o1.y * lpv.y);
+    pr[0] = rp && (!CM || cp[0]);
+    pr[1] = rp && (!CM || cp[1]);
+    if constexpr (CM || decltype(G)::value) {
+      if (!pr[0]) { rr.x = 0.f; lpo.x = lpv.x; }
+      if (!pr[1]) { rr.y = 0.f; lpo.y = lpv.y; }
+    }
+  };
+  // lambda_v after the step: omk * lambda_v + gather of r (rows r-1..r+2)
+  auto lam_v = [&](auto G, int r, const float2* cr, float2& lx, float2& lz, const float2& rm1,
+                   const float2& r00, const float2& rp1, const float2& rp2) {
+    constexpr bool GG = decltype(G)::value;
+    const float rl1 = lane_prev(r00.y), rr0 = lane_next(r00.x), rr1 = lane_next(r00.y);
+    const float2 o2 = cr[2 * AM_CROWS * 64], o3 = cr[3 * AM_CROWS * 64];
+    bool rx = true, rz = true, rin = true, ml = true;
+    if constexpr (GG) {
+      rx = row_x(r); rz = row_z(r); rin = row_in(r); ml = rin && r <= d.NZ - 2;
+    }
+    const bool mx0 = rx && (!CM || cx[0]), mx1 = rx && (!CM || cx[1]);
+    const bool mz0 = rz && (!CM || cz[0]), mz1 = rz && (!CM || cz[1]);
+    const bool me0 = rin && (!CM || ce[0]), me1 = rin && (!CM || ce[1]);
+    const bool mi0 = ml && (!CM || cin[0]), mi1 = ml && (!CM || cin[1]);
+    float lx0 = lx.x, lz0 = lz.x, lx1 = lx.y, lz1 = lz.y;
+    if (mx0) lx0 = o2.x * lx0;
+    if (mz0) lz0 = o3.x * lz0;
+    if (me0) lx0 += kC1 * (r00.y - r00.x) + kC2 * (rr0 - rl1);
+    if (mi0) lz0 += kC1 * (rp1.x - r00.x) + kC2 * (rp2.x - rm1.x);
+    if (mx1) lx1 = o2.y * lx1;
+    if (mz1) lz1 = o3.y * lz1;
+    if (me1) lx1 += kC1 * (rr0 - r00.y) + kC2 * (rr1 - r00.x);
+    if (mi1) lz1 += kC1 * (rp1.y - r00.y) + kC2 * (rp2.y - rm1.y);
+    lx = make_float2(lx0, lx1);
+    lz = make_float2(lz0, lz1);
+  };
+
+  // four-row register windows, slot = row & 3
+  float2 WLP[4], WLX[4], WLZ[4], WQX0[4], WQZ0[4], WR1[4], WLP1[4], WT1[4];
+  float2 WLX1[4], WLZ1[4], WQX1[4], WQZ1[4], WR2[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    WLP[k] = WLX[k] = WLZ[k] = WQX0[k] = WQZ0[k] = WR1[k] = WLP1[k] = WT1[k] = f2zero();
+    WLX1[k] = WLZ1[k] = WQX1[k] = WQZ1[k] = WR2[k] = f2zero();
+  }
+  // this shot's rows, loaded AM_PF iterations ahead: lambda of row i+6, D^it
+  // of row i+5, D^(it-1) of row i+2 (rows clamped into what the segment needs)
+  const int ja = max(0, zs - 6), jb = min(d.NZ, ze + 6) - 1;
+  float2 pf_lp[AM_PF], pf_lx[AM_PF], pf_lz[AM_PF], pf_d0[AM_PF], pf_d1[AM_PF];
+  auto prefetch = [&](auto K, int i) {
+    constexpr int k = decltype(K)::value;
+    const int j = min(max(i + 6, ja), jb);
+    pf_lp[k] = row_ld2(LP0, j, d.PX, xo4);
+    pf_lx[k] = row_ld2(LX0, j, d.PX, xo4);
+    pf_lz[k] = row_ld2(LZ0, j, d.PX, xo4);
+    pf_d0[k] = row_ld2(D0, min(max(i + 5, zs), ze - 1), d.PX, xo4);
+    pf_d1[k] = row_ld2(D1, min(max(i + 2, zs), ze - 1), d.PX, xo4);
+  };
+
+  auto iter = [&](auto U, auto G, int base) {
+    constexpr int uu = decltype(U)::value;
+    constexpr bool GG = decltype(G)::value;
+    const int i = base + uu;
+    constexpr int pk = uu % AM_PF;
+    const float2 lp_in = pf_lp[pk], lx_in = pf_lx[pk], lz_in = pf_lz[pk], d0 = pf_d0[pk],
+                 d1 = pf_d1[pk];
+    prefetch(std::integral_constant<int, pk>{}, i + AM_PF);
+    // --- row j = i+6: lambda of step it+1, q0 ---
+    {
+      constexpr int sj = (uu + 6) & 3;
+      const int j = i + 6;
+      float2 lp = lp_in, lx = lx_in, lz = lz_in;
+      if constexpr (GG) {
+        if (!row_in(j)) lp = lx = lz = f2zero();
+      }
+      if constexpr (CM) {
+        if (!lane_ok) lp = lx = lz = f2zero();
+      }
+      WLP[sj] = lp; WLX[sj] = lx; WLZ[sj] = lz;
+      make_q(G, j, sCl + (j & (AM_CROWS - 1)) * 64, lx, lz, WQX0[sj], WQZ0[sj]);
+    }
+    // --- row b = i+5: lambda_p' of step it, r1, lambda_p after it, imaging ---
+    {
+      constexpr int sb = (uu + 5) & 3;
+      const int b = i + 5;
+      float2 lpv, lpo;
+      bool pr[2];
+      lam_p(G, b, sCl + (b & (AM_CROWS - 1)) * 64, WLP[sb], WQX0[sb], WQZ0[(uu + 3) & 3],
+            WQZ0[(uu + 4) & 3], WQZ0[sb], WQZ0[(uu + 6) & 3], lpv, WR1[sb], lpo, pr);
+      if (!GG || (b >= zs && b < ze)) {
+        float2 t1 = f2zero();
+        if (pr[0]) t1.x = lpv.x * d0.x;
+        if (pr[1]) t1.y = lpv.y * d0.y;
+        WT1[sb] = t1;
+      }
+      if constexpr (GG) {
+        if (it >= 1 && a.R > 0 && row_in(b) && a.rrow[b]) inject(b, it - 1, &lpo, nullptr, nullptr);
+        if (a.gw && b == ssz && b >= zs && b < ze && own_lane) {
+          if (x0 == ssx && cin[0]) a.gw[(long long)s * a.nt + it] = a.dt * lpv.x;
+          if (x0 + 1 == ssx && cin[1]) a.gw[(long long)s * a.nt + it] = a.dt * lpv.y;
+        }
+      }
+      WLP1[sb] = lpo;
+    }
+    // --- row c = i+3: lambda_v after step it, q1 ---
+    {
+      constexpr int sc = (uu + 3) & 3;
+      const int c = i + 3;
+      const float2* cr = sCl + (c & (AM_CROWS - 1)) * 64;
+      float2 lx = WLX[sc], lz = WLZ[sc];
+      lam_v(G, c, cr, lx, lz, WR1[(uu + 2) & 3], WR1[sc], WR1[uu & 3], WR1[(uu + 1) & 3]);
+      if constexpr (GG) {
+        if (it >= 1 && a.R > 0 && row_in(c) && a.rrow[c]) inject(c, it - 1, nullptr, &lx, &lz);
+      }
+      WLX1[sc] = lx; WLZ1[sc] = lz;
+      make_q(G, c, cr, lx, lz, WQX1[sc], WQZ1[sc]);
+    }
+    // --- row dr = i+2: lambda_p' of step it-1, r2, lambda_p (stored), imaging ---
+    {
+      constexpr int sd = (uu + 2) & 3;
+      const int dr = i + 2;
+      float2 lpv, lpo, r2;
+      bool pr[2];
+      lam_p(G, dr, sCl + (dr & (AM_CROWS - 1)) * 64, WLP1[sd], WQX1[sd], WQZ1[uu & 3],
+            WQZ1[(uu + 1) & 3], WQZ1[sd], WQZ1[(uu + 3) & 3], lpv, r2, lpo, pr);
+      WR2[sd] = r2;
+      if (!GG || (dr >= zs && dr < ze)) {
+        if constexpr (GG) {
+          if (it - 1 >= 1 && a.R > 0 && a.rrow[dr]) inject(dr, it - 2, &lpo, nullptr, nullptr);
+        }
+        if (own_lane) row_st2(LP1, dr, d.PX, xo4, lpo);
+        float2 t2 = f2zero();
+        if (pr[0]) t2.x = lpv.x * d1.x;
+        if (pr[1]) t2.y = lpv.y * d1.y;
+        sT[0][w][dr & (AM_SLOTS - 1)][lane] = WT1[sd];
+        sT[1][w][dr & (AM_SLOTS - 1)][lane] = t2;
+        if constexpr (GG) {
+          if (a.gw && dr == ssz && own_lane) {
+            if (x0 == ssx && cin[0]) a.gw[(long long)s * a.nt + it - 1] = a.dt * lpv.x;
+            if (x0 + 1 == ssx && cin[1]) a.gw[(long long)s * a.nt + it - 1] = a.dt * lpv.y;
+          }
+        }
+      }
+    }
+    // --- row i: lambda_v after step it-1 (stored) ---
+    {
+      constexpr int se = uu & 3;
+      if (!GG || (i >= zs && i < ze)) {
+        float2 lx = WLX1[se], lz = WLZ1[se];
+        lam_v(G, i, sCl + (i & (AM_CROWS - 1)) * 64, lx, lz, WR2[(uu + 3) & 3], WR2[se],
+              WR2[(uu + 1) & 3], WR2[(uu + 2) & 3]);
+        if constexpr (GG) {
+          if (it - 1 >= 1 && a.R > 0 && a.rrow[i]) inject(i, it - 2, nullptr, &lx, &lz);
+        }
+        if (own_lane) {
+          row_st2(LX1, i, d.PX, xo4, lx);
+          row_st2(LZ1, i, d.PX, xo4, lz);
+        }
+      }
+    }
+  };
+
+  // coefficient ring: per block 5 planes x 4 rows, one row-plane per wavefront
+  // at a time (uniform row base, lane offset)
+  constexpr int CRP = 20, CPW = (CRP + NW - 1) / NW;
+  float2 cpf[CPW];
+  auto coef_load = [&](int r0) {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int rp = w + k * NW;  // plane * 4 + row
+      if (rp < CRP) {
+        const int r = min(max(r0 + (rp & 3), 0), d.NZ - 1);
+        cpf[k] = row_ld2(a.coef + (rp >> 2) * d.plane, r, d.PX, xo4);
+      }
+    }
+  };
+  auto coef_store = [&](int r0) {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int rp = w + k * NW;
+      if (rp < CRP) sC[rp >> 2][(r0 + (rp & 3)) & (AM_CROWS - 1)][lane] = cpf[k];
+    }
+  };
+  const int i0 = zs - 12;
+  // rows i0 .. i0+11 for the first block
+  coef_load(i0); coef_store(i0);
+  coef_load(i0 + 4); coef_store(i0 + 4);
+  coef_load(i0 + 8); coef_store(i0 + 8);
+  __syncthreads();
+  if (active) {
+    prefetch(std::integral_constant<int, 0>{}, i0);
+    if constexpr (AM_PF > 1) prefetch(std::integral_constant<int, 1 % AM_PF>{}, i0 + 1);
+    if constexpr (AM_PF > 2) {
+      prefetch(std::integral_constant<int, 2 % AM_PF>{}, i0 + 2);
+      prefetch(std::integral_constant<int, 3 % AM_PF>{}, i0 + 3);
+    }
+  }
+
+  constexpr int RED = 4 * 128 / (NW * 64);  // reduced cells per thread per block
+  float* __restrict__ g1 = a.gA1 + (long long)group * d.plane;
+  for (int base = i0; base < ze; base += 4) {
+    // coefficient rows of the next block (base+12..base+15; this block uses
+    // base..base+9, the ring rows base+10, base+11 are already there)
+    coef_load(base + 12);
+    // the gradient cells this block completes (rows base+2..base+5), loaded early
+    float gv[RED];
+#pragma unroll
+    for (int k = 0; k < RED; ++k) {
+      const int q = threadIdx.x + k * NW * 64;
+      const int row = base + 2 + (q >> 7), cl = q & 127, col = xs + cl;
+      const bool ok = row >= zs && row < ze && cl >= 6 && cl < 6 + AM_OWN && col < d.NX;
+      gv[k] = ok ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(g1) +
+                                                     (long long)row * d.PX * 4 + (unsigned)col * 4u)
+                 : 0.f;
+    }
+    if (active) {
+      // rows this block's stages touch: base .. base+9
+      const int rr = base + lane;
+      const bool rcv = lane < 10 && rr >= 0 && rr < d.NZ && a.rrow[rr];
+      const bool plain = base >= zs && base + 8 < ze && base >= d.f + 1 && base + 9 <= d.NZ - 3 &&
+                         __ballot(rcv) == 0 && !(ssz >= base && ssz <= base + 9);
+      if (plain) {
+        iter(std::integral_constant<int, 0>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 1>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 2>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 3>{}, std::false_type{}, base);
+      } else {
+        iter(std::integral_constant<int, 0>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 1>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 2>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 3>{}, std::true_type{}, base);
+      }
+    }
+    coef_store(base + 12);
+    __syncthreads();
+    // sums over the group's shots in shot order, per step, then step it's sum
+    // and step it-1's added to the plane (as adj2_step_kernel's flush)
+#pragma unroll
+    for (int k = 0; k < RED; ++k) {
+      const int q = threadIdx.x + k * NW * 64;
+      const int row = base + 2 + (q >> 7), cl = q & 127, col = xs + cl;
+      if (!(row >= zs && row < ze && cl >= 6 && cl < 6 + AM_OWN && col < d.NX)) continue;
+      const int slot = row & (AM_SLOTS - 1);
+      const float* t0 = reinterpret_cast<const float*>(&sT[0][0][slot][0]) + cl;
+      const float* t1 = reinterpret_cast<const float*>(&sT[1][0][slot][0]) + cl;
+      float s0 = 0.f, s1 = 0.f;
+      for (int ww = 0; ww < ns; ++ww) {
+        s0 = s0 - t0[ww * AM_SLOTS * 128];
+        s1 = s1 - t1[ww * AM_SLOTS * 128];
+      }
+      float v = gv[k] + s0;
+      v = v + s1;
+      *reinterpret_cast<float*>(reinterpret_cast<char*>(g1) + (long long)row * d.PX * 4 +
+                                (unsigned)col * 4u) = v;
+    }
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void adj2m_kernel(AdjArgs a) {
+  __shared__ float2 sT[2][NW][AM_SLOTS][64];  // imaging terms per step, shot, row slot, lane
+  __shared__ float2 sC[5][AM_CROWS][64];      // coefficient ring: plane, row slot, lane
+  const int groups = (a.S + a.spb - 1) / a.spb;
+  const int u = xcd_tile();
+  if (u >= a.ntiles * groups) return;  // uniform per workgroup
+  const int group = u % groups;
+  const int seg = (u / groups) % a.am_segs;
+  const int strip = u / groups / a.am_segs;
+  const int xs = strip * AM_OWN - 6;
+  if (xs >= 2 && xs + 127 <= a.d.NX - 3)
+    adj2m_body<NW, false>(a, group, seg, strip, sT, sC);
+  else
+    adj2m_body<NW, true>(a, group, seg, strip, sT, sC);
+}
+
 // lambda^{nt} = scatter of the records of the last step (zeroed buffer in)
 __global__ void adj_inject_kernel(Dims d, int S, int nt, int it, int R, const int* __restrict__ rcell,
                                   const int* __restrict__ rhead, const int* __restrict__ rnext,
@@ -2485,6 +2938,8 @@ struct Plan {
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
+  bool amarch;      // z-marching two-step adjoint (adj2m_kernel) instead of tiles
+  int am_strips, am_segs, am_seg_h;  // its column strips, z segments and rows per segment
   bool two;         // two time steps per forward / recompute launch
   int TZ2, TX2, tiles_x2;  // tiles of the two-step kernels (16 x 128 or 24 x 96)
   int mseg, mseg_h; // z segments per strip and rows per segment for it
@@ -2543,13 +2998,33 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.spb = spb;
   pl.groups = (pl.S + spb - 1) / spb;
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
-  if (p->march < 0 || p->march > 2) return fail(GBW_ERR_INVALID, "march must be 0, 1 or 2");
+  if (p->march < 0 || p->march > 3) return fail(GBW_ERR_INVALID, "march must be 0, 1, 2 or 3");
   if (p->steps_per_launch < 0 || p->steps_per_launch > 2)
     return fail(GBW_ERR_INVALID, "steps_per_launch must be 0, 1 or 2");
   pl.two = p->steps_per_launch != 1;
   // z-marching recompute only on request: with coefficients reloaded per shot
   // it measured slower than tiles on 128-column strips
   pl.march = p->march == 2;
+  // z-marching two-step adjoint: one wavefront per shot of a group of at most 8;
+  // chosen by default for gradients of 8 or more shots without a free surface
+  // (it fills its workgroups and measured faster than tiles: 64 shots of
+  // 390 x 1740, 426 vs 622 us per launch)
+  pl.amarch = pl.two && pass >= 1 &&
+              (p->march == 3 || (p->march == 0 && pass == 1 && !d.fs && pl.S >= 8));
+  if (pl.amarch) {
+    pl.spb = std::min(pl.spb, 8);
+    pl.groups = (pl.S + pl.spb - 1) / pl.spb;
+  }
+  pl.am_strips = (d.NX + AM_OWN - 1) / AM_OWN;
+  {
+    // as many z segments as keep all workgroups resident at once (one per
+    // CU: its registers and LDS allow no second), segments of at least 16 rows
+    const int per_seg = pl.am_strips * pl.groups;
+    int ns = std::max(1, AM_TARGET_WGS / per_seg);
+    ns = std::min(ns, std::max(1, d.NZ / 16));
+    pl.am_seg_h = ((d.NZ + ns - 1) / ns + 3) / 4 * 4;
+    pl.am_segs = (d.NZ + pl.am_seg_h - 1) / pl.am_seg_h;
+  }
   {
     const int strips = pl.tiles_x;
     int ns = std::max(1, std::min((2048 + strips * pl.S - 1) / (strips * pl.S), (d.NZ + 47) / 48));
@@ -2756,8 +3231,22 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
     fwd_step_kernel<16, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
 }
 
-// two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles
+// two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles, or z-marching
+// wavefronts (no free surface, no density gradient)
 void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
+  if (pl.amarch && !a.gA2 && !pl.d.fs) {
+    AdjArgs am = a;
+    am.ntiles = pl.am_strips * pl.am_segs;
+    am.am_segs = pl.am_segs;
+    am.am_seg_h = pl.am_seg_h;
+    const unsigned n = (unsigned)(((long long)am.ntiles * pl.groups + 7) / 8 * 8);
+    LaunchTimer timer("adj2m_kernel", st);
+    if (pl.spb <= 4)
+      adj2m_kernel<4><<<n, 256, 0, st>>>(am);
+    else
+      adj2m_kernel<8><<<n, 512, 0, st>>>(am);
+    return;
+  }
   AdjArgs a2 = a;
   a2.tiles_x = pl.tiles_x2;
   a2.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);

@@ -62,9 +62,10 @@ def auto_checkpoint_interval(nt: int) -> int:
 class LaunchChoice:
     """Launch parameters the library accepts: ``tile_rows`` (16 or 32 rows for
     one-step launches; 24 selects 24 x 96 tiles for two-step launches),
-    ``shots_per_block`` and ``march`` (gradient recompute: 1 tiles, 2 z-marching
-    strips), ``steps_per_launch`` (1 or 2 time steps per forward / recompute
-    launch); zeros select the library's defaults.  ``bwd_tile_rows`` and
+    ``shots_per_block`` and ``march`` (gradient: 1 tiles, 2 z-marching
+    recompute strips, 3 z-marching two-step adjoint wavefronts),
+    ``steps_per_launch`` (1 or 2 time steps per forward / recompute launch);
+    zeros select the library's defaults.  ``bwd_tile_rows`` and
     ``bwd_shots_per_block`` apply to the gradient call (0 = same as the
     forward's): the forward and adjoint kernels have different register
     budgets, so their best tiles differ.  The autotuner fills these."""

@@ -68,8 +68,11 @@ typedef struct gbw_acoustic2d_problem {
                              /* launches; 24 selects 24 x 96 tiles for two-step ones    */
                              /* (else 16 x 128); 0 = default                            */
   int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
-  int32_t march;             /* launch choice for the gradient's recompute: 0 auto,     */
-                             /* 1 tiles, 2 z-marching strips                             */
+  int32_t march;             /* launch choice for the gradient: 0 auto, 1 tiles,        */
+                             /* 2 z-marching recompute strips (one step per launch),    */
+                             /* 3 z-marching two-step adjoint (tiled recompute; without */
+                             /* a free surface or rho gradient it falls back to tiles); */
+                             /* auto takes 3 for 8 or more shots without a free surface */
   int32_t steps_per_launch;  /* launch choice: time steps per forward/recompute launch */
                              /* (1 or 2; 0 = default 2; two-step launches tile 16 rows) */
 } gbw_acoustic2d_problem;

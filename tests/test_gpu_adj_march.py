@@ -1,0 +1,114 @@
+"""The z-marching two-step adjoint (launch choice march = 3, adj2m_kernel)
+against the tiled two-step adjoint (adj2_step_kernel): gradients with respect
+to vp and the wavelets bit-identical, on grids with several column strips and
+z segments (the last of each partial), odd step counts and checkpoint
+intervals (so one-step launches mix in), shot groups of 1-8 shots, adjoint
+sources on p, vx and vz, and sources and receivers near the strip and segment
+boundaries; then at the full 64-shot 350 x 1700 size for a shortened time
+axis.  With a free surface or a density gradient the library keeps the tiled
+kernel, which the last case checks."""
+
+import numpy as np
+import pytest
+import torch
+
+from geobrain_amd import synthetic
+from geobrain_amd.physics import wave
+from geobrain_amd.physics.wave import kernels as K
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _problem(nz, nx, nt, S, seed, rcv_z=2, src_z=None):
+    vp = synthetic.grf_model(nz, nx, 5.0, seed=seed)
+    w = synthetic.ricker_padded(15.0, 5e-4, nt)
+    damp = wave.bc_pml(nx, nz, 5.0, 5.0, 20, float(vp.max()), False).astype(np.float32)
+    rx = np.arange(0, nx, 2)
+    sx = synthetic.surface_shots(nx, S)
+    sz = np.arange(S) % 3 if src_z is None else np.asarray(src_z)
+    return dict(vp=vp, rho=synthetic.gardner(vp), damp=damp,
+                w=np.stack([w * (1 + 0.1 * s) for s in range(S)]).astype(np.float32),
+                sx=sx, sz=sz, rx=rx, rz=np.full(len(rx), rcv_z), nz=nz, nx=nx, nt=nt)
+
+
+def _gradient(p, launch, fs=False, segments=3, rho_grad=False, seed=1):
+    """Seismograms, vp / wavelet (/ rho) gradients of sum(g_p p + g_vx vx + g_vz vz)."""
+    T = lambda a, **kw: torch.as_tensor(np.asarray(a), device=DEV, **kw)  # noqa: E731
+    vp = T(p["vp"], dtype=torch.float32).requires_grad_(True)
+    rho = T(p["rho"], dtype=torch.float32).requires_grad_(rho_grad)
+    w = T(p["w"], dtype=torch.float32).requires_grad_(True)
+    K.set_launch_choice(launch)
+    try:
+        out = wave.acoustic_forward_kernel(
+            nx=p["nx"], nz=p["nz"], dx=5.0, dz=5.0, nt=p["nt"], dt=5e-4, nabc=20,
+            free_surface=fs, src_x=T(p["sx"]), src_z=T(p["sz"]), src_n=len(p["sx"]), src_v=w,
+            rcv_x=T(p["rx"]), rcv_z=T(p["rz"]), rcv_n=len(p["rx"]), damp=T(p["damp"]),
+            vp=vp, rho=rho, checkpoint_segments=segments)
+        rng = np.random.default_rng(seed)
+        loss = sum((T(rng.standard_normal(out[k].shape).astype(np.float32)) * out[k]).sum()
+                   for k in ("p", "vx", "vz"))
+        loss.backward()
+    finally:
+        K.set_launch_choice(None)
+    torch.cuda.synchronize()
+    res = {k: out[k].detach().cpu().numpy() for k in ("p", "vx", "vz")}
+    res["grad_vp"] = vp.grad.cpu().numpy()
+    res["grad_w"] = w.grad.cpu().numpy()
+    if rho_grad:
+        res["grad_rho"] = rho.grad.cpu().numpy()
+    return res
+
+
+def _assert_same(a, b, what):
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=True), f"{what}: {k} differs"
+
+
+@pytest.mark.parametrize("nz,nx,nt,S,spb,segments", [
+    (70, 300, 241, 5, 2, 3),    # 3 strips (last partial), odd nt, odd intervals
+    (130, 430, 160, 8, 8, 4),   # full 8-shot groups, 5 strips
+    (45, 200, 97, 3, 1, 2),     # one shot per group, small grid
+    (90, 650, 120, 7, 4, 3),    # groups of 4 and 3 shots, 7 strips
+])
+def test_march_adjoint_equals_tiled(nz, nx, nt, S, spb, segments):
+    print(f"[case] march adjoint vs tiled: {nz}x{nx}, nt {nt}, {S} shots, spb {spb}", flush=True)
+    p = _problem(nz, nx, nt, S, seed=nz + nx)
+    tiled = _gradient(p, K.LaunchChoice(0, spb, 1, 0, spb, 2), segments=segments)
+    march = _gradient(p, K.LaunchChoice(0, spb, 3, 0, spb, 2), segments=segments)
+    assert float(np.abs(tiled["grad_vp"]).max()) > 0
+    _assert_same(tiled, march, "march vs tiled")
+
+
+def test_march_adjoint_sources_and_receivers_at_boundaries():
+    """Sources on every row band a segment boundary can cut, receivers deep in
+    the grid (a row inside a segment), and a shot group of 6."""
+    print("[case] march adjoint, sources and receivers near boundaries", flush=True)
+    nz, nx, nt, S = 100, 480, 130, 6
+    p = _problem(nz, nx, nt, S, seed=9, rcv_z=37, src_z=[0, 11, 12, 13, 48, 99])
+    p["sx"] = np.array([0, 95, 96, 97, 212, 479])
+    tiled = _gradient(p, K.LaunchChoice(0, 6, 1, 0, 6, 2), segments=5)
+    march = _gradient(p, K.LaunchChoice(0, 6, 3, 0, 6, 2), segments=5)
+    _assert_same(tiled, march, "march vs tiled")
+
+
+def test_march_adjoint_full_size_64_shots():
+    """The benchmark's grid and shots (350 x 1700, 64 shots, receivers on every
+    column), 200 steps, checkpoints every 50: identical gradients."""
+    print("[case] march adjoint, 64 shots 350x1700, 200 steps", flush=True)
+    p = _problem(350, 1700, 200, 64, seed=3)
+    p["rx"] = np.arange(1700)
+    p["rz"] = np.zeros(1700, np.int64)
+    p["sz"] = np.zeros(64, np.int64)
+    tiled = _gradient(p, K.LaunchChoice(16, 32, 1, 16, 8, 2), segments=4)
+    march = _gradient(p, K.LaunchChoice(16, 32, 3, 16, 8, 2), segments=4)
+    _assert_same(tiled, march, "march vs tiled")
+
+
+def test_march_choice_keeps_tiles_for_free_surface_and_rho():
+    print("[case] march choice with free surface / density gradient", flush=True)
+    p = _problem(60, 260, 90, 3, seed=4)
+    for fs, rho in ((True, False), (False, True)):
+        tiled = _gradient(p, K.LaunchChoice(0, 2, 1, 0, 2, 2), fs=fs, rho_grad=rho)
+        march = _gradient(p, K.LaunchChoice(0, 2, 3, 0, 2, 2), fs=fs, rho_grad=rho)
+        _assert_same(tiled, march, f"free_surface={fs} rho={rho}")
