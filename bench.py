@@ -36,6 +36,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s fwd+adjoint (64-shot batch vs per-shot) and % of HBM roofline"
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line on stderr when a phase of the run ends (the JSON line stays
+    the only output on stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def workload(n_shots=64, nz=350, nx=1700, nt=4000, nabc=20, dx=5.0, dt=5e-4, f0=15.0):
@@ -723,9 +730,11 @@ def main():
             ev[3].record(stream)
         return loss
 
+    progress("headline workload ready" + ("" if choice is None else f", launch {choice}"))
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    progress(f"{args.warmup} warm-up gradient(s) done")
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     # timed steps: no instrumentation (the events above are four per step, on
@@ -735,6 +744,7 @@ def main():
         step(evs[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    progress(f"{args.steps} timed gradient(s): {wall / args.steps:.3f} s each")
     t_fwd = np.mean([e[0].elapsed_time(e[1]) for e in evs]) / 1e3
     t_bwd = np.mean([e[2].elapsed_time(e[3]) for e in evs]) / 1e3
     # one more step, instrumented: the library brackets each time-step kernel
@@ -815,11 +825,13 @@ def main():
     if not args.no_eager:
         base = per_shot_gpu_baseline(w, d_obs)
         rec["per_shot_gpu"] = base
+        progress("per-shot GPU baseline done")
         rec["speedup_vs_per_shot_gpu"] = round(value / base["value"], 1)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, shots=args.cpu_shots, steps=args.cpu_steps)
         rec["cpu_oracle"] = cpu_oracle_baseline(w, steps=args.oracle_steps)
         rec["speedup_vs_cpu_baseline"] = round(value / rec["cpu_baseline"]["value"], 1)
+        progress("CPU baselines done")
     if not args.no_uneven:
         del d_obs, vp, common
         torch.cuda.empty_cache()
@@ -829,6 +841,7 @@ def main():
             blaunch = B.BatchLaunch(*[int(v) for v in args.batch_launch.split(",")])
         rec["uneven_batch"] = uneven_batch(steps=3, warmup=1, tune=not args.no_tune,
                                            launch=blaunch)
+        progress("uneven batch forward done")
         torch.cuda.empty_cache()
         glaunch = None
         if args.grad_batch_launch:
@@ -837,10 +850,12 @@ def main():
             glaunch = B.BatchLaunch(shots_per_block=spb, tile_rows=rows)
         rec["uneven_gradient"] = uneven_gradient(steps=2, warmup=1, tune=not args.no_tune,
                                                  launch=glaunch)
+        progress("uneven batch gradient done")
     if not args.no_3d:
         torch.cuda.empty_cache()
         rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d,
                                        grad=not args.no_grad3d)
+        progress("3D workload done")
     print(json.dumps(rec), flush=True)
 
 
