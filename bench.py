@@ -427,14 +427,17 @@ def workload_3d(n_shots=8, n=200, nt=1000, nabc=20, dx=10.0, dt=8e-4, f0=10.0):
 
 
 def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int = 1300,
-               nt: int = 1000, grad: bool = True) -> dict:
+               nt: int = 1000, grad: bool = True, launch=None) -> dict:
     """Times acoustic_forward_kernel_3d (all nt steps = one bench step) on
     ``workload_3d``; roofline of the time step at its algorithmic 32 B per
     cell-update (p, vx, vy, vz each read and written once).  With ``grad``,
-    also one vp gradient of the same problem."""
+    also one vp gradient of the same problem.  ``launch`` pins a
+    ``kernels.LaunchChoice3D`` (default: the library's choice)."""
     import torch
     from geobrain_amd import _native
     from geobrain_amd.physics import wave
+    from geobrain_amd.physics.wave import kernels as K
+    K.set_launch_choice_3d(launch)
     dev = torch.device("cuda:0")
     w = workload_3d(nt=nt)
     S, nt = w["wavelet"].shape
@@ -458,7 +461,7 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
     pad = [w["nz"] + 2 * w["nabc"], w["ny"] + 2 * w["nabc"], w["nx"] + 2 * w["nabc"]]
     cells = S * pad[0] * pad[1] * pad[2]
     step_kernels = [k for k in kt if k in ("p3_step_kernel", "v3_step_kernel",
-                                           "fwd3_march_kernel<forward>")]
+                                           "fwd3_march_kernel<forward>", "fwd3f_kernel")]
     kernels = {k: {"launches": n, "total_s": round(ms / 1e3, 4),
                    "avg_launch_us": round(ms / n * 1e3, 2)} for k, (n, ms) in kt.items()}
     # one time step = one launch of each step kernel
@@ -486,6 +489,7 @@ def forward_3d(steps: int = 3, warmup: int = 1, cpu: bool = True, cpu_steps: int
                      "traffic": traffic, "avg_time_step_us": round(step_us, 2),
                      "bytes_per_time_step": b_step},
         "kernels": kernels,
+        "launch": dataclasses.asdict(K._launch3),
         "check": {"p_max": float(out["p"].abs().max())},
         "cpu_oracle": None,
     }
@@ -622,6 +626,14 @@ def fwi_precision(n_iter: int = 20, shots: int = 128, nz: int = 400, nx: int = 2
             "fp64_oracle": "reduced-size comparison in tests/test_gpu_precision.py"}
 
 
+def launch3d(spec):
+    """``shots_per_block,step_kernel,z_chunk`` -> kernels.LaunchChoice3D (None: default)."""
+    if not spec:
+        return None
+    from geobrain_amd.physics.wave import kernels as K
+    return K.LaunchChoice3D(*[int(v) for v in spec.split(",")])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -658,6 +670,8 @@ def main():
                     help="skip the uneven-batch forward workload (96 problems, one HIP graph)")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D forward workload")
     ap.add_argument("--nt3d", type=int, default=1000, help="time steps of the 3D workload")
+    ap.add_argument("--launch3d", type=str, default=None,
+                    help="3D launch choice shots_per_block,step_kernel,z_chunk (default: library's)")
     ap.add_argument("--no-grad3d", action="store_true", help="skip the 3D gradient timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (the reference "
@@ -686,7 +700,7 @@ def main():
         return
     if args.workload == "fwd3d":
         rec = forward_3d(steps=args.steps, warmup=args.warmup, cpu=not args.no_cpu, nt=args.nt3d,
-                         grad=not args.no_grad3d)
+                         grad=not args.no_grad3d, launch=launch3d(args.launch3d))
         rec.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                     "higher_is_better": True, "scaling": "weak", "vs_baseline": None})
         print(json.dumps(rec), flush=True)
@@ -854,7 +868,7 @@ def main():
     if not args.no_3d:
         torch.cuda.empty_cache()
         rec["forward_3d"] = forward_3d(steps=2, warmup=1, cpu=not args.no_cpu, nt=args.nt3d,
-                                       grad=not args.no_grad3d)
+                                       grad=not args.no_grad3d, launch=launch3d(args.launch3d))
         progress("3D workload done")
     print(json.dumps(rec), flush=True)
 

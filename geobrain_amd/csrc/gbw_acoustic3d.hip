@@ -1095,6 +1095,321 @@ __global__ __launch_bounds__(MT3) void fwd3_march_kernel(March3Args a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused step, rows in wavefronts (launch choice step_kernel = 2).  A workgroup
+// of 16 wavefronts owns a 61-column x 13-row (x, y) window of one z segment
+// for every shot of a group (at most 8).  Lane l of wavefront w holds the cell
+// (y0 - 1 + w, xs + l) of each shot, and the workgroup marches down z.  Per
+// plane z:
+//   P: p' of plane z from the old state: vx of the neighbouring columns from
+//      the adjacent lanes (DPP wavefront shifts; the window's outer columns
+//      load theirs), vy of the rows above and below from memory (the rows of
+//      the neighbouring wavefronts: L1 / L2), vz of planes z-2 .. z+1 from a
+//      register queue; the source term; then vz' of plane z - 2, whose stencil
+//      is p' of planes z-3 .. z (a register queue of p'); p' of plane z to LDS;
+//   barrier;
+//   V: vx', vy' of plane z: p' of the neighbouring columns by DPP, of the rows
+//      above and below from LDS.
+// Each field is read from HBM once per step and written once; the energy sums
+// run over the group's shots in shot order with one read-modify-write of the
+// group's partials per cell.  The state is out of place (A -> B): the window's
+// halo cells are recomputed from A while neighbouring windows write B.
+// Owned cells: lanes 1..61 (vx' reads p' two columns right), wavefronts 1..13
+// (vy' reads p' two rows down); every other lane and wavefront is halo.
+// Arithmetic per cell is p3_step_kernel's and v3_step_kernel's, in the same
+// order, so the step is bit-identical to them.
+// ---------------------------------------------------------------------------
+constexpr int F3_NW = 16;     // wavefronts (rows) per workgroup
+constexpr int F3_OWN_X = 61;  // owned columns per window (lanes 1..61)
+constexpr int F3_OWN_Y = 13;  // owned rows per window (wavefronts 1..13)
+
+struct Fused3Args {
+  Dims3 d;
+  int S, it, nt, spb;
+  float dt;
+  const float* A;           // state before the step, [field][shot][plane]
+  float* B;                 // state after the step
+  long long fstride;        // S * plane
+  const float* coef;
+  const int* src_cell;
+  const float* wav; long long wav_stride;
+  float* energy;            // [group][4][nz][ny][nx] or null
+  int strips, ytiles, groups, seg_h;
+  long long units;
+};
+
+// lane l-1's value (lane 0: 0) and lane l+1's value (lane 63: 0)
+__device__ __forceinline__ float f3_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float f3_next(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+// per-lane column flags of the fused step (bits of one word)
+enum : unsigned {
+  F3_OWN = 1u, F3_CP = 2u, F3_CVX = 4u, F3_CVY = 8u, F3_CE = 16u, F3_XIN = 32u,
+  F3_L0 = 64u, F3_L01 = 128u, F3_L63 = 256u
+};
+
+// the loads of one shot at one plane (issued ahead of their use)
+struct F3Load {
+  float p, vx, vxh, vy, vym2, vym1, vyp1, vzn;
+};
+
+// Buffer accesses: a 128-bit resource (base, size in bytes) in scalar
+// registers, a 32-bit lane byte offset and a uniform byte offset (the shot's
+// plane).  Reads beyond the size return 0 and writes beyond it are dropped.
+typedef __amdgpu_buffer_rsrc_t f3_rsrc;
+__device__ __forceinline__ f3_rsrc f3_buf(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+// (the b32 builtins move raw 32-bit words: bit casts, not conversions)
+__device__ __forceinline__ float f3_bld(f3_rsrc r, unsigned voff, unsigned soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void f3_bst(f3_rsrc r, unsigned voff, unsigned soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+template <int SPB>
+__global__ __launch_bounds__(F3_NW * 64) void fwd3f_kernel(Fused3Args a) {
+  __shared__ float lp[2][SPB][F3_NW][64];  // p' of the plane: parity, shot, row, lane
+  // vz of planes z-2 .. z+1 of each thread's own cell (a ring by plane; each
+  // thread reads only what it wrote, so the ring needs no barrier)
+  __shared__ float vzr[3][SPB][F3_NW][64];
+  const Dims3 d = a.d;
+  const long long L = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (L >= a.units) return;  // uniform per workgroup
+  const int grp = (int)(L % a.groups);
+  long long t = L / a.groups;
+  const int strip = (int)(t % a.strips);
+  t /= a.strips;
+  const int ytile = (int)(t % a.ytiles);
+  const int zseg = (int)(t / a.ytiles);
+  // segment bounds; no boundary at f, so planes f-1 and f (the mirrors) share a segment
+  auto zbound = [&](int i) {
+    int b = min(d.NZ, i * a.seg_h);
+    if (d.fs && b == d.f) b = min(d.NZ, d.f + 1);
+    return b;
+  };
+  const int za = zbound(zseg), zb = zbound(zseg + 1);
+  if (za >= zb) return;
+  const int s0 = grp * a.spb, ns = min(min(SPB, a.spb), a.S - s0);
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x = strip * F3_OWN_X - 1 + lane;
+  const int y = ytile * F3_OWN_Y - 1 + w;  // uniform per wavefront
+  const unsigned xc = (unsigned)min(max(x, 0), d.PX - 1);
+  // the vx the window's outer lanes need from outside it: lanes 0, 1 load
+  // column x - 2, lane 63 column x + 1
+  const unsigned xh = (unsigned)min(max(lane <= 1 ? x - 2 : (lane == 63 ? x + 1 : x), 0), d.PX - 1);
+  const bool xin = x >= 0 && x < d.NX, yin = y >= 0 && y < d.NY;
+  unsigned cflags = 0;
+  cflags |= (lane >= 1 && lane <= F3_OWN_X && xin) ? F3_OWN : 0u;
+  cflags |= (x >= 2 && x <= d.NX - 3) ? F3_CP : 0u;
+  cflags |= (x >= 1 && x <= d.NX - 3) ? F3_CVX : 0u;
+  cflags |= (x >= 1 && x <= d.NX - 2) ? F3_CVY : 0u;  // vy and vz
+  cflags |= (x >= d.nabc && x < d.nabc + d.nx) ? F3_CE : 0u;
+  cflags |= xin ? F3_XIN : 0u;
+  cflags |= lane == 0 ? F3_L0 : 0u;
+  cflags |= lane <= 1 ? F3_L01 : 0u;
+  cflags |= lane == 63 ? F3_L63 : 0u;
+  const bool own_row = w >= 1 && w <= F3_OWN_Y && yin;  // uniform
+  const bool rp = y >= 2 && y <= d.NY - 3;
+  const bool rvx = y >= 1 && y <= d.NY - 2, rvy = y >= 1 && y <= d.NY - 3;  // rvx: vx and vz
+  const bool re = y >= d.nabc && y < d.nabc + d.ny;
+  auto yclamp = [&](int v) { return min(max(v, 0), d.NY - 1); };
+  const unsigned ry = (unsigned)yclamp(y), rym2 = (unsigned)yclamp(y - 2),
+                 rym1 = (unsigned)yclamp(y - 1), ryp1 = (unsigned)yclamp(y + 1);
+  const unsigned PX = (unsigned)d.PX, NY = (unsigned)d.NY;
+  auto zc = [&](int z) { return (unsigned)min(max(z, 0), d.NZ - 1); };
+  // byte offset of (z, row, column) in a plane
+  auto bo = [&](unsigned z, unsigned row, unsigned col) { return ((z * NY + row) * PX + col) * 4u; };
+  const unsigned PB = (unsigned)(d.plane * 4);  // bytes of one plane (the host checks the sizes)
+  const long long gbase = (long long)s0 * d.plane;
+  const unsigned gbytes = (unsigned)ns * PB;
+  const f3_rsrc rP = f3_buf(a.A + gbase, gbytes), rVX = f3_buf(a.A + a.fstride + gbase, gbytes),
+                rVY = f3_buf(a.A + 2 * a.fstride + gbase, gbytes),
+                rVZ = f3_buf(a.A + 3 * a.fstride + gbase, gbytes);
+  const f3_rsrc wP = f3_buf(a.B + gbase, gbytes), wVX = f3_buf(a.B + a.fstride + gbase, gbytes),
+                wVY = f3_buf(a.B + 2 * a.fstride + gbase, gbytes),
+                wVZ = f3_buf(a.B + 3 * a.fstride + gbase, gbytes);
+  const f3_rsrc rC = f3_buf(a.coef, 6u * PB);
+  const long long inner = (long long)d.nz * d.ny * d.nx;
+  const unsigned IB = (unsigned)(inner * 4);
+  const bool has_e = a.energy != nullptr;
+  const f3_rsrc rE = f3_buf(has_e ? a.energy + (long long)grp * 4 * inner : a.coef,
+                            has_e ? 4u * IB : 0u);
+  const bool erow = has_e && own_row && re;  // uniform
+  const unsigned eb = (unsigned)(((y - d.nabc) * d.nx + (x - d.nabc)) * 4);  // owned interior only
+  const unsigned EPB = (unsigned)(d.ny * d.nx * 4);  // bytes of one energy plane
+  const int cell0 = y * d.PX + x;  // cell index within a plane (valid rows and columns)
+  // per shot: source cell, source term of this step
+  int sc[SPB];
+  float wv[SPB];
+#pragma unroll
+  for (int k = 0; k < SPB; ++k) {
+    const int s = s0 + min(k, ns - 1);
+    sc[k] = k < ns ? a.src_cell[s] : -1;
+    wv[k] = k < ns ? a.wav[(long long)s * a.wav_stride + a.it] : 0.f;
+  }
+  // shot k's plane offset (inactive shots read the group's last shot)
+  auto soff = [&](int k) { return (unsigned)min(k, ns - 1) * PB; };
+  // the loads of shot k at plane z
+  auto issue = [&](int k, int z) {
+    const unsigned zz = zc(z), so = soff(k);
+    const unsigned o = bo(zz, ry, xc);
+    F3Load v;
+    v.p = f3_bld(rP, o, so);
+    v.vx = f3_bld(rVX, o, so);
+    v.vxh = f3_bld(rVX, bo(zz, ry, xh), so);
+    v.vy = f3_bld(rVY, o, so);
+    v.vym2 = f3_bld(rVY, bo(zz, rym2, xc), so);
+    v.vym1 = f3_bld(rVY, bo(zz, rym1, xc), so);
+    v.vyp1 = f3_bld(rVY, bo(zz, ryp1, xc), so);
+    v.vzn = f3_bld(rVZ, bo(zc(z + 1), ry, xc), so);
+    return v;
+  };
+  // queues: vz of planes z-2, z-1, z (old state) in the LDS ring, slot
+  // (plane + 3) % 3; p' of planes z-3, z-2, z-1 in registers
+  float pq0[SPB], pq1[SPB], pq2[SPB], hvx[SPB], hvy[SPB];
+  auto slot = [](int z) { return (z + 3) % 3; };
+#pragma unroll
+  for (int k = 0; k < SPB; ++k) {
+    vzr[slot(za - 3)][k][w][lane] = f3_bld(rVZ, bo(zc(za - 3), ry, xc), soff(k));
+    vzr[slot(za - 2)][k][w][lane] = f3_bld(rVZ, bo(zc(za - 2), ry, xc), soff(k));
+    vzr[slot(za - 1)][k][w][lane] = f3_bld(rVZ, bo(zc(za - 1), ry, xc), soff(k));
+    pq0[k] = pq1[k] = pq2[k] = 0.f;
+    hvx[k] = hvy[k] = 0.f;
+  }
+  float a2q0 = f3_bld(rC, bo(zc(za - 3), ry, xc), 5u * PB);  // a2 of plane z-2
+  float a2q1 = f3_bld(rC, bo(zc(za - 2), ry, xc), 5u * PB);  // a2 of plane z-1
+  // loads in flight: shots 0 and 1 of the first plane
+  F3Load q0 = issue(0, za - 1);
+  F3Load q1 = issue(SPB > 1 ? 1 : 0, SPB > 1 ? za - 1 : za);
+  int par = 0;
+  for (int z = za - 1; z <= zb + 1; ++z) {
+    unsigned cf = cflags;
+    asm volatile("" : "+v"(cf));  // re-derive the lane tests each plane (no SGPR masks kept)
+    const bool zown = z >= za && z < zb;  // uniform: this plane's p, vx, vy are stored
+    const int zv = z - 2;                 // the plane whose vz' this iteration completes
+    const bool vown = zv >= za && zv < zb;
+    const unsigned o = bo(zc(z), ry, xc);
+    const unsigned ov = bo(zc(zv), ry, xc);
+    const bool ez_row = erow && vown && zv >= d.nabc && zv < d.nabc + d.nz;
+    const bool e_on = erow && zown && z >= d.nabc && z < d.nabc + d.nz;  // uniform
+    const bool ecell = (cf & F3_OWN) && (cf & F3_CE);
+    // every load of the plane that is not a shot's state, issued before the
+    // shots' loads so that waiting for it never waits for those: the
+    // coefficients and the energy partials (read at a valid offset when unused)
+    const float omk1 = f3_bld(rC, o, 0u), a1 = f3_bld(rC, o, PB);
+    const float omk4v = f3_bld(rC, ov, 4u * PB);
+    const float a2z = f3_bld(rC, o, 5u * PB);
+    const float omk2 = f3_bld(rC, o, 2u * PB), omk3 = f3_bld(rC, o, 3u * PB);
+    const unsigned oez = (ez_row && ecell) ? (unsigned)(zv - d.nabc) * EPB + eb : 0u;
+    const unsigned oe = (e_on && ecell) ? (unsigned)(z - d.nabc) * EPB + eb : 0u;
+    const float ez0 = f3_bld(rE, oez, 3u * IB);
+    const float ep0 = f3_bld(rE, oe, 0u), ex0 = f3_bld(rE, oe, IB), ey0 = f3_bld(rE, oe, 2u * IB);
+    const bool zP = z >= d.f + 1 && z <= d.NZ - 3 && rp;  // uniform parts of the regions
+    const bool zVZ = zv >= d.f && zv <= d.NZ - 3 && rvx;
+    const bool zvalid = yin && z >= 0 && z < d.NZ;
+    const int cell = z * d.NY * d.PX + cell0;
+    const bool mirror_vz = d.fs && zv == d.f;  // vz' of plane f reads p'(f-1) = -p'(f+1)
+    const bool st_row = own_row;
+    float ez = 0.f;
+    const int s_m2 = slot(z - 2), s_m1 = slot(z - 1), s_00 = slot(z);
+    // ---- phase P ----
+#pragma unroll
+    for (int k = 0; k < SPB; ++k) {
+      const float vzm2 = vzr[s_m2][k][w][lane], vzm1 = vzr[s_m1][k][w][lane],
+                  vz00 = vzr[s_00][k][w][lane];
+      const F3Load v = q0;
+      q0 = q1;
+      // two (plane, shot) pairs ahead
+      q1 = issue((k + 2) % SPB, z + (k + 2) / SPB);
+      const bool act = k < ns;  // uniform
+      const float l1 = f3_prev(v.vx), l2 = f3_prev(l1), r1 = f3_next(v.vx), hn = f3_next(v.vxh);
+      const float vm1 = (cf & F3_L0) ? hn : l1;
+      const float vm2 = (cf & F3_L01) ? v.vxh : l2;
+      const float vp1 = (cf & F3_L63) ? v.vxh : r1;
+      const float dvx = d4(v.vx, vm1, vp1, vm2);
+      const float dvy = d4(v.vy, v.vym1, v.vyp1, v.vym2);
+      const float dvz = d4(vz00, vzm1, v.vzn, vzm2);
+      float sum = dvx + dvy;
+      sum = sum + dvz;
+      const float u = omk1 * v.p;
+      const float pt = u - a1 * sum;
+      float pn = (zP && (cf & F3_CP)) ? pt : v.p;
+      if (zvalid && (cf & F3_XIN) && cell == sc[k]) pn = pn + a.dt * wv[k];
+      // vz' of plane z-2: p' of planes z-3 (pq0), z-2 (pq1), z-1 (pq2), z (pn)
+      const float pu = mirror_vz ? -pq2[k] : pq0[k];
+      const float vzo = vzm2;
+      const float uz = omk4v * vzo;
+      const float vzt = uz - a2q0 * d4(pq2[k], pq1[k], pn, pu);
+      const float vzf = (zVZ && (cf & F3_CVY)) ? vzt : vzo;
+      if (act) {
+        if (zown) lp[par][k][w][lane] = pn;
+        const unsigned so = (unsigned)k * PB;
+        if (st_row && (cf & F3_OWN)) {
+          if (vown) {
+            if (!(d.fs && zv == d.f - 1)) f3_bst(wVZ, ov, so, vzf);
+            if (mirror_vz) f3_bst(wVZ, ov - NY * PX * 4u, so, vzf);  // vz[f-1] = vz[f]
+            if (ez_row) ez = ez + vzf * vzf;
+          }
+          if (zown && !(d.fs && z == d.f - 1)) f3_bst(wP, o, so, pn);
+          if (d.fs && z == d.f + 1 && d.f - 1 >= za && d.f - 1 < zb)
+            f3_bst(wP, o - 2u * NY * PX * 4u, so, -pn);  // p[f-1] = -p[f+1]
+        }
+      }
+      vzr[s_m2][k][w][lane] = v.vzn;  // plane z+1 takes plane z-2's slot
+      pq0[k] = pq1[k]; pq1[k] = pq2[k]; pq2[k] = pn;
+      hvx[k] = v.vx; hvy[k] = v.vy;
+    }
+    if (ez_row && ecell) f3_bst(rE, oez, 3u * IB, ez0 + ez);
+    a2q0 = a2q1;
+    a2q1 = a2z;
+    if (zown) {
+      __syncthreads();
+      // ---- phase V ----
+      if (own_row) {
+        const bool zv_in = z >= d.f && z <= d.NZ - 2;
+        const bool inVX = zv_in && rvx && (cf & F3_CVX), inVY = zv_in && rvy && (cf & F3_CVY);
+        float ep = 0.f, ex = 0.f, ey = 0.f;
+#pragma unroll
+        for (int k = 0; k < SPB; ++k) {
+          if (k >= ns) break;  // uniform
+          const unsigned so = (unsigned)k * PB;
+          const float pc = pq2[k];
+          const float pm1 = f3_prev(pc), pp1 = f3_next(pc), pp2 = f3_next(pp1);
+          const float pym1 = lp[par][k][w - 1][lane], pyp1 = lp[par][k][w + 1][lane],
+                      pyp2 = lp[par][k][w + 2][lane];
+          const float ux = omk2 * hvx[k];
+          const float tvx = ux - a2z * d4(pp1, pc, pp2, pm1);
+          const float uy = omk3 * hvy[k];
+          const float tvy = uy - a2z * d4(pyp1, pc, pyp2, pym1);
+          const float vxf = inVX ? tvx : hvx[k];
+          const float vyf = inVY ? tvy : hvy[k];
+          if (cf & F3_OWN) {
+            f3_bst(wVX, o, so, vxf);
+            f3_bst(wVY, o, so, vyf);
+          }
+          if (e_on) {
+            ep = ep + pc * pc;
+            ex = ex + vxf * vxf;
+            ey = ey + vyf * vyf;
+          }
+        }
+        if (e_on && ecell) {
+          f3_bst(rE, oe, 0u, ep0 + ep);
+          f3_bst(rE, oe, IB, ex0 + ex);
+          f3_bst(rE, oe, 2u * IB, ey0 + ey);
+        }
+      }
+      par ^= 1;
+    }
+  }
+}
+
 // shared wavelet: grad[it] = sum over shots of the shot's source term, in shot order
 __global__ void wavelet_sum3_kernel(const float* gws, int S, float* gw, int it) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -1151,8 +1466,10 @@ struct Plan3 {
   Dims3 d;
   int S, R, nt, spb, groups, echunk, k, nseg;
   bool march;                       // fused z-marching step (state in and out of place)
+  bool fused;                       // fused step with rows in wavefronts (state out of place)
   int tiles_x, tiles_y, zc;         // z-marching decomposition
   long long units;
+  int f_strips, f_ytiles;           // fused step: 61 x 13 windows
   size_t off_coef, off_state, off_E, off_src, off_rcell;
   size_t off_lam, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
       off_urcv, off_rowstart;
@@ -1186,12 +1503,26 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   // in-volume float4 accesses use 32-bit byte offsets
   if (d.plane >= (1LL << 30)) return fail(GBW_ERR_INVALID, "padded volume exceeds 2^30 cells");
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
-  if (p->step_kernel != 0 && p->step_kernel != 1)
-    return fail(GBW_ERR_INVALID, "step_kernel must be 0 or 1");
+  if (p->step_kernel < 0 || p->step_kernel > 2)
+    return fail(GBW_ERR_INVALID, "step_kernel must be 0, 1 or 2");
   if (p->z_chunk < 0) return fail(GBW_ERR_INVALID, "z_chunk must be >= 0");
   pl.march = p->step_kernel == 1;
+  // the fused step runs the forward pass; the gradient's recompute keeps the
+  // in-place p / v kernels
+  pl.fused = p->step_kernel == 2 && pass == 0;
   int spb;
-  if (pl.march) {
+  if (pl.fused) {
+    // its buffer accesses take 32-bit byte offsets: the six coefficient planes
+    // and a group's planes of one field must each stay below 4 GiB
+    const long long pb = d.plane * 4;
+    if (6 * pb > 0xFFFFFFFFLL)
+      return fail(GBW_ERR_INVALID, "the fused step needs a padded volume below 2^28.4 cells");
+    const int most = (int)std::min<long long>(8, 0xFFFFFFFFLL / pb);
+    spb = p->shots_per_block > 0 ? std::min(p->shots_per_block, pl.S) : std::min(pl.S, most);
+    if (spb > most)
+      return fail(GBW_ERR_INVALID, "shots_per_block must be at most " + std::to_string(most) +
+                                       " with the fused step on this grid");
+  } else if (pl.march) {
     spb = p->shots_per_block > 0 ? p->shots_per_block : 4;
     if (spb != 1 && spb != 2 && spb != 4)
       return fail(GBW_ERR_INVALID, "shots_per_block must be 0, 1, 2 or 4 with the z-marching kernel");
@@ -1211,6 +1542,8 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
     pl.zc = p->z_chunk > 0 ? std::min(p->z_chunk, d.NZ) : (int)((d.NZ + nzc - 1) / nzc);
     pl.units = per_z * ((d.NZ + pl.zc - 1) / pl.zc);
   }
+  pl.f_strips = (d.NX + F3_OWN_X - 1) / F3_OWN_X;
+  pl.f_ytiles = (d.NY + F3_OWN_Y - 1) / F3_OWN_Y;
   pl.echunk = p->energy_chunk > 0 ? std::min(p->energy_chunk, p->nt) : p->nt;
   pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
   pl.nseg = pl.k > 0 ? (pl.nt + pl.k - 1) / pl.k : 0;
@@ -1220,7 +1553,7 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t at = o; o += al256(bytes); return at; };
   pl.off_coef = take(6 * (size_t)d.plane * sizeof(float));
-  pl.off_state = take((pl.march ? 8 : 4) * fld);
+  pl.off_state = take((pl.march || pl.fused ? 8 : 4) * fld);
   pl.off_src = take((size_t)pl.S * sizeof(int));
   pl.off_rcell = take(R1 * sizeof(int));
   if (pass == 0) {
@@ -1319,6 +1652,48 @@ int march_step3(const Plan3& pl, March3Args& m, bool rec, hipStream_t st) {
   return 0;
 }
 
+// z segments of the fused step: the count that best fills the GPU's compute
+// units in whole rounds of resident workgroups (one per CU), discounted by the
+// three planes of p' each segment recomputes at its ends
+int fused_segments(const Plan3& pl, int z_chunk, int cus) {
+  const int NZ = pl.d.NZ;
+  if (z_chunk > 0) return std::max(1, (NZ + z_chunk - 1) / z_chunk);
+  const long long cols = (long long)pl.f_strips * pl.f_ytiles * pl.groups;
+  int best = 1;
+  double best_eff = 0.0;
+  for (int zs = 1; zs <= std::max(1, NZ / 8); ++zs) {
+    const int h = (NZ + zs - 1) / zs;
+    const long long units = cols * ((NZ + h - 1) / h);
+    const long long rounds = (units + cus - 1) / cus;
+    const double eff = (double)units / (double)(rounds * cus) * h / (h + 3.0);
+    if (eff > best_eff + 1e-9) { best_eff = eff; best = zs; }
+  }
+  return best;
+}
+
+// one fused step: state A -> B
+int fused_step3(const Plan3& pl, Fused3Args& f, int z_chunk, hipStream_t st) {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = v;
+  }
+  const int zs = fused_segments(pl, z_chunk, cus);
+  f.seg_h = (pl.d.NZ + zs - 1) / zs;
+  f.strips = pl.f_strips; f.ytiles = pl.f_ytiles; f.groups = pl.groups; f.spb = pl.spb;
+  f.units = (long long)pl.f_strips * pl.f_ytiles * pl.groups * ((pl.d.NZ + f.seg_h - 1) / f.seg_h);
+  const unsigned blocks = (unsigned)((f.units + 7) / 8 * 8);
+  LaunchTimer timer("fwd3f_kernel", st);
+  // the instance holds the next power of two of the group's shots
+  if (pl.spb <= 1) fwd3f_kernel<1><<<blocks, F3_NW * 64, 0, st>>>(f);
+  else if (pl.spb <= 2) fwd3f_kernel<2><<<blocks, F3_NW * 64, 0, st>>>(f);
+  else if (pl.spb <= 4) fwd3f_kernel<4><<<blocks, F3_NW * 64, 0, st>>>(f);
+  else fwd3f_kernel<8><<<blocks, F3_NW * 64, 0, st>>>(f);
+  LAUNCH_CHECK();
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1388,13 +1763,20 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
   March3Args m{};
   m.d = d; m.S = pl.S; m.nt = pl.nt; m.dt = prob->dt; m.fstride = fstride; m.coef = coef;
   m.src_cell = a.src_cell; m.wav = a.wav; m.wav_stride = a.wav_stride; m.energy = a.energy;
-  if (pl.march && (rc = fill3(nxt, 4 * fstride, 0.f, st))) return rc;
+  Fused3Args fa{};
+  fa.d = d; fa.S = pl.S; fa.nt = pl.nt; fa.dt = prob->dt; fa.fstride = fstride; fa.coef = coef;
+  fa.src_cell = a.src_cell; fa.wav = a.wav; fa.wav_stride = a.wav_stride; fa.energy = a.energy;
+  if ((pl.march || pl.fused) && (rc = fill3(nxt, 4 * fstride, 0.f, st))) return rc;
   for (int it = 0; it < pl.nt; ++it) {
     if (pl.k > 0 && it % pl.k == 0)
       HIP_TRY(hipMemcpyAsync(static_cast<char*>(out->checkpoints) +
                                  (size_t)(it / pl.k) * 4 * fstride * sizeof(float),
                              cur, 4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
-    if (pl.march) {
+    if (pl.fused) {
+      fa.it = it; fa.A = cur; fa.B = nxt;
+      if ((rc = fused_step3(pl, fa, prob->z_chunk, st))) return rc;
+      std::swap(cur, nxt);
+    } else if (pl.march) {
       m.it = it; m.A = cur; m.B = nxt;
       if ((rc = march_step3(pl, m, false, st))) return rc;
       std::swap(cur, nxt);

@@ -456,10 +456,12 @@ def pad_model_3d(v: Tensor, pml: int, nz: int, ny: int, nx: int,
 @dataclass
 class LaunchChoice3D:
     """Launch parameters of the 3D library calls: ``step_kernel`` 0 (separate
-    p and v kernels, default) or 1 (fused z-marching step);
+    p and v kernels, default), 1 (fused z-marching step, tiles in workgroups)
+    or 2 (fused step, rows in wavefronts; forward pass only);
     ``shots_per_block`` (any with the p / v kernels; 1, 2 or 4 with the
-    z-marching kernel); ``z_chunk`` (z planes per z-marching workgroup).
-    Zeros select the library's defaults."""
+    z-marching kernel; 1 to 8 with the fused step); ``z_chunk`` (z
+    planes per z-marching workgroup or fused-step segment).  Zeros select the
+    library's defaults."""
     shots_per_block: int = 0
     step_kernel: int = 0
     z_chunk: int = 0

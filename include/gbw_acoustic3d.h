@@ -43,10 +43,13 @@ typedef struct gbw_acoustic3d_problem {
   int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
   int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
   int32_t shots_per_block;   /* launch choice: shots per workgroup (0 = all); 1, 2 or 4
-                                with the z-marching kernel (0 = 4)                     */
+                                with the z-marching kernel (0 = 4); 1 to 8 with the
+                                fused step (0 = min(src_n, 8))                         */
   int32_t checkpoint_every;  /* steps per checkpoint interval (0 = no checkpoints)     */
   int32_t step_kernel;       /* launch choice: 0 separate p and v step kernels (default),
-                                1 fused z-marching step                                */
+                                1 fused z-marching step (tiles in workgroups),
+                                2 fused step with rows in wavefronts (forward pass;
+                                the gradient's recompute keeps the p and v kernels)   */
   int32_t z_chunk;           /* launch choice: z planes per z-marching workgroup (0 = auto) */
 } gbw_acoustic3d_problem;
 

@@ -292,10 +292,16 @@ def test_workspace_queries_3d():
                      (dict(nabc=0), "nabc"), (dict(nt=0), "nt must be positive"),
                      (dict(src_n=0), "src_n"), (dict(dt=0.0), "dt and dz"),
                      (dict(nx=1100, ny=1000, nz=1000), "exceeds 2^30 cells"),
-                     (dict(step_kernel=2), "step_kernel"),
-                     (dict(step_kernel=1, shots_per_block=3), "shots_per_block")):
+                     (dict(step_kernel=3), "step_kernel"),
+                     (dict(step_kernel=1, shots_per_block=3), "shots_per_block"),
+                     (dict(step_kernel=2, src_n=12, shots_per_block=9), "shots_per_block")):
         assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(**bad)), 0) == 0
         assert msg in _native.last_error()
+    # the fused step keeps two states (in and out of place); its gradient's
+    # recompute runs the in-place kernels
+    fused = lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(step_kernel=2)), 0)
+    assert fused >= ws + 4 * 4 * plane
+    assert lib.gbw_acoustic3d_workspace_bytes(ctypes.byref(_problem3(step_kernel=2, checkpoint_every=50)), 1) == bwd
 
 
 def test_product_path_3d_refuses_cpu_tensors():
