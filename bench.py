@@ -654,7 +654,7 @@ def main():
     ap.add_argument("--launch", type=str, default=None,
                     help="pin a launch choice instead of autotuning: comma-separated "
                          "tile_rows,shots_per_block,march,bwd_tile_rows,bwd_shots_per_block,"
-                         "steps_per_launch")
+                         "steps_per_launch[,fwd_march]")
     ap.add_argument("--workload", choices=["gradient", "fwi-bf16", "fwd3d"], default="gradient",
                     help="gradient: the 64-shot batched gradient (default, plus the uneven "
                          "batch and the 3D forward); fwi-bf16: the BF16-checkpoint FWI "
@@ -782,10 +782,10 @@ def main():
                      "adj_step_kernel": 28.0, "recomp_march_kernel": 28.0}
     kernels = {}
     for name, (launches, ms) in ktimes.items():
-        # adj2m_kernel: the z-marching two-step adjoint
+        # adj2m_kernel / fwd2m_kernel: the z-marching two-step adjoint / forward
         base = name.replace("fwd2_", "fwd_").replace("adj2_", "adj_").replace(
-            "adj2m_kernel", "adj_step_kernel")
-        steps_per = 2 if ("fwd2_" in name or "adj2" in name) else 1
+            "adj2m_kernel", "adj_step_kernel").replace("fwd2m_kernel", "fwd_step_kernel")
+        steps_per = 2 if ("fwd2" in name or "adj2" in name) else 1
         b = per_cell_step[base] * steps_per * S * NZ * NX
         avg_s = ms / launches / 1e3
         kernels[name] = {"launches": launches, "total_s": round(ms / 1e3, 4),

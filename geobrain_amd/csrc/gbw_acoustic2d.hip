@@ -141,6 +141,7 @@ struct FwdArgs {
   Dims d;
   int S, spb, tiles_x, ntiles;   // shots, shots per workgroup, tiles along x, workgroup units
   int seg_h;                     // z-marching kernels: rows per segment
+  int am_segs;                   // z-marching two-step forward: segments per column strip
   const float* P0; const float* VX0; const float* VZ0;  // input state fields (shot-major)
   float* P1; float* VX1; float* VZ1;                    // output state fields
   const float* coef;
@@ -2645,6 +2646,422 @@ __global__ __launch_bounds__(NW * 64) void adj2m_kernel(AdjArgs a) {
     adj2m_body<NW, true>(a, group, seg, strip, sT, sC);
 }
 
+// ---------------------------------------------------------------------------
+// Two forward steps (it, it+1) by z-marching wavefronts (launch choice
+// fwd_march = 2; no free surface, no density-gradient brackets).  The window,
+// segment and shot-group geometry is adj2m_kernel's: one wavefront holds one
+// shot on a 128-column window (2 cells per lane, 116 owned) and walks a z
+// segment top to bottom; the NW wavefronts of a workgroup are the shots of one
+// shot group and share the coefficient rows through the LDS ring.  Iteration
+// i finishes row i of the second step and works on five rows at once:
+//   row i+6: vz of the input state (loaded; p and vx of row i+5 with it);
+//   row i+5: p' of step it (vx neighbours by DPP, vz rows i+3 .. i+6);
+//   row i+3: vx', vz' of step it (p' rows i+2 .. i+5), its receivers, energy;
+//   row i+2: p'' of step it+1 (vx', vz' rows i .. i+3);
+//   row i:   vx'', vz'' of step it+1 (p'' rows i-1 .. i+2); row i stored.
+// The per-cell arithmetic is fwd2_step_kernel's, in the same order, so the
+// state, the seismograms and the recompute brackets equal it bit for bit.
+// Energy: each wavefront adds its shot's squares of the two steps, and every
+// 4 rows the workgroup sums them over its shots in shot order and adds them to
+// the group's partial sums (the same sums as the tiled kernel, associated per
+// shot first; the maps agree with it to rounding).
+// ---------------------------------------------------------------------------
+constexpr int FM_ESLOTS = 8;  // LDS row slots of the energy squares: two 4-row blocks
+
+template <int NW, bool CM, bool RECOMP>
+__device__ __forceinline__ void fwd2m_body(const FwdArgs& a, int group, int seg, int strip,
+                                           float2 (*sE)[NW][FM_ESLOTS][64],
+                                           float2 (*sC)[AM_CROWS][64]) {
+  const Dims d = a.d;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s_begin = group * a.spb;
+  const int ns = min(a.S, s_begin + a.spb) - s_begin;
+  const bool active = w < ns;  // uniform per wavefront
+  const int s = s_begin + (active ? w : 0);
+  const int zs = seg * a.seg_h, ze = min(d.NZ, zs + a.seg_h);
+  const int xs = strip * AM_OWN - 6;
+  const int x0 = xs + 2 * lane;
+  const bool lane_ok = x0 >= 0 && x0 <= d.PX - 2;
+  const unsigned xo4 = (unsigned)min(max(x0, 0), d.PX - 2) * 4u;  // lane byte offset in a row
+  const bool own_lane = lane >= 3 && lane < 3 + AM_OWN / 2 && x0 < d.NX;
+  // column tests of the lane's two cells: in the grid; P, X, Z regions
+  bool cin[2], cp[2], cx[2], cz[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int x = x0 + e;
+    cin[e] = lane_ok && x < d.NX;
+    cp[e] = cin[e] && x >= 2 && x <= d.NX - 3;
+    cx[e] = cin[e] && x >= 1 && x <= d.NX - 3;
+    cz[e] = cin[e] && x >= 1 && x <= d.NX - 2;
+  }
+  const long long so = (long long)s * d.plane;
+  const float* P0 = a.P0 + so;
+  const float* VX0 = a.VX0 + so;
+  const float* VZ0 = a.VZ0 + so;
+  float* P1 = a.P1 + so;
+  float* VX1 = a.VX1 + so;
+  float* VZ1 = a.VZ1 + so;
+  float* D0 = RECOMP ? a.dbuf + so : nullptr;
+  float* D1 = RECOMP ? a.dbuf + a.bstride + so : nullptr;
+  const int scell = a.src_cell[s];
+  const int ssz = scell / d.PX, ssx = scell - ssz * d.PX;
+  const float* wv = a.wav + (long long)s * a.wav_stride + a.it;
+  const float supd0 = a.dt * wv[0], supd1 = a.dt * wv[1];
+  const int it = a.it;
+  const bool rec = !RECOMP && a.rec_p != nullptr;
+  const float2* sCl = &sC[0][0][lane];  // this lane's column in the coefficient ring
+  auto row_in = [&](int r) { return r >= 0 && r < d.NZ; };
+  auto row_p = [&](int r) { return r >= d.f + 1 && r <= d.NZ - 3; };
+  auto row_x = [&](int r) { return r >= d.f && r <= d.NZ - 2; };
+  auto row_z = [&](int r) { return r >= d.f && r <= d.NZ - 3; };
+  auto seg_row = [&](int r) { return r >= zs && r < ze; };
+  // the receivers of row r record p, vx, vz of step `step` (same list order as
+  // the tile kernels; one lane per cell)
+  auto record = [&](int r, int step, const float2& pv, const float2& vx, const float2& vz) {
+    const int2 rh = *reinterpret_cast<const int2*>(
+        reinterpret_cast<const char*>(a.rhead) + (long long)r * d.PX * 4 + xo4);
+    const long long ob = ((long long)s * a.nt + step) * a.R;
+    for (int hh = rh.x; hh >= 0; hh = a.rnext[hh]) {
+      a.rec_p[ob + hh] = pv.x;
+      a.rec_vx[ob + hh] = vx.x;
+      a.rec_vz[ob + hh] = vz.x;
+    }
+    for (int hh = rh.y; hh >= 0; hh = a.rnext[hh]) {
+      a.rec_p[ob + hh] = pv.y;
+      a.rec_vx[ob + hh] = vx.y;
+      a.rec_vz[ob + hh] = vz.y;
+    }
+  };
+  // p-update of row r (acoustic_2d.py:213-237): vx of the row (neighbours by
+  // DPP), vz of rows r-2 .. r+1, the old p; cells outside the P region keep p,
+  // rows outside the grid are zero.  Returns p'; br: the bracket (0 off the region)
+  auto p_upd = [&](auto G, int r, const float2* cr, const float2& p, const float2& vx,
+                   const float2& zm2, const float2& zm1, const float2& z00, const float2& zp1,
+                   float2& br) {
+    const float vl0 = lane_prev(vx.x), vl1 = lane_prev(vx.y), vr0 = lane_next(vx.x);
+    const float2 o1 = cr[0], c1 = cr[1 * AM_CROWS * 64];
+    float2 pn, b;
+    {
+      float s1 = vx.x - vl1;
+      s1 = s1 + z00.x;
+      s1 = s1 - zm1.x;
+      float s2 = vx.y - vl0;
+      s2 = s2 + zp1.x;
+      s2 = s2 - zm2.x;
+      const float t1 = kC1 * s1;
+      const float t2 = kC2 * s2;
+      b.x = t1 + t2;
+      const float u = o1.x * p.x;
+      pn.x = u - c1.x * b.x;
+    }
+    {
+      float s1 = vx.y - vx.x;
+      s1 = s1 + z00.y;
+      s1 = s1 - zm1.y;
+      float s2 = vr0 - vl1;
+      s2 = s2 + zp1.y;
+      s2 = s2 - zm2.y;
+      const float t1 = kC1 * s1;
+      const float t2 = kC2 * s2;
+      b.y = t1 + t2;
+      const float u = o1.y * p.y;
+      pn.y = u - c1.y * b.y;
+    }
+    bool rp = true;
+    if constexpr (decltype(G)::value) rp = row_p(r);
+    const bool m0 = rp && (!CM || cp[0]), m1 = rp && (!CM || cp[1]);
+    if constexpr (CM || decltype(G)::value) {
+      if (!m0) { pn.x = p.x; b.x = 0.f; }
+      if (!m1) { pn.y = p.y; b.y = 0.f; }
+    }
+    if constexpr (decltype(G)::value) {
+      if (!row_in(r)) pn = f2zero();
+    }
+    br = b;
+    return pn;
+  };
+  // v-update of row r (acoustic_2d.py:243-265) from p' of rows r-1 .. r+2 (x
+  // neighbours of row r by DPP); cells outside the X / Z regions keep v
+  auto v_upd = [&](auto G, int r, const float2* cr, const float2& pu, const float2& pc,
+                   const float2& pd1, const float2& pd2, float2& vx, float2& vz) {
+    const float pl1 = lane_prev(pc.y), pr0 = lane_next(pc.x), pr1 = lane_next(pc.y);
+    const float2 o2 = cr[2 * AM_CROWS * 64], o3 = cr[3 * AM_CROWS * 64], c2 = cr[4 * AM_CROWS * 64];
+    float2 nx, nz;
+    {
+      const float d1 = pc.y - pc.x;
+      const float d2 = pr0 - pl1;
+      const float b = kC1 * d1 + kC2 * d2;
+      const float u = o2.x * vx.x;
+      nx.x = u - c2.x * b;
+    }
+    {
+      const float d1 = pr0 - pc.y;
+      const float d2 = pr1 - pc.x;
+      const float b = kC1 * d1 + kC2 * d2;
+      const float u = o2.y * vx.y;
+      nx.y = u - c2.y * b;
+    }
+    {
+      const float d1 = pd1.x - pc.x;
+      const float d2 = pd2.x - pu.x;
+      const float b = kC1 * d1 + kC2 * d2;
+      const float u = o3.x * vz.x;
+      nz.x = u - c2.x * b;
+    }
+    {
+      const float d1 = pd1.y - pc.y;
+      const float d2 = pd2.y - pu.y;
+      const float b = kC1 * d1 + kC2 * d2;
+      const float u = o3.y * vz.y;
+      nz.y = u - c2.y * b;
+    }
+    bool rx = true, rz = true;
+    if constexpr (decltype(G)::value) { rx = row_x(r); rz = row_z(r); }
+    const bool mx0 = rx && (!CM || cx[0]), mx1 = rx && (!CM || cx[1]);
+    const bool mz0 = rz && (!CM || cz[0]), mz1 = rz && (!CM || cz[1]);
+    if constexpr (CM || decltype(G)::value) {
+      if (!mx0) nx.x = vx.x;
+      if (!mx1) nx.y = vx.y;
+      if (!mz0) nz.x = vz.x;
+      if (!mz1) nz.y = vz.y;
+    }
+    if constexpr (decltype(G)::value) {
+      if (!row_in(r)) { nx = f2zero(); nz = f2zero(); }
+    }
+    vx = nx;
+    vz = nz;
+  };
+  auto src_add = [&](float2& pn, float supd) {
+    if (x0 == ssx && cin[0]) pn.x = pn.x + supd;
+    if (x0 + 1 == ssx && cin[1]) pn.y = pn.y + supd;
+  };
+  auto sq = [](const float2& v) { return make_float2(v.x * v.x, v.y * v.y); };
+
+  // four-row register windows, slot = row & 3
+  float2 WVZ[4], WVX[4], WP1[4], WVX1[4], WVZ1[4], WP2[4], WEP[4], WEX[4], WEZ[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    WVZ[k] = WVX[k] = WP1[k] = WVX1[k] = WVZ1[k] = WP2[k] = f2zero();
+    WEP[k] = WEX[k] = WEZ[k] = f2zero();
+  }
+  // this shot's rows, loaded AM_PF iterations ahead: vz of row i+6, p and vx
+  // of row i+5 (rows clamped into what the segment needs)
+  const int ja = max(0, zs - 6), jb = min(d.NZ, ze + 6) - 1;
+  float2 pf_p[AM_PF], pf_vx[AM_PF], pf_vz[AM_PF];
+  auto prefetch = [&](auto K, int i) {
+    constexpr int k = decltype(K)::value;
+    const int j6 = min(max(i + 6, ja), jb), j5 = min(max(i + 5, ja), jb);
+    pf_vz[k] = row_ld2(VZ0, j6, d.PX, xo4);
+    pf_p[k] = row_ld2(P0, j5, d.PX, xo4);
+    pf_vx[k] = row_ld2(VX0, j5, d.PX, xo4);
+  };
+
+  auto iter = [&](auto U, auto G, int base) {
+    constexpr int uu = decltype(U)::value;
+    constexpr bool GG = decltype(G)::value;
+    const int i = base + uu;
+    constexpr int pk = uu % AM_PF;
+    float2 vz_in = pf_vz[pk], p_in = pf_p[pk], vx_in = pf_vx[pk];
+    prefetch(std::integral_constant<int, pk>{}, i + AM_PF);
+    if constexpr (GG) {
+      if (!row_in(i + 6)) vz_in = f2zero();
+      if (!row_in(i + 5)) { p_in = f2zero(); vx_in = f2zero(); }
+    }
+    if constexpr (CM) {
+      if (!lane_ok) { vz_in = f2zero(); p_in = f2zero(); vx_in = f2zero(); }
+    }
+    // --- row i+6: the input vz ---
+    WVZ[(uu + 6) & 3] = vz_in;
+    // --- row r1 = i+5: p' of step it ---
+    {
+      const int r1 = i + 5;
+      float2 br;
+      float2 pn = p_upd(G, r1, sCl + (r1 & (AM_CROWS - 1)) * 64, p_in, vx_in, WVZ[(uu + 3) & 3],
+                        WVZ[(uu + 4) & 3], WVZ[(uu + 5) & 3], WVZ[(uu + 6) & 3], br);
+      if constexpr (GG) {
+        if (r1 == ssz) src_add(pn, supd0);
+      }
+      WP1[(uu + 5) & 3] = pn;
+      WVX[(uu + 5) & 3] = vx_in;
+      if constexpr (RECOMP) {
+        if ((!GG || seg_row(r1)) && own_lane) row_st2(D0, r1, d.PX, xo4, br);
+      }
+    }
+    // --- row r3 = i+3: vx', vz' of step it; its receivers and energy squares ---
+    {
+      constexpr int s3 = (uu + 3) & 3;
+      const int r3 = i + 3;
+      float2 vx = WVX[s3], vz = WVZ[s3];
+      v_upd(G, r3, sCl + (r3 & (AM_CROWS - 1)) * 64, WP1[(uu + 2) & 3], WP1[s3], WP1[uu & 3],
+            WP1[(uu + 1) & 3], vx, vz);
+      WVX1[s3] = vx;
+      WVZ1[s3] = vz;
+      if constexpr (!RECOMP) {
+        WEP[s3] = sq(WP1[s3]);
+        WEX[s3] = sq(vx);
+        WEZ[s3] = sq(vz);
+        if constexpr (GG) {
+          if (rec && own_lane && seg_row(r3) && a.rrow[r3]) record(r3, it, WP1[s3], vx, vz);
+        }
+      }
+    }
+    // --- row r2 = i+2: p'' of step it+1 ---
+    {
+      const int r2 = i + 2;
+      float2 br;
+      float2 pn = p_upd(G, r2, sCl + (r2 & (AM_CROWS - 1)) * 64, WP1[(uu + 2) & 3],
+                        WVX1[(uu + 2) & 3], WVZ1[uu & 3], WVZ1[(uu + 1) & 3], WVZ1[(uu + 2) & 3],
+                        WVZ1[(uu + 3) & 3], br);
+      if constexpr (GG) {
+        if (r2 == ssz) src_add(pn, supd1);
+      }
+      WP2[(uu + 2) & 3] = pn;
+      if constexpr (RECOMP) {
+        if ((!GG || seg_row(r2)) && own_lane) row_st2(D1, r2, d.PX, xo4, br);
+      }
+    }
+    // --- row i: vx'', vz'' of step it+1; the row is stored ---
+    {
+      constexpr int s0 = uu & 3;
+      float2 vx = WVX1[s0], vz = WVZ1[s0];
+      const float2 pc = WP2[s0];
+      v_upd(G, i, sCl + (i & (AM_CROWS - 1)) * 64, WP2[(uu + 3) & 3], pc, WP2[(uu + 1) & 3],
+            WP2[(uu + 2) & 3], vx, vz);
+      if (!GG || seg_row(i)) {
+        if (own_lane) {
+          row_st2(P1, i, d.PX, xo4, pc);
+          row_st2(VX1, i, d.PX, xo4, vx);
+          row_st2(VZ1, i, d.PX, xo4, vz);
+        }
+        if constexpr (!RECOMP) {
+          if constexpr (GG) {
+            if (rec && own_lane && a.rrow[i]) record(i, it + 1, pc, vx, vz);
+          }
+          const int sl = i & (FM_ESLOTS - 1);
+          const float2 ep = sq(pc), ex = sq(vx), ez = sq(vz);
+          sE[0][w][sl][lane] = make_float2(WEP[s0].x + ep.x, WEP[s0].y + ep.y);
+          sE[1][w][sl][lane] = make_float2(WEX[s0].x + ex.x, WEX[s0].y + ex.y);
+          sE[2][w][sl][lane] = make_float2(WEZ[s0].x + ez.x, WEZ[s0].y + ez.y);
+        }
+      }
+    }
+  };
+
+  // coefficient ring: per block 5 planes x 4 rows, one row-plane per wavefront
+  // at a time (uniform row base, lane offset)
+  constexpr int CRP = 20, CPW = (CRP + NW - 1) / NW;
+  float2 cpf[CPW];
+  auto coef_load = [&](int r0) {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int rp = w + k * NW;  // plane * 4 + row
+      if (rp < CRP) {
+        const int r = min(max(r0 + (rp & 3), 0), d.NZ - 1);
+        cpf[k] = row_ld2(a.coef + (rp >> 2) * d.plane, r, d.PX, xo4);
+      }
+    }
+  };
+  auto coef_store = [&](int r0) {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      const int rp = w + k * NW;
+      if (rp < CRP) sC[rp >> 2][(r0 + (rp & 3)) & (AM_CROWS - 1)][lane] = cpf[k];
+    }
+  };
+  const int i0 = zs - 12;
+  // rows i0 .. i0+11 for the first block
+  coef_load(i0); coef_store(i0);
+  coef_load(i0 + 4); coef_store(i0 + 4);
+  coef_load(i0 + 8); coef_store(i0 + 8);
+  __syncthreads();
+  if (active) {
+    prefetch(std::integral_constant<int, 0>{}, i0);
+    if constexpr (AM_PF > 1) prefetch(std::integral_constant<int, 1 % AM_PF>{}, i0 + 1);
+    if constexpr (AM_PF > 2) {
+      prefetch(std::integral_constant<int, 2 % AM_PF>{}, i0 + 2);
+      prefetch(std::integral_constant<int, 3 % AM_PF>{}, i0 + 3);
+    }
+  }
+
+  constexpr int RED = 4 * 128 / (NW * 64);  // reduced cells per thread per block and field
+  const long long inner = (long long)d.nz * d.nx;
+  float* __restrict__ eg = (!RECOMP && a.energy) ? a.energy + (long long)group * 3 * inner : nullptr;
+  for (int base = i0; base < ze; base += 4) {
+    // coefficient rows of the next block (base+12..base+15; this block uses
+    // base..base+8, the ring rows base+9 .. base+11 are already there)
+    coef_load(base + 12);
+    // the energy partials this block completes (rows base..base+3), loaded early
+    float ev[3][RED];
+    long long eo[RED];
+    bool eok[RED];
+#pragma unroll
+    for (int k = 0; k < RED; ++k) {
+      const int q = threadIdx.x + k * NW * 64;
+      const int row = base + (q >> 7), cl = q & 127, col = xs + cl;
+      eok[k] = eg && row >= zs && row < ze && cl >= 6 && cl < 6 + AM_OWN && col < d.NX &&
+               row >= d.nabc && row < d.nabc + d.nz && col >= d.nabc && col < d.nabc + d.nx;
+      eo[k] = eok[k] ? (long long)(row - d.nabc) * d.nx + (col - d.nabc) : 0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) ev[f][k] = eok[k] ? eg[f * inner + eo[k]] : 0.f;
+    }
+    if (active) {
+      // rows this block's stages touch: base .. base+9
+      const int rr = base + lane;
+      const bool rcv = rec && lane < 10 && rr >= 0 && rr < d.NZ && a.rrow[rr];
+      const bool plain = base >= zs && base + 9 < ze && base >= d.f + 1 && base + 9 <= d.NZ - 3 &&
+                         __ballot(rcv) == 0 && !(ssz >= base && ssz <= base + 9);
+      if (plain) {
+        iter(std::integral_constant<int, 0>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 1>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 2>{}, std::false_type{}, base);
+        iter(std::integral_constant<int, 3>{}, std::false_type{}, base);
+      } else {
+        iter(std::integral_constant<int, 0>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 1>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 2>{}, std::true_type{}, base);
+        iter(std::integral_constant<int, 3>{}, std::true_type{}, base);
+      }
+    }
+    coef_store(base + 12);
+    __syncthreads();
+    if constexpr (!RECOMP) {
+      // energy: each cell's squares summed over the group's shots in shot order
+#pragma unroll
+      for (int k = 0; k < RED; ++k) {
+        if (!eok[k]) continue;
+        const int q = threadIdx.x + k * NW * 64;
+        const int row = base + (q >> 7), cl = q & 127;
+        const int slot = row & (FM_ESLOTS - 1);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          const float* t = reinterpret_cast<const float*>(&sE[f][0][slot][0]) + cl;
+          float acc = 0.f;
+          for (int ww = 0; ww < ns; ++ww) acc = acc + t[ww * FM_ESLOTS * 128];
+          eg[f * inner + eo[k]] = ev[f][k] + acc;
+        }
+      }
+    }
+  }
+}
+
+template <int NW, bool RECOMP>
+__global__ __launch_bounds__(NW * 64) void fwd2m_kernel(FwdArgs a) {
+  __shared__ float2 sE[RECOMP ? 1 : 3][NW][FM_ESLOTS][64];  // energy squares: field, shot, row slot, lane
+  __shared__ float2 sC[5][AM_CROWS][64];                     // coefficient ring: plane, row slot, lane
+  const int groups = (a.S + a.spb - 1) / a.spb;
+  const int u = xcd_tile();
+  if (u >= a.ntiles * groups) return;  // uniform per workgroup
+  const int group = u % groups;
+  const int seg = (u / groups) % a.am_segs;
+  const int strip = u / groups / a.am_segs;
+  const int xs = strip * AM_OWN - 6;
+  if (xs >= 2 && xs + 127 <= a.d.NX - 3)
+    fwd2m_body<NW, false, RECOMP>(a, group, seg, strip, sE, sC);
+  else
+    fwd2m_body<NW, true, RECOMP>(a, group, seg, strip, sE, sC);
+}
+
 // lambda^{nt} = scatter of the records of the last step (zeroed buffer in)
 __global__ void adj_inject_kernel(Dims d, int S, int nt, int it, int R, const int* __restrict__ rcell,
                                   const int* __restrict__ rhead, const int* __restrict__ rnext,
@@ -2939,6 +3356,7 @@ struct Plan {
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
   bool amarch;      // z-marching two-step adjoint (adj2m_kernel) instead of tiles
+  bool fmarch;      // z-marching two-step forward / recompute (fwd2m_kernel) instead of tiles
   int am_strips, am_segs, am_seg_h;  // its column strips, z segments and rows per segment
   bool two;         // two time steps per forward / recompute launch
   int TZ2, TX2, tiles_x2;  // tiles of the two-step kernels (16 x 128 or 24 x 96)
@@ -3011,7 +3429,13 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   // 390 x 1740, 426 vs 622 us per launch)
   pl.amarch = pl.two && pass >= 1 &&
               (p->march == 3 || (p->march == 0 && pass == 1 && !d.fs && pl.S >= 8));
-  if (pl.amarch) {
+  if (p->fwd_march < 0 || p->fwd_march > 2)
+    return fail(GBW_ERR_INVALID, "fwd_march must be 0, 1 or 2");
+  // z-marching two-step forward and recompute: one wavefront per shot of a
+  // group of at most 8, on request; without a free surface, and not for the
+  // density gradient's recompute (its extra brackets keep the tiles)
+  pl.fmarch = pl.two && !d.fs && pass != 2 && p->fwd_march == 2;
+  if (pl.amarch || pl.fmarch) {
     pl.spb = std::min(pl.spb, 8);
     pl.groups = (pl.S + pl.spb - 1) / pl.spb;
   }
@@ -3213,6 +3637,19 @@ inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.t
 template <bool RECOMP>
 void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
+  if (steps == 2 && pl.fmarch && !a.gxbuf && !a.one) {
+    FwdArgs am = a;
+    am.ntiles = pl.am_strips * pl.am_segs;
+    am.am_segs = pl.am_segs;
+    am.seg_h = pl.am_seg_h;
+    const unsigned n = (unsigned)(((long long)am.ntiles * pl.groups + 7) / 8 * 8);
+    LaunchTimer timer(RECOMP ? "fwd2m_kernel<recompute>" : "fwd2m_kernel<forward>", st);
+    if (pl.spb <= 4)
+      fwd2m_kernel<4, RECOMP><<<n, 256, 0, st>>>(am);
+    else
+      fwd2m_kernel<8, RECOMP><<<n, 512, 0, st>>>(am);
+    return;
+  }
   LaunchTimer timer(steps == 2 ? (RECOMP ? "fwd2_step_kernel<recompute>" : "fwd2_step_kernel<forward>")
                                : (RECOMP ? "fwd_step_kernel<recompute>" : "fwd_step_kernel<forward>"),
                     st);

@@ -64,8 +64,10 @@ class LaunchChoice:
     one-step launches; 24 selects 24 x 96 tiles for two-step launches),
     ``shots_per_block`` and ``march`` (gradient: 1 tiles, 2 z-marching
     recompute strips, 3 z-marching two-step adjoint wavefronts),
-    ``steps_per_launch`` (1 or 2 time steps per forward / recompute launch);
-    zeros select the library's defaults.  ``bwd_tile_rows`` and
+    ``steps_per_launch`` (1 or 2 time steps per forward / recompute launch),
+    ``fwd_march`` (two-step forward and recompute launches: 1 tiles, 2
+    z-marching wavefronts); zeros select the library's defaults.
+    ``bwd_tile_rows`` and
     ``bwd_shots_per_block`` apply to the gradient call (0 = same as the
     forward's): the forward and adjoint kernels have different register
     budgets, so their best tiles differ.  The autotuner fills these."""
@@ -75,6 +77,7 @@ class LaunchChoice:
     bwd_tile_rows: int = 0
     bwd_shots_per_block: int = 0
     steps_per_launch: int = 0
+    fwd_march: int = 0
 
 
 _launch_override: Optional[LaunchChoice] = None
@@ -415,6 +418,7 @@ def acoustic_forward_kernel(
     prob.shots_per_block = choice.shots_per_block if choice else 0
     prob.march = choice.march if choice else 0
     prob.steps_per_launch = choice.steps_per_launch if choice else 0
+    prob.fwd_march = choice.fwd_march if choice else 0
     bwd_launch = (prob.tile_rows, prob.shots_per_block)
     if choice and (choice.bwd_tile_rows or choice.bwd_shots_per_block):
         bwd_launch = (choice.bwd_tile_rows, choice.bwd_shots_per_block)
