@@ -714,8 +714,12 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
       }
     }
   };
+  // the energy maps are sums of squares over shots and steps: one fused
+  // multiply-add per term (they agree with the reference's separately rounded
+  // products to rounding, as the two-step association already does)
   auto add_sq = [](float4& acc, const float4& v) {
-    acc = make_float4(acc.x + v.x * v.x, acc.y + v.y * v.y, acc.z + v.z * v.z, acc.w + v.w * v.w);
+    acc = make_float4(__builtin_fmaf(v.x, v.x, acc.x), __builtin_fmaf(v.y, v.y, acc.y),
+                      __builtin_fmaf(v.z, v.z, acc.z), __builtin_fmaf(v.w, v.w, acc.w));
   };
   // receivers of a tile slot: p, vx, vz of step `step`; for the free-surface
   // row f the slot also writes the vz samples of row f-1 (the mirror's source)
@@ -3441,12 +3445,21 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   }
   pl.am_strips = (d.NX + AM_OWN - 1) / AM_OWN;
   {
-    // as many z segments as keep all workgroups resident at once (one per
-    // CU: its registers and LDS allow no second), segments of at least 16 rows
+    // z segments of the marching kernels: one workgroup runs per CU (its
+    // registers and LDS allow no second), so a launch takes about
+    // ceil(workgroups / CUs) rounds of one segment each, and a segment costs
+    // its rows plus the 12-row pipeline start; take the count that minimises
+    // that (segments of at least 16 rows)
     const int per_seg = pl.am_strips * pl.groups;
-    int ns = std::max(1, AM_TARGET_WGS / per_seg);
-    ns = std::min(ns, std::max(1, d.NZ / 16));
-    pl.am_seg_h = ((d.NZ + ns - 1) / ns + 3) / 4 * 4;
+    int best_ns = 1;
+    long long best = -1;
+    for (int ns = 1; ns <= std::max(1, d.NZ / 16); ++ns) {
+      const int h = ((d.NZ + ns - 1) / ns + 3) / 4 * 4;
+      const long long units = (long long)per_seg * ((d.NZ + h - 1) / h);
+      const long long cost = (units + AM_TARGET_WGS - 1) / AM_TARGET_WGS * (h + 12);
+      if (best < 0 || cost < best) { best = cost; best_ns = ns; }
+    }
+    pl.am_seg_h = ((d.NZ + best_ns - 1) / best_ns + 3) / 4 * 4;
     pl.am_segs = (d.NZ + pl.am_seg_h - 1) / pl.am_seg_h;
   }
   {
