@@ -2201,7 +2201,7 @@ __global__ __launch_bounds__(NTH) void adj2_step_kernel(AdjArgs a) {
 
 // ---------------------------------------------------------------------------
 // Two adjoint steps (it, it-1) by z-marching wavefronts (launch choice
-// march = 3; no free surface, no density gradient).  The per-cell arithmetic
+// march = 3; no density gradient).  The per-cell arithmetic
 // is adj2_step_kernel's, in the same order, and the imaging sums are added in
 // the same order, so the results equal it bit for bit.
 //
@@ -2264,8 +2264,9 @@ __device__ __forceinline__ void row_st2(float* plane, int row, int PX, unsigned 
 }
 
 // CM: the window reaches a column outside the P, X or Z region (first and
-// last strips): per-cell column masks
-template <int NW, bool CM>
+// last strips): per-cell column masks.  FS: free surface (the mirror
+// transposes at rows f-1 .. f+1)
+template <int NW, bool CM, bool FS>
 __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg, int strip,
                                            float2 (*sT)[NW][AM_SLOTS][64],
                                            float2 (*sC)[AM_CROWS][64]) {
@@ -2349,9 +2350,12 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
   // lambda_p' = lambda_p + gather of q (rows r-2..r+1 of qz, columns of qx),
   // then r = a1 * lambda_p' on the P region (pr: cells in it) and lambda_p
   // after the step, omk1 * lambda_p' on the P region
+  // With a free surface, the transpose of p[f-1] = -p[f+1]: lambda_p' of row
+  // f-1 (kept in *sfm) is taken from row f+1 and row f-1 becomes 0 (rows above
+  // f carry no q, so row f-1's gather is the reference's closed form)
   auto lam_p = [&](auto G, int r, const float2* cr, const float2& lp, const float2& qx,
                    const float2& zm2, const float2& zm1, const float2& z00, const float2& zp1,
-                   float2& lpv, float2& rr, float2& lpo, bool* pr) {
+                   float2& lpv, float2& rr, float2& lpo, bool* pr, float2* sfm) {
     const float ql0 = lane_prev(qx.x), ql1 = lane_prev(qx.y), qr0 = lane_next(qx.x);
     {
       const float az = kC1 * (z00.x - zm1.x) + kC2 * (zp1.x - zm2.x);
@@ -2374,6 +2378,15 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
     }
     if constexpr (CM) {
       if (!lane_ok) lpv = f2zero();
+    }
+    if constexpr (FS && decltype(G)::value) {
+      if (r == d.f - 1) {
+        *sfm = lpv;
+        lpv = f2zero();
+      } else if (r == d.f + 1) {
+        lpv.x = lpv.x - sfm->x;
+        lpv.y = lpv.y - sfm->y;
+      }
     }
     const float2 c1 = cr[1 * AM_CROWS * 64], o1 = cr[0];
     rr = make_float2(c1.x * lpv.x, c1.y * lpv.y);
@@ -2412,6 +2425,9 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
     lz = make_float2(lz0, lz1);
   };
 
+  // free surface: lambda_p' of row f-1 in each step, lambda_vz of row f-1
+  // after the first step (folded into row f for the second)
+  float2 sfm0 = f2zero(), sfm1 = f2zero(), szf = f2zero();
   // four-row register windows, slot = row & 3
   float2 WLP[4], WLX[4], WLZ[4], WQX0[4], WQZ0[4], WR1[4], WLP1[4], WT1[4];
   float2 WLX1[4], WLZ1[4], WQX1[4], WQZ1[4], WR2[4];
@@ -2453,6 +2469,13 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       if constexpr (CM) {
         if (!lane_ok) lp = lx = lz = f2zero();
       }
+      if constexpr (FS && GG) {
+        if (j == d.f) {  // transpose of vz[f-1] = vz[f]: row f-1 joins row f
+          const float2 up = WLZ[(uu + 5) & 3];
+          lz = make_float2(lz.x + up.x, lz.y + up.y);
+          WLZ[(uu + 5) & 3] = f2zero();
+        }
+      }
       WLP[sj] = lp; WLX[sj] = lx; WLZ[sj] = lz;
       make_q(G, j, sCl + (j & (AM_CROWS - 1)) * 64, lx, lz, WQX0[sj], WQZ0[sj]);
     }
@@ -2463,7 +2486,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       float2 lpv, lpo;
       bool pr[2];
       lam_p(G, b, sCl + (b & (AM_CROWS - 1)) * 64, WLP[sb], WQX0[sb], WQZ0[(uu + 3) & 3],
-            WQZ0[(uu + 4) & 3], WQZ0[sb], WQZ0[(uu + 6) & 3], lpv, WR1[sb], lpo, pr);
+            WQZ0[(uu + 4) & 3], WQZ0[sb], WQZ0[(uu + 6) & 3], lpv, WR1[sb], lpo, pr, &sfm0);
       if (!GG || (b >= zs && b < ze)) {
         float2 t1 = f2zero();
         if (pr[0]) t1.x = lpv.x * d0.x;
@@ -2488,6 +2511,15 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       lam_v(G, c, cr, lx, lz, WR1[(uu + 2) & 3], WR1[sc], WR1[uu & 3], WR1[(uu + 1) & 3]);
       if constexpr (GG) {
         if (it >= 1 && a.R > 0 && row_in(c) && a.rrow[c]) inject(c, it - 1, nullptr, &lx, &lz);
+        // the second step's transpose of vz[f-1] = vz[f]
+        if constexpr (FS) {
+          if (c == d.f - 1) {
+            szf = lz;
+            lz = f2zero();
+          } else if (c == d.f) {
+            lz = make_float2(lz.x + szf.x, lz.y + szf.y);
+          }
+        }
       }
       WLX1[sc] = lx; WLZ1[sc] = lz;
       make_q(G, c, cr, lx, lz, WQX1[sc], WQZ1[sc]);
@@ -2499,7 +2531,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       float2 lpv, lpo, r2;
       bool pr[2];
       lam_p(G, dr, sCl + (dr & (AM_CROWS - 1)) * 64, WLP1[sd], WQX1[sd], WQZ1[uu & 3],
-            WQZ1[(uu + 1) & 3], WQZ1[sd], WQZ1[(uu + 3) & 3], lpv, r2, lpo, pr);
+            WQZ1[(uu + 1) & 3], WQZ1[sd], WQZ1[(uu + 3) & 3], lpv, r2, lpo, pr, &sfm1);
       WR2[sd] = r2;
       if (!GG || (dr >= zs && dr < ze)) {
         if constexpr (GG) {
@@ -2633,7 +2665,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
   }
 }
 
-template <int NW>
+template <int NW, bool FS>
 __global__ __launch_bounds__(NW * 64) void adj2m_kernel(AdjArgs a) {
   __shared__ float2 sT[2][NW][AM_SLOTS][64];  // imaging terms per step, shot, row slot, lane
   __shared__ float2 sC[5][AM_CROWS][64];      // coefficient ring: plane, row slot, lane
@@ -2645,9 +2677,9 @@ __global__ __launch_bounds__(NW * 64) void adj2m_kernel(AdjArgs a) {
   const int strip = u / groups / a.am_segs;
   const int xs = strip * AM_OWN - 6;
   if (xs >= 2 && xs + 127 <= a.d.NX - 3)
-    adj2m_body<NW, false>(a, group, seg, strip, sT, sC);
+    adj2m_body<NW, false, FS>(a, group, seg, strip, sT, sC);
   else
-    adj2m_body<NW, true>(a, group, seg, strip, sT, sC);
+    adj2m_body<NW, true, FS>(a, group, seg, strip, sT, sC);
 }
 
 // ---------------------------------------------------------------------------
@@ -3428,11 +3460,11 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   // it measured slower than tiles on 128-column strips
   pl.march = p->march == 2;
   // z-marching two-step adjoint: one wavefront per shot of a group of at most 8;
-  // chosen by default for gradients of 8 or more shots without a free surface
-  // (it fills its workgroups and measured faster than tiles: 64 shots of
-  // 390 x 1740, 426 vs 622 us per launch)
+  // chosen by default for gradients of 8 or more shots without the density
+  // gradient (it fills its workgroups and measured faster than tiles: 64 shots
+  // of 390 x 1740, 426 vs 622 us per launch)
   pl.amarch = pl.two && pass >= 1 &&
-              (p->march == 3 || (p->march == 0 && pass == 1 && !d.fs && pl.S >= 8));
+              (p->march == 3 || (p->march == 0 && pass == 1 && pl.S >= 8));
   if (p->fwd_march < 0 || p->fwd_march > 2)
     return fail(GBW_ERR_INVALID, "fwd_march must be 0, 1 or 2");
   // z-marching two-step forward and recompute: one wavefront per shot of a
@@ -3682,19 +3714,26 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
 }
 
 // two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles, or z-marching
-// wavefronts (no free surface, no density gradient)
+// wavefronts (no density gradient)
 void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
-  if (pl.amarch && !a.gA2 && !pl.d.fs) {
+  if (pl.amarch && !a.gA2) {
     AdjArgs am = a;
     am.ntiles = pl.am_strips * pl.am_segs;
     am.am_segs = pl.am_segs;
     am.am_seg_h = pl.am_seg_h;
     const unsigned n = (unsigned)(((long long)am.ntiles * pl.groups + 7) / 8 * 8);
     LaunchTimer timer("adj2m_kernel", st);
-    if (pl.spb <= 4)
-      adj2m_kernel<4><<<n, 256, 0, st>>>(am);
-    else
-      adj2m_kernel<8><<<n, 512, 0, st>>>(am);
+    if (pl.d.fs) {
+      if (pl.spb <= 4)
+        adj2m_kernel<4, true><<<n, 256, 0, st>>>(am);
+      else
+        adj2m_kernel<8, true><<<n, 512, 0, st>>>(am);
+    } else {
+      if (pl.spb <= 4)
+        adj2m_kernel<4, false><<<n, 256, 0, st>>>(am);
+      else
+        adj2m_kernel<8, false><<<n, 512, 0, st>>>(am);
+    }
     return;
   }
   AdjArgs a2 = a;

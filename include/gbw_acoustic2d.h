@@ -70,9 +70,8 @@ typedef struct gbw_acoustic2d_problem {
   int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
   int32_t march;             /* launch choice for the gradient: 0 auto, 1 tiles,        */
                              /* 2 z-marching recompute strips (one step per launch),    */
-                             /* 3 z-marching two-step adjoint (with a free surface or a */
-                             /* rho gradient the tiled adjoint runs); auto takes 3 for  */
-                             /* 8 or more shots without a free surface                  */
+                             /* 3 z-marching two-step adjoint (with a rho gradient the  */
+                             /* tiled adjoint runs); auto takes 3 for 8 or more shots   */
   int32_t steps_per_launch;  /* launch choice: time steps per forward/recompute launch */
                              /* (1 or 2; 0 = default 2; two-step launches tile 16 rows) */
   int32_t fwd_march;         /* launch choice for two-step forward and recompute        */

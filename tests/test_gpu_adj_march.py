@@ -5,8 +5,9 @@ z segments (the last of each partial), odd step counts and checkpoint
 intervals (so one-step launches mix in), shot groups of 1-8 shots, adjoint
 sources on p, vx and vz, and sources and receivers near the strip and segment
 boundaries; then at the full 64-shot 350 x 1700 size for a shortened time
-axis.  With a free surface or a density gradient the library keeps the tiled
-kernel, which the last case checks."""
+axis; with a free surface (segment boundaries at the mirrored row, the
+surface and the row below it; receivers and sources on those rows).  With a
+density gradient the library keeps the tiled kernel, which a case checks."""
 
 import numpy as np
 import pytest
@@ -105,10 +106,72 @@ def test_march_adjoint_full_size_64_shots():
     _assert_same(tiled, march, "march vs tiled")
 
 
-def test_march_choice_keeps_tiles_for_free_surface_and_rho():
+def test_march_choice_keeps_tiles_for_rho_and_runs_with_free_surface():
     print("[case] march choice with free surface / density gradient", flush=True)
     p = _problem(60, 260, 90, 3, seed=4)
-    for fs, rho in ((True, False), (False, True)):
+    for fs, rho in ((True, False), (False, True), (True, True)):
         tiled = _gradient(p, K.LaunchChoice(0, 2, 1, 0, 2, 2), fs=fs, rho_grad=rho)
         march = _gradient(p, K.LaunchChoice(0, 2, 3, 0, 2, 2), fs=fs, rho_grad=rho)
         _assert_same(tiled, march, f"free_surface={fs} rho={rho}")
+
+
+def _gradient_nabc(p, launch, nabc, segments=3, seed=1):
+    """_gradient with a free surface and `nabc` absorbing layers."""
+    T = lambda a, **kw: torch.as_tensor(np.asarray(a), device=DEV, **kw)  # noqa: E731
+    damp = wave.bc_pml(p["nx"], p["nz"], 5.0, 5.0, nabc, float(p["vp"].max()), False)
+    vp = T(p["vp"], dtype=torch.float32).requires_grad_(True)
+    w = T(p["w"], dtype=torch.float32).requires_grad_(True)
+    K.set_launch_choice(launch)
+    try:
+        out = wave.acoustic_forward_kernel(
+            nx=p["nx"], nz=p["nz"], dx=5.0, dz=5.0, nt=p["nt"], dt=5e-4, nabc=nabc,
+            free_surface=True, src_x=T(p["sx"]), src_z=T(p["sz"]), src_n=len(p["sx"]), src_v=w,
+            rcv_x=T(p["rx"]), rcv_z=T(p["rz"]), rcv_n=len(p["rx"]),
+            damp=T(damp.astype(np.float32)), vp=vp,
+            rho=T(p["rho"], dtype=torch.float32), checkpoint_segments=segments)
+        rng = np.random.default_rng(seed)
+        loss = sum((T(rng.standard_normal(out[k].shape).astype(np.float32)) * out[k]).sum()
+                   for k in ("p", "vx", "vz"))
+        loss.backward()
+    finally:
+        K.set_launch_choice(None)
+    torch.cuda.synchronize()
+    res = {k: out[k].detach().cpu().numpy() for k in ("p", "vx", "vz")}
+    res["grad_vp"] = vp.grad.cpu().numpy()
+    res["grad_w"] = w.grad.cpu().numpy()
+    return res
+
+
+@pytest.mark.parametrize("nabc", [8, 19, 20, 21])
+def test_march_adjoint_free_surface(nabc):
+    """With a free surface (the mirrored row f-1 = nabc-1): the z-marching
+    adjoint equals the tiled one bit for bit.  Its z segments are 20 rows here,
+    so 19, 20 and 21 absorbing layers put a segment boundary at f+1, f and
+    f-1; receivers on the mirrored row, the surface and the rows below it,
+    sources on rows f-1 .. f+2 and deeper; an odd step count."""
+    print(f"[case] march adjoint, free surface, nabc {nabc}", flush=True)
+    nz, nx, nt, S = 76, 300, 101, 6
+    p = _problem(nz, nx, nt, S, seed=40 + nabc, src_z=[-1, 0, 1, 2, 9, 33])
+    p["rx"] = np.concatenate([np.arange(0, nx, 3), np.arange(1, nx, 7), np.arange(2, nx, 5),
+                              np.arange(0, nx, 11)])
+    p["rz"] = np.concatenate([np.full(len(np.arange(0, nx, 3)), -1),
+                              np.zeros(len(np.arange(1, nx, 7)), np.int64),
+                              np.ones(len(np.arange(2, nx, 5)), np.int64),
+                              np.full(len(np.arange(0, nx, 11)), 4)])
+    tiled = _gradient_nabc(p, K.LaunchChoice(0, 6, 1, 0, 6, 2), nabc)
+    march = _gradient_nabc(p, K.LaunchChoice(0, 6, 3, 0, 6, 2), nabc)
+    assert float(np.abs(tiled["grad_vp"]).max()) > 0
+    _assert_same(tiled, march, f"free surface, nabc {nabc}")
+
+
+def test_march_adjoint_free_surface_full_size_64_shots():
+    """The benchmark's grid and shots with a free surface (receivers on the
+    surface row), 200 steps: identical gradients."""
+    print("[case] march adjoint, free surface, 64 shots 350x1700, 200 steps", flush=True)
+    p = _problem(350, 1700, 200, 64, seed=5)
+    p["rx"] = np.arange(1700)
+    p["rz"] = np.zeros(1700, np.int64)
+    p["sz"] = np.ones(64, np.int64)
+    tiled = _gradient(p, K.LaunchChoice(16, 32, 1, 16, 8, 2), fs=True, segments=4)
+    march = _gradient(p, K.LaunchChoice(16, 32, 3, 16, 8, 2), fs=True, segments=4)
+    _assert_same(tiled, march, "free surface, march vs tiled")
