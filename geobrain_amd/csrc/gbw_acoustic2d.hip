@@ -2243,13 +2243,6 @@ constexpr int AM_TARGET_WGS = GBW_AM_TARGET_WGS;  // workgroups to aim for (MI35
 constexpr int AM_PF = GBW_AM_PF;  // iterations a wavefront loads its shot's rows ahead (1, 2, 4)
 static_assert(AM_PF == 1 || AM_PF == 2 || AM_PF == 4, "the 4-row unroll needs AM_PF | 4");
 
-// lane l-1's value (lane 0: 0) and lane l+1's value (lane 63: 0)
-__device__ __forceinline__ float lane_prev(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-__device__ __forceinline__ float lane_next(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
 __device__ __forceinline__ float2 f2zero() { return make_float2(0.f, 0.f); }
 // float2 at byte offset `lane_off` of row `row` of a plane (uniform row base,
 // 32-bit lane offset)

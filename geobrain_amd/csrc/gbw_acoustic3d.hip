@@ -1138,13 +1138,6 @@ struct Fused3Args {
   long long units;
 };
 
-// lane l-1's value (lane 0: 0) and lane l+1's value (lane 63: 0)
-__device__ __forceinline__ float f3_prev(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-__device__ __forceinline__ float f3_next(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
 // per-lane column flags of the fused step (bits of one word)
 enum : unsigned {
   F3_OWN = 1u, F3_CP = 2u, F3_CVX = 4u, F3_CVY = 8u, F3_CE = 16u, F3_XIN = 32u,
@@ -1328,7 +1321,7 @@ __global__ __launch_bounds__(F3_NW * 64) void fwd3f_kernel(Fused3Args a) {
       // two (plane, shot) pairs ahead
       q1 = issue((k + 2) % SPB, z + (k + 2) / SPB);
       const bool act = k < ns;  // uniform
-      const float l1 = f3_prev(v.vx), l2 = f3_prev(l1), r1 = f3_next(v.vx), hn = f3_next(v.vxh);
+      const float l1 = lane_prev(v.vx), l2 = lane_prev(l1), r1 = lane_next(v.vx), hn = lane_next(v.vxh);
       const float vm1 = (cf & F3_L0) ? hn : l1;
       const float vm2 = (cf & F3_L01) ? v.vxh : l2;
       const float vp1 = (cf & F3_L63) ? v.vxh : r1;
@@ -1380,7 +1373,7 @@ __global__ __launch_bounds__(F3_NW * 64) void fwd3f_kernel(Fused3Args a) {
           if (k >= ns) break;  // uniform
           const unsigned so = (unsigned)k * PB;
           const float pc = pq2[k];
-          const float pm1 = f3_prev(pc), pp1 = f3_next(pc), pp2 = f3_next(pp1);
+          const float pm1 = lane_prev(pc), pp1 = lane_next(pc), pp2 = lane_next(pp1);
           const float pym1 = lp[par][k][w - 1][lane], pyp1 = lp[par][k][w + 1][lane],
                       pyp2 = lp[par][k][w + 2][lane];
           const float ux = omk2 * hvx[k];

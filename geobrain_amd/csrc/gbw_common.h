@@ -1,7 +1,7 @@
 // Utilities shared by the library's source files: the per-thread error text
 // behind gbw_last_error(), HIP status checks, the optional per-kernel event
-// timing behind gbw_set_kernel_timing() / gbw_kernel_times(), and float4
-// accesses at 32-bit byte offsets.
+// timing behind gbw_set_kernel_timing() / gbw_kernel_times(), wavefront lane
+// shifts, and float4 accesses at 32-bit byte offsets.
 #ifndef GBW_COMMON_H
 #define GBW_COMMON_H
 
@@ -109,6 +109,15 @@ inline int flush_timing(hipStream_t st) {
   }
   t_pending.clear();
   return 0;
+}
+
+// lane l-1's value (lane 0: 0) and lane l+1's value (lane 63: 0) of a
+// wavefront: DPP wavefront shifts by one lane (bound_ctrl writes 0 at the end)
+__device__ __forceinline__ float lane_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
 // element i of a float4 array at a 32-bit byte offset (i < 2^28): the access
