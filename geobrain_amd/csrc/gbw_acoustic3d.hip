@@ -1664,8 +1664,8 @@ int fused_segments(const Plan3& pl, int z_chunk, int cus) {
   return best;
 }
 
-// one fused step: state A -> B
-int fused_step3(const Plan3& pl, Fused3Args& f, int z_chunk, hipStream_t st) {
+// the fused step's decomposition (once per call: the current device's CU count)
+void fused_setup3(const Plan3& pl, Fused3Args& f, int z_chunk) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     int v = 0;
@@ -1676,6 +1676,10 @@ int fused_step3(const Plan3& pl, Fused3Args& f, int z_chunk, hipStream_t st) {
   f.seg_h = (pl.d.NZ + zs - 1) / zs;
   f.strips = pl.f_strips; f.ytiles = pl.f_ytiles; f.groups = pl.groups; f.spb = pl.spb;
   f.units = (long long)pl.f_strips * pl.f_ytiles * pl.groups * ((pl.d.NZ + f.seg_h - 1) / f.seg_h);
+}
+
+// one fused step: state A -> B
+int fused_step3(const Plan3& pl, const Fused3Args& f, hipStream_t st) {
   const unsigned blocks = (unsigned)((f.units + 7) / 8 * 8);
   LaunchTimer timer("fwd3f_kernel", st);
   // the instance holds the next power of two of the group's shots
@@ -1760,6 +1764,7 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
   fa.d = d; fa.S = pl.S; fa.nt = pl.nt; fa.dt = prob->dt; fa.fstride = fstride; fa.coef = coef;
   fa.src_cell = a.src_cell; fa.wav = a.wav; fa.wav_stride = a.wav_stride; fa.energy = a.energy;
   if ((pl.march || pl.fused) && (rc = fill3(nxt, 4 * fstride, 0.f, st))) return rc;
+  if (pl.fused) fused_setup3(pl, fa, prob->z_chunk);
   for (int it = 0; it < pl.nt; ++it) {
     if (pl.k > 0 && it % pl.k == 0)
       HIP_TRY(hipMemcpyAsync(static_cast<char*>(out->checkpoints) +
@@ -1767,7 +1772,7 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
                              cur, 4 * fstride * sizeof(float), hipMemcpyDeviceToDevice, st));
     if (pl.fused) {
       fa.it = it; fa.A = cur; fa.B = nxt;
-      if ((rc = fused_step3(pl, fa, prob->z_chunk, st))) return rc;
+      if ((rc = fused_step3(pl, fa, st))) return rc;
       std::swap(cur, nxt);
     } else if (pl.march) {
       m.it = it; m.A = cur; m.B = nxt;
