@@ -519,6 +519,7 @@ constexpr int LW2 = GW + 6;  // LDS row: staged groups plus a zero guard each si
 constexpr unsigned F_OK = 1u, F_PV = 2u, F_P1 = 4u, F_V1 = 8u, F_P2 = 16u, F_T = 32u, F_RCV = 64u;
 constexpr unsigned F_PFULL = 128u, F_VFULL = 1u << 28;  // all 4 cells in the P / X and Z regions
 constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surface exception
+constexpr unsigned F_RMORE = 1u << 30;  // forward: a cell of the slot has several receivers
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
 // one workgroup's tile `tile` and shot group `group` of a problem
@@ -557,6 +558,8 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   // --- per-slot constants (registers across the shot loop) ---
   unsigned idx[NS], fl[NS];
   float4 c_omk1[NS], c_a1[NS], c_omk2[NS], c_omk3[NS], c_a2[NS];
+  // forward: first receiver of each cell of a receiver slot (-1: none), by slot
+  __shared__ int4 sRH[RECOMP ? 1 : SR * W2];
 #pragma unroll
   for (int n = 0; n < NS; ++n) {
     const int item = tid + n * NTH;
@@ -592,6 +595,17 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
       f |= F_RCV;
     if (((f >> M_PREG) & 15u) == 15u) f |= F_PFULL;
     if (((f >> M_XREG) & 15u) == 15u && ((f >> M_ZREG) & 15u) == 15u) f |= F_VFULL;
+    if constexpr (!RECOMP) {
+      // the slot's receivers: the first of each cell to LDS, and whether a
+      // cell has more
+      if (f & F_RCV) {
+        const int4 rh = reinterpret_cast<const int4*>(a.rhead)[idx[n]];
+        sRH[item] = rh;
+        if ((rh.x >= 0 && a.rnext[rh.x] >= 0) || (rh.y >= 0 && a.rnext[rh.y] >= 0) ||
+            (rh.z >= 0 && a.rnext[rh.z] >= 0) || (rh.w >= 0 && a.rnext[rh.w] >= 0))
+          f |= F_RMORE;
+      }
+    }
     fl[n] = f;
     c_omk1[n] = p1 ? omk1[idx[n]] : f4zero();
     c_a1[n] = p1 ? a1[idx[n]] : f4zero();
@@ -722,19 +736,28 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
                       __builtin_fmaf(v.z, v.z, acc.z), __builtin_fmaf(v.w, v.w, acc.w));
   };
   // receivers of a tile slot: p, vx, vz of step `step`; for the free-surface
-  // row f the slot also writes the vz samples of row f-1 (the mirror's source)
+  // row f the slot also writes the vz samples of row f-1 (the mirror's source).
+  // The first receiver of each cell comes from registers, so a shot's samples
+  // are stores only (no dependent loads in front of the next barrier); cells
+  // with more receivers walk their list.
   auto record = [&](int n, int s, int step, int gz, const float4& pv, const float4& vx, const float4& vz) {
     const unsigned ti = idx[n];
-    const int4 rh = reinterpret_cast<const int4*>(a.rhead)[ti];
+    const int4 rh = sRH[RECOMP ? 0 : tid + n * NTH];
     const bool mirror_row = d.fs && gz == d.f - 1;
     const long long ob = ((long long)s * a.nt + step) * a.R;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      for (int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w)); hh >= 0;
-           hh = a.rnext[hh]) {
-        a.rec_p[ob + hh] = f4get(pv, e);
-        a.rec_vx[ob + hh] = f4get(vx, e);
-        if (!mirror_row) a.rec_vz[ob + hh] = f4get(vz, e);
+      int hh = e == 0 ? rh.x : (e == 1 ? rh.y : (e == 2 ? rh.z : rh.w));
+      if (hh < 0) continue;
+      a.rec_p[ob + hh] = f4get(pv, e);
+      a.rec_vx[ob + hh] = f4get(vx, e);
+      if (!mirror_row) a.rec_vz[ob + hh] = f4get(vz, e);
+      if (fl[n] & F_RMORE) {
+        for (hh = a.rnext[hh]; hh >= 0; hh = a.rnext[hh]) {
+          a.rec_p[ob + hh] = f4get(pv, e);
+          a.rec_vx[ob + hh] = f4get(vx, e);
+          if (!mirror_row) a.rec_vz[ob + hh] = f4get(vz, e);
+        }
       }
     }
     if (d.fs && gz == d.f) {
