@@ -160,8 +160,9 @@ def _tune_trials(args, trial, vp, cands, log, repeats):
 
     # the recompute and adjoint inside the gradient call follow the forward's
     # steps_per_launch and take their own tiles; the z-marching recompute
-    # exists for one-step launches only, the z-marching adjoint (march 3, at
-    # most 8 shots per workgroup) for two-step launches
+    # exists for one-step launches only, the z-marching adjoint (march 3) for
+    # two-step launches: it steps at most 8 shots per workgroup, and the tiled
+    # recompute beside it keeps the candidate's shots_per_block
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
     spl = best_f[2]
     bwd_cands = cands if spl == 1 else [(rows, spb) for rows in (16, 24)
@@ -171,8 +172,6 @@ def _tune_trials(args, trial, vp, cands, log, repeats):
         for tr, spb in bwd_cands:
             if march == 2 and tr != cands[0][0]:
                 continue  # the marching recompute ignores tile_rows
-            if march == 3 and spb > 8:
-                continue  # the marching adjoint takes at most 8 shots per workgroup
             K.set_launch_choice(K.LaunchChoice(best_f[0], best_f[1], march, tr, spb, best_f[2]))
             t = _timed(bwd, repeats)
             log(f"gradient tile_rows={tr} shots_per_block={spb} march={march}: {t:.2f} ms")

@@ -3405,6 +3405,7 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Plan {
   Dims d;
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
+  int rspb, rgroups;  // shot groups of the gradient's tiled recompute (the adjoint's may be smaller)
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
   bool amarch;      // z-marching two-step adjoint (adj2m_kernel) instead of tiles
@@ -3487,10 +3488,15 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   // group of at most 8, on request; without a free surface, and not for the
   // density gradient's recompute (its extra brackets keep the tiles)
   pl.fmarch = pl.two && !d.fs && pass != 2 && p->fwd_march == 2;
+  // the tiled recompute keeps the requested shot groups when only the adjoint
+  // marches (its workgroups take at most 8 shots)
+  pl.rspb = pl.spb;
   if (pl.amarch || pl.fmarch) {
     pl.spb = std::min(pl.spb, 8);
     pl.groups = (pl.S + pl.spb - 1) / pl.spb;
   }
+  if (pl.fmarch) pl.rspb = pl.spb;
+  pl.rgroups = (pl.S + pl.rspb - 1) / pl.rspb;
   pl.am_strips = (d.NX + AM_OWN - 1) / AM_OWN;
   {
     // z segments of the marching kernels: one workgroup runs per CU (its
@@ -3696,14 +3702,18 @@ inline unsigned tile_grid(const Plan& pl) { return (unsigned)((pl.tiles_x * pl.t
 // (the two-step kernel always tiles 16 rows, whatever tile_rows the one-step
 // and adjoint kernels use)
 template <bool RECOMP>
-void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
-  dim3 grid(tile_grid(pl), pl.groups);
+void launch_steps(const Plan& pl, const FwdArgs& a_in, int steps, hipStream_t st) {
+  // shot groups: the recompute's own (rspb), else the pass's
+  FwdArgs a = a_in;
+  a.spb = RECOMP ? pl.rspb : pl.spb;
+  const int groups = RECOMP ? pl.rgroups : pl.groups;
+  dim3 grid(tile_grid(pl), groups);
   if (steps == 2 && pl.fmarch && !a.gxbuf && !a.one) {
     FwdArgs am = a;
     am.ntiles = pl.am_strips * pl.am_segs;
     am.am_segs = pl.am_segs;
     am.seg_h = pl.am_seg_h;
-    const unsigned n = (unsigned)(((long long)am.ntiles * pl.groups + 7) / 8 * 8);
+    const unsigned n = (unsigned)(((long long)am.ntiles * groups + 7) / 8 * 8);
     LaunchTimer timer(RECOMP ? "fwd2m_kernel<recompute>" : "fwd2m_kernel<forward>", st);
     if (pl.spb <= 4)
       fwd2m_kernel<4, RECOMP><<<n, 256, 0, st>>>(am);
@@ -3718,7 +3728,7 @@ void launch_steps(const Plan& pl, const FwdArgs& a, int steps, hipStream_t st) {
     FwdArgs a2 = a;
     a2.tiles_x = pl.tiles_x2;
     a2.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
-    const dim3 g2((unsigned)((a2.ntiles + 7) / 8 * 8), pl.groups);
+    const dim3 g2((unsigned)((a2.ntiles + 7) / 8 * 8), groups);
     if (pl.TZ2 == 24)
       fwd2_step_kernel<24, 96, FWD2_THREADS, RECOMP><<<g2, FWD2_THREADS, 0, st>>>(a2);
     else

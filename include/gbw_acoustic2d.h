@@ -67,7 +67,9 @@ typedef struct gbw_acoustic2d_problem {
   int32_t tile_rows;         /* launch choice: rows per tile: 16 or 32 for one-step     */
                              /* launches; 24 selects 24 x 96 tiles for two-step ones    */
                              /* (else 16 x 128); 0 = default                            */
-  int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default) */
+  int32_t shots_per_block;   /* launch choice: shots one workgroup steps (0 = default); */
+                             /* the z-marching kernels take at most 8 of them, the      */
+                             /* gradient's tiled recompute beside them all              */
   int32_t march;             /* launch choice for the gradient: 0 auto, 1 tiles,        */
                              /* 2 z-marching recompute strips (one step per launch),    */
                              /* 3 z-marching two-step adjoint (with a rho gradient the  */
