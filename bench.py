@@ -782,8 +782,10 @@ def main():
                      "adj_step_kernel": 28.0, "recomp_march_kernel": 28.0}
     kernels = {}
     for name, (launches, ms) in ktimes.items():
-        # adj2m_kernel / fwd2m_kernel: the z-marching two-step adjoint / forward
-        base = name.replace("fwd2_", "fwd_").replace("adj2_", "adj_").replace(
+        # adj2m_kernel / fwd2m_kernel: the z-marching two-step adjoint / forward;
+        # fwd2_mixed_kernel: the two-phase two-step forward / recompute
+        base = name.replace("fwd2_mixed_kernel", "fwd_step_kernel").replace(
+            "fwd2_", "fwd_").replace("adj2_", "adj_").replace(
             "adj2m_kernel", "adj_step_kernel").replace("fwd2m_kernel", "fwd_step_kernel")
         steps_per = 2 if ("fwd2" in name or "adj2" in name) else 1
         b = per_cell_step[base] * steps_per * S * NZ * NX

@@ -69,7 +69,7 @@ def forward_candidates(n_shots: int) -> List[Tuple[int, int, int]]:
     forward call (two-step launches always tile 16 rows)."""
     cands = [(tr, spb, 1) for tr, spb in default_candidates(n_shots)]
     for rows in (16, 24):
-        cands += [(rows, spb, 2) for spb in (0, 4, 8, 16, 32) if spb < n_shots or spb == 0]
+        cands += [(rows, spb, 2) for spb in (0, 4, 8, 16, 32, 64) if spb <= n_shots]
     return cands
 
 
@@ -166,8 +166,8 @@ def _tune_trials(args, trial, vp, cands, log, repeats):
     best_b, best_bt = (best_f[0], best_f[1], 1), float("inf")
     spl = best_f[2]
     bwd_cands = cands if spl == 1 else [(rows, spb) for rows in (16, 24)
-                                        for spb in (0, 4, 8, 16, 32)
-                                        if spb == 0 or spb < args["src_n"]]
+                                        for spb in (0, 4, 8, 16, 32, 64)
+                                        if spb <= args["src_n"]]
     for march in ((1, 2) if spl == 1 else (1, 3)):
         for tr, spb in bwd_cands:
             if march == 2 and tr != cands[0][0]:

@@ -142,6 +142,7 @@ struct FwdArgs {
   int S, spb, tiles_x, ntiles;   // shots, shots per workgroup, tiles along x, workgroup units
   int seg_h;                     // z-marching kernels: rows per segment
   int am_segs;                   // z-marching two-step forward: segments per column strip
+  int mix_ta, mix_spb_b;         // two-phase two-step launches: phase-A tiles, phase-B group size
   const float* P0; const float* VX0; const float* VZ0;  // input state fields (shot-major)
   float* P1; float* VX1; float* VZ1;                    // output state fields
   const float* coef;
@@ -520,13 +521,19 @@ constexpr unsigned F_OK = 1u, F_PV = 2u, F_P1 = 4u, F_V1 = 8u, F_P2 = 16u, F_T =
 constexpr unsigned F_PFULL = 128u, F_VFULL = 1u << 28;  // all 4 cells in the P / X and Z regions
 constexpr unsigned F_AINT = 1u << 29;  // adjoint: no region, grid or free-surface exception
 constexpr unsigned F_RMORE = 1u << 30;  // forward: a cell of the slot has several receivers
+// two-phase two-step launches: shots per phase-B group (groups of 4, 8, 16
+// and 32 measured 339, 315, 323 and 332 us per forward launch on 64 shots of
+// 390 x 1740 with phase-A groups of 32 or 64)
+constexpr int MIX_SPB_B = 8;
 constexpr int M_PREG = 8, M_XREG = 12, M_ZREG = 16, M_PIN = 20, M_ENRG = 24;
 
-// one workgroup's tile `tile` and shot group `group` of a problem
+// one workgroup's tile `tile` and shots [s_begin, s_end) of a problem; the
+// forward's energy sums go to partial map `epart` of a.energy
 // PHASED: the batched gradient's launches, in which a problem may start at the
 // second step (a.skip1); the other kernels compile without that test
 template <int TZ, int TXT, int NTH, bool RECOMP, bool PHASED = false>
-__device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group) {
+__device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int s_begin, int s_end,
+                                          int epart) {
   // this kernel's tile width (columns) and the staging geometry that follows
   constexpr int TX = TXT, GW = TXT / 4, W2 = GW + 4, LW2 = GW + 6;
   constexpr int SR = TZ + 12;                    // staged rows (-6 .. TZ+5)
@@ -540,8 +547,6 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   const int tid = threadIdx.x;
   const int x0 = (tile % a.tiles_x) * TX;
   const int z0 = (tile / a.tiles_x) * TZ;
-  const int s_begin = group * a.spb;
-  const int s_end = min(a.S, s_begin + a.spb);
   if (s_begin >= s_end) return;  // uniform per workgroup
   const float4* __restrict__ omk1 = reinterpret_cast<const float4*>(a.coef);
   const float4* __restrict__ a1 = reinterpret_cast<const float4*>(a.coef + d.plane);
@@ -969,7 +974,7 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   if constexpr (!RECOMP) {
     if (a.energy) {
       const long long inner = (long long)d.nz * d.nx;
-      float* eg = a.energy + (long long)blockIdx.y * 3 * inner;
+      float* eg = a.energy + (long long)epart * 3 * inner;
 #pragma unroll
       for (int n = 0; n < NS; ++n) {
         if (fl[n] & F_T) {
@@ -991,11 +996,57 @@ __device__ __forceinline__ void fwd2_tile(const FwdArgs& a, int tile, int group)
   }
 }
 
+// shots of shot group `group` (a.spb per group)
+template <int TZ, int TXT, int NTH, bool RECOMP, bool PHASED = false>
+__device__ __forceinline__ void fwd2_group(const FwdArgs& a, int tile, int group) {
+  fwd2_tile<TZ, TXT, NTH, RECOMP, PHASED>(a, tile, group * a.spb, min(a.S, (group + 1) * a.spb),
+                                          group);
+}
+
 template <int TZ, int TXT, int NTH, bool RECOMP>
 __global__ __launch_bounds__(NTH) void fwd2_step_kernel(FwdArgs a) {
   const int tile = xcd_tile();
   if (tile >= a.ntiles) return;
-  fwd2_tile<TZ, TXT, NTH, RECOMP>(a, tile, blockIdx.y);
+  fwd2_group<TZ, TXT, NTH, RECOMP>(a, tile, blockIdx.y);
+}
+
+// Two-step forward / recompute in two phases of work units, so the launch
+// ends in full rounds of one workgroup per CU without giving up large shot
+// groups.  Phase A: the first mix_ta tiles (a multiple of the CU count) in
+// groups of a.spb shots, which take whole rounds; phase B: the other tiles in
+// groups of mix_spb_b shots, small units that fill the last rounds evenly.
+// Workgroups go to the 8 XCDs round-robin (blockIdx.x & 7); each XCD runs an
+// equal share of both phases, phase A first, and within a phase its units in
+// (group, tile) order over a contiguous range of tiles, so the workgroups
+// resident on an XCD step neighbouring tiles through the same shots at about
+// the same time and share the tiles' halos in that XCD's L2.  Energy: unit
+// (group g, tile t) adds to partial map g (each map cell has one writer).
+inline int mix_spb_b(int spb) { return std::min(spb, MIX_SPB_B); }
+
+template <int TZ, int TXT, int NTH, bool RECOMP>
+__global__ __launch_bounds__(NTH) void fwd2_mixed_kernel(FwdArgs a) {
+  const int k = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int ta8 = a.mix_ta >> 3;                   // phase-A tiles per XCD
+  const int ga = (a.S + a.spb - 1) / a.spb;        // phase-A groups
+  const int na8 = ta8 * ga;                        // phase-A units per XCD
+  int tile, s0, s1, g;
+  if (j < na8) {
+    g = j / ta8;
+    tile = k * ta8 + j % ta8;
+    s0 = g * a.spb;
+    s1 = min(a.S, s0 + a.spb);
+  } else {
+    const int tb = a.ntiles - a.mix_ta;            // phase-B tiles
+    const int gb = (a.S + a.mix_spb_b - 1) / a.mix_spb_b;
+    const int nb8 = (tb * gb + 7) / 8;             // phase-B units per XCD
+    const int u = k * nb8 + (j - na8);
+    if (u >= tb * gb) return;  // uniform per workgroup
+    g = u / tb;
+    tile = a.mix_ta + u % tb;
+    s0 = g * a.mix_spb_b;
+    s1 = min(a.S, s0 + a.mix_spb_b);
+  }
+  fwd2_tile<TZ, TXT, NTH, RECOMP>(a, tile, s0, s1, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -1047,7 +1098,7 @@ __global__ __launch_bounds__(FWD2_THREADS) void fwd2_batch_kernel(
   a.it = it;
   a.one = single || a.nt - it == 1;
   const int local = u - unit_start[p];
-  fwd2_tile<TZ, TXT, FWD2_THREADS, false>(a, local % a.ntiles, local / a.ntiles);
+  fwd2_group<TZ, TXT, FWD2_THREADS, false>(a, local % a.ntiles, local / a.ntiles);
   if (done && it + (a.one ? 1 : 2) >= a.nt) {  // the problem's last launch
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -3182,7 +3233,7 @@ __global__ __launch_bounds__(FWD2_THREADS) void grad_batch_step_kernel(
   a.skip1 = a.it == -1;
   a.one = 0;
   const int local = u - unit_start[p];
-  fwd2_tile<TZ, TXT, FWD2_THREADS, RECOMP, true>(a, local % a.ntiles, local / a.ntiles);
+  fwd2_group<TZ, TXT, FWD2_THREADS, RECOMP, true>(a, local % a.ntiles, local / a.ntiles);
 }
 
 // per-problem fills and copies of the batched gradient (blockIdx.y = problem):
@@ -3410,6 +3461,9 @@ struct Plan {
   bool march;       // z-marching recompute instead of tiles
   bool amarch;      // z-marching two-step adjoint (adj2m_kernel) instead of tiles
   bool fmarch;      // z-marching two-step forward / recompute (fwd2m_kernel) instead of tiles
+  bool mixed;       // two-phase two-step forward / recompute launches (fwd2_mixed_kernel)
+  int mix_ta;       // their phase-A tiles
+  int eparts;       // energy partial maps (shot groups; the two-phase launches' larger count)
   int am_strips, am_segs, am_seg_h;  // its column strips, z segments and rows per segment
   bool two;         // two time steps per forward / recompute launch
   int TZ2, TX2, tiles_x2;  // tiles of the two-step kernels (16 x 128 or 24 x 96)
@@ -3482,8 +3536,8 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   // of 390 x 1740, 426 vs 622 us per launch)
   pl.amarch = pl.two && pass >= 1 &&
               (p->march == 3 || (p->march == 0 && pass == 1 && pl.S >= 8));
-  if (p->fwd_march < 0 || p->fwd_march > 2)
-    return fail(GBW_ERR_INVALID, "fwd_march must be 0, 1 or 2");
+  if (p->fwd_march < 0 || p->fwd_march > 3)
+    return fail(GBW_ERR_INVALID, "fwd_march must be 0, 1, 2 or 3");
   // z-marching two-step forward and recompute: one wavefront per shot of a
   // group of at most 8, on request; without a free surface, and not for the
   // density gradient's recompute (its extra brackets keep the tiles)
@@ -3497,6 +3551,21 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   }
   if (pl.fmarch) pl.rspb = pl.spb;
   pl.rgroups = (pl.S + pl.rspb - 1) / pl.rspb;
+  // two-phase two-step launches: phase A takes the largest multiple of the CU
+  // count of tiles, phase B the rest in groups of MIX_SPB_B shots; on request
+  // (fwd_march 3), and by default where phase A's groups are the larger
+  // (measured 334 -> 316 us per forward and 348 -> 326 us per recompute launch,
+  // 64 shots of 390 x 1740)
+  {
+    const int t2 = pl.tiles_x2 * ((d.NZ + pl.TZ2 - 1) / pl.TZ2);
+    pl.mix_ta = t2 / AM_TARGET_WGS * AM_TARGET_WGS;
+    const bool fits = pl.two && !pl.fmarch && pl.mix_ta > 0 && pl.mix_ta < t2;
+    const int spb = pass == 0 ? pl.spb : pl.rspb;  // the two-step launches' groups
+    pl.mixed = fits && (p->fwd_march == 3 || (p->fwd_march == 0 && spb > MIX_SPB_B));
+    pl.eparts = pl.groups;
+    if (pl.mixed && pass == 0)
+      pl.eparts = std::max(pl.groups, (pl.S + mix_spb_b(pl.spb) - 1) / mix_spb_b(pl.spb));
+  }
   pl.am_strips = (d.NX + AM_OWN - 1) / AM_OWN;
   {
     // z segments of the marching kernels: one workgroup runs per CU (its
@@ -3541,7 +3610,7 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   pl.off_LA = pl.off_LB = pl.off_D = pl.off_GX = pl.off_GZ = pl.off_G1 = pl.off_G2 = pl.off_GW = 0;
   pl.off_E = 0;
   if (pass == 0) {
-    pl.off_E = take((size_t)pl.groups * 3 * inner);
+    pl.off_E = take((size_t)pl.eparts * 3 * inner);
   } else {
     pl.off_LA = take(state);
     pl.off_LB = take(state);
@@ -3719,6 +3788,22 @@ void launch_steps(const Plan& pl, const FwdArgs& a_in, int steps, hipStream_t st
       fwd2m_kernel<4, RECOMP><<<n, 256, 0, st>>>(am);
     else
       fwd2m_kernel<8, RECOMP><<<n, 512, 0, st>>>(am);
+    return;
+  }
+  if (steps == 2 && pl.mixed) {
+    FwdArgs a2 = a;
+    a2.tiles_x = pl.tiles_x2;
+    a2.ntiles = pl.tiles_x2 * ((pl.d.NZ + pl.TZ2 - 1) / pl.TZ2);
+    a2.mix_ta = pl.mix_ta;
+    a2.mix_spb_b = mix_spb_b(a.spb);
+    const int ga = (a.S + a.spb - 1) / a.spb, gb = (a.S + a2.mix_spb_b - 1) / a2.mix_spb_b;
+    const int tb = a2.ntiles - pl.mix_ta;
+    const unsigned n = 8u * (unsigned)(pl.mix_ta / 8 * ga + (tb * gb + 7) / 8);
+    LaunchTimer timer(RECOMP ? "fwd2_mixed_kernel<recompute>" : "fwd2_mixed_kernel<forward>", st);
+    if (pl.TZ2 == 24)
+      fwd2_mixed_kernel<24, 96, FWD2_THREADS, RECOMP><<<n, FWD2_THREADS, 0, st>>>(a2);
+    else
+      fwd2_mixed_kernel<16, 128, FWD2_THREADS, RECOMP><<<n, FWD2_THREADS, 0, st>>>(a2);
     return;
   }
   LaunchTimer timer(steps == 2 ? (RECOMP ? "fwd2_step_kernel<recompute>" : "fwd2_step_kernel<forward>")
@@ -4107,7 +4192,7 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
   unsigned* MX = at<unsigned>(workspace, pl.off_MX);
   const long long slot = 3 * fstride;
   if (energy) {
-    if ((rc = fill(E, (long long)pl.groups * 3 * inner, 0.f, st))) return rc;
+    if ((rc = fill(E, (long long)pl.eparts * 3 * inner, 0.f, st))) return rc;
     if (out->energy_p && (rc = fill(out->energy_p, inner, 0.f, st))) return rc;
     if (out->energy_vx && (rc = fill(out->energy_vx, inner, 0.f, st))) return rc;
     if (out->energy_vz && (rc = fill(out->energy_vz, inner, 0.f, st))) return rc;
@@ -4151,7 +4236,7 @@ int gbw_acoustic2d_forward(const gbw_acoustic2d_problem* prob, const gbw_acousti
     const bool chunk_end = ((it + 1) % pl.echunk == 0) || it + 1 == pl.nt;
     if (energy && chunk_end) {
       energy_flush_kernel<<<(unsigned)((3 * inner + 255) / 256), 256, 0, st>>>(
-          E, pl.groups, inner, out->energy_p, out->energy_vx, out->energy_vz);
+          E, pl.eparts, inner, out->energy_p, out->energy_vx, out->energy_vz);
       LAUNCH_CHECK();
     }
     if (se > 0 && (((it + 1) % se == 0) || it + 1 == pl.nt)) {

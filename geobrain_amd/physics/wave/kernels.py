@@ -65,8 +65,10 @@ class LaunchChoice:
     ``shots_per_block`` and ``march`` (gradient: 1 tiles, 2 z-marching
     recompute strips, 3 z-marching two-step adjoint wavefronts),
     ``steps_per_launch`` (1 or 2 time steps per forward / recompute launch),
-    ``fwd_march`` (two-step forward and recompute launches: 1 tiles, 2
-    z-marching wavefronts); zeros select the library's defaults.
+    ``fwd_march`` (two-step forward and recompute launches: 1 tiles in shot
+    groups, 2 z-marching wavefronts, 3 two-phase tiles: the first multiple of
+    256 tiles in shot groups and the rest in groups of 8 shots); zeros select
+    the library's defaults.
     ``bwd_tile_rows`` and
     ``bwd_shots_per_block`` apply to the gradient call (0 = same as the
     forward's): the forward and adjoint kernels have different register

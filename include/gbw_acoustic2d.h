@@ -77,10 +77,13 @@ typedef struct gbw_acoustic2d_problem {
   int32_t steps_per_launch;  /* launch choice: time steps per forward/recompute launch */
                              /* (1 or 2; 0 = default 2; two-step launches tile 16 rows) */
   int32_t fwd_march;         /* launch choice for two-step forward and recompute        */
-                             /* launches: 0 auto (tiles), 1 tiles, 2 z-marching      */
-                             /* wavefronts                                              */
-                             /* (without a free surface; the rho gradient's recompute   */
-                             /* keeps tiles)                                            */
+                             /* launches: 1 tiles in shot groups; 2 z-marching          */
+                             /* wavefronts (without a free surface; the rho gradient's  */
+                             /* recompute keeps tiles); 3 two-phase tiles (the first    */
+                             /* multiple of 256 tiles in shot groups, the rest in       */
+                             /* groups of 8 shots); 0 auto: 3 where the grid has more   */
+                             /* than 256 two-step tiles and groups of more than 8       */
+                             /* shots, else 1                                           */
 } gbw_acoustic2d_problem;
 
 /* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model

@@ -1,0 +1,84 @@
+"""Two-phase two-step forward and recompute (launch choice fwd_march = 3, the
+default where it applies; fwd2_mixed_kernel: the first multiple-of-256 tiles
+in the requested shot groups, the rest in groups of 8 shots, each XCD an equal
+share of both) against the shot-group launches (fwd_march = 1) and the FP32
+oracle: seismograms and vp / rho / wavelet gradients bit-identical (every cell
+gets the same arithmetic), energy maps within 1e-6 relative L2 (phase-B tiles
+sum their shots in smaller pieces).  Grids of 308 and 350 two-step tiles
+(phase A 256), shot counts that leave partial groups in both phases, phase-A
+groups smaller than, equal to and larger than 8, odd step counts (one-step
+launches mix in), a free surface, the density gradient's recompute, and the
+benchmark's 64-shot problem for a shortened time axis."""
+
+import numpy as np
+import pytest
+
+import oracle
+from geobrain_amd.physics.wave import kernels as K
+from tests._fixtures import chunk_len, rel_l2
+from tests.test_gpu_adj_march import _problem
+from tests.test_gpu_fwd_march import ENERGY, _assert_same, _run
+
+pytestmark = pytest.mark.gpu
+
+
+def _launch(spb, fwd_march, adj_march=1, bwd_spb=None):
+    # tile_rows, shots_per_block, march, bwd_tile_rows, bwd_shots_per_block,
+    # steps_per_launch, fwd_march
+    return K.LaunchChoice(0, spb, adj_march, 0, bwd_spb or spb, 2, fwd_march)
+
+
+@pytest.mark.parametrize("nz,nx,nt,S,spb,segments", [
+    (300, 1700, 61, 5, 4, 2),     # 308 tiles; groups of 4 + 1 in both phases
+    (300, 1700, 40, 11, 8, 2),    # groups of 8 + 3 in both phases
+    (350, 1700, 33, 9, 9, 3),     # 350 tiles; one phase-A group, phase B 8 + 1
+    (300, 1700, 24, 20, 16, 2),   # phase A 16 + 4, phase B 8 + 8 + 4
+])
+def test_two_phase_equals_shot_groups(nz, nx, nt, S, spb, segments):
+    print(f"[case] two-phase vs shot groups: {nz}x{nx}, nt {nt}, {S} shots, spb {spb}", flush=True)
+    p = _problem(nz, nx, nt, S, seed=nz + nx + 5)
+    groups = _run(p, _launch(spb, 1), segments=segments)
+    mixed = _run(p, _launch(spb, 3), segments=segments)
+    assert float(np.abs(groups["grad_vp"]).max()) > 0
+    _assert_same(groups, mixed, "two-phase vs shot groups")
+    both = _run(p, _launch(spb, 3, 3), segments=segments)
+    _assert_same(groups, both, "two-phase with the marching adjoint vs shot groups")
+
+
+def test_two_phase_matches_oracle():
+    """Seismograms bit-identical to the FP32 oracle; energy maps within 1e-6."""
+    print("[case] two-phase forward vs FP32 oracle, 300x1700, 30 steps, 6 shots", flush=True)
+    nz, nx, nt = 300, 1700, 30
+    p = _problem(nz, nx, nt, 6, seed=47)
+    got = _run(p, _launch(4, 3), segments=2)
+    want = oracle.forward(nz=nz, nx=nx, nabc=20, free_surface=False, dt=5e-4, dz=5.0,
+                          src_z=p["sz"], src_x=p["sx"], src_v=p["w"], rcv_z=p["rz"],
+                          rcv_x=p["rx"], vp=p["vp"], rho=p["rho"], damp=p["damp"],
+                          chunk=chunk_len(nt, 2), precision="f32")
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    for k in ENERGY:
+        assert rel_l2(got[k], want[k]) <= 1e-6, k
+
+
+@pytest.mark.parametrize("fs,rho", [(True, False), (False, True)])
+def test_two_phase_free_surface_and_density_gradient(fs, rho):
+    print(f"[case] two-phase, free_surface={fs} rho gradient={rho}", flush=True)
+    p = _problem(300, 1700, 25, 4, seed=53)
+    groups = _run(p, _launch(4, 1), fs=fs, rho_grad=rho)
+    mixed = _run(p, _launch(4, 3), fs=fs, rho_grad=rho)
+    _assert_same(groups, mixed, f"free_surface={fs} rho={rho}")
+
+
+def test_two_phase_full_size_64_shots():
+    """The benchmark's grid and shots (350 x 1700, 64 shots, receivers on every
+    column), 200 steps, checkpoints every 50, forward groups of 32 and recompute
+    groups of 32 beside the marching adjoint's 8."""
+    print("[case] two-phase forward and recompute, 64 shots 350x1700, 200 steps", flush=True)
+    p = _problem(350, 1700, 200, 64, seed=59)
+    p["rx"] = np.arange(1700)
+    p["rz"] = np.zeros(1700, np.int64)
+    p["sz"] = np.zeros(64, np.int64)
+    groups = _run(p, K.LaunchChoice(16, 32, 3, 16, 8, 2, 1), segments=4)
+    mixed = _run(p, K.LaunchChoice(16, 32, 3, 16, 32, 2, 3), segments=4)
+    _assert_same(groups, mixed, "two-phase vs shot groups")
