@@ -2397,10 +2397,13 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       if (lz && a.g_vz) lz->y = lz->y + a.g_vz[o];
     }
   };
+  // The stage helpers take the row's coefficients through a getter C(plane)
+  // (plane: 0 omk1, 1 a1, 2 omk2, 3 omk3, 4 a2): an LDS ring read where the
+  // helper needs it, or a value read earlier (see the iteration).
   // q = a2 * lambda_v of row r (X / Z regions only); G: test the row
-  auto make_q = [&](auto G, int r, const float2* cr, const float2& lx, const float2& lz,
+  auto make_q = [&](auto G, int r, auto C, const float2& lx, const float2& lz,
                     float2& qx, float2& qz) {
-    const float2 c2 = cr[4 * AM_CROWS * 64];
+    const float2 c2 = C(4);
     qx = make_float2(c2.x * lx.x, c2.y * lx.y);
     qz = make_float2(c2.x * lz.x, c2.y * lz.y);
     if constexpr (decltype(G)::value) {
@@ -2420,7 +2423,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
   // With a free surface, the transpose of p[f-1] = -p[f+1]: lambda_p' of row
   // f-1 (kept in *sfm) is taken from row f+1 and row f-1 becomes 0 (rows above
   // f carry no q, so row f-1's gather is the reference's closed form)
-  auto lam_p = [&](auto G, int r, const float2* cr, const float2& lp, const float2& qx,
+  auto lam_p = [&](auto G, int r, auto C, const float2& lp, const float2& qx,
                    const float2& zm2, const float2& zm1, const float2& z00, const float2& zp1,
                    float2& lpv, float2& rr, float2& lpo, bool* pr, float2* sfm) {
     const float ql0 = lane_prev(qx.x), ql1 = lane_prev(qx.y), qr0 = lane_next(qx.x);
@@ -2455,7 +2458,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
         lpv.y = lpv.y - sfm->y;
       }
     }
-    const float2 c1 = cr[1 * AM_CROWS * 64], o1 = cr[0];
+    const float2 c1 = C(1), o1 = C(0);
     rr = make_float2(c1.x * lpv.x, c1.y * lpv.y);
     lpo = make_float2(o1.x * lpv.x, o1.y * lpv.y);
     pr[0] = rp && (!CM || cp[0]);
@@ -2466,11 +2469,11 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
     }
   };
   // lambda_v after the step: omk * lambda_v + gather of r (rows r-1..r+2)
-  auto lam_v = [&](auto G, int r, const float2* cr, float2& lx, float2& lz, const float2& rm1,
+  auto lam_v = [&](auto G, int r, auto C, float2& lx, float2& lz, const float2& rm1,
                    const float2& r00, const float2& rp1, const float2& rp2) {
     constexpr bool GG = decltype(G)::value;
     const float rl1 = lane_prev(r00.y), rr0 = lane_next(r00.x), rr1 = lane_next(r00.y);
-    const float2 o2 = cr[2 * AM_CROWS * 64], o3 = cr[3 * AM_CROWS * 64];
+    const float2 o2 = C(2), o3 = C(3);
     bool rx = true, rz = true, rin = true, ml = true;
     if constexpr (GG) {
       rx = row_x(r); rz = row_z(r); rin = row_in(r); ml = rin && r <= d.NZ - 2;
@@ -2525,6 +2528,32 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
     const float2 lp_in = pf_lp[pk], lx_in = pf_lx[pk], lz_in = pf_lz[pk], d0 = pf_d0[pk],
                  d1 = pf_d1[pk];
     prefetch(std::integral_constant<int, pk>{}, i + AM_PF);
+    // the rows' coefficients: without a free surface the ten values the
+    // stages use are read from the LDS ring up front, so the reads' latencies
+    // overlap (adjoint 426 -> 422 us per launch, 64 shots of 390 x 1740); the
+    // free-surface instance has no registers to spare and reads each where its
+    // stage needs it
+    constexpr int CP = AM_CROWS * 64;
+    auto ring = [&](int row) { return sCl + (row & (AM_CROWS - 1)) * 64; };
+    float2 cpre[10] = {};
+    if constexpr (!FS) {
+      const float2 *crj = ring(i + 6), *crb = ring(i + 5), *crc = ring(i + 3), *crd = ring(i + 2),
+                   *cri = ring(i);
+      cpre[0] = crj[4 * CP]; cpre[1] = crb[CP]; cpre[2] = crb[0]; cpre[3] = crc[2 * CP];
+      cpre[4] = crc[3 * CP]; cpre[5] = crc[4 * CP]; cpre[6] = crd[CP]; cpre[7] = crd[0];
+      cpre[8] = cri[2 * CP]; cpre[9] = cri[3 * CP];
+    }
+    // getter of row `row`'s planes: v[plane] read up front, or the ring
+    const float2 z = f2zero();
+    auto getter = [&](int row, float2 v0, float2 v1, float2 v2, float2 v3, float2 v4) {
+      return [&, row, v0, v1, v2, v3, v4](int plane) -> float2 {
+        if constexpr (FS) {
+          return ring(row)[plane * CP];
+        } else {
+          return plane == 0 ? v0 : (plane == 1 ? v1 : (plane == 2 ? v2 : (plane == 3 ? v3 : v4)));
+        }
+      };
+    };
     // --- row j = i+6: lambda of step it+1, q0 ---
     {
       constexpr int sj = (uu + 6) & 3;
@@ -2544,7 +2573,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
         }
       }
       WLP[sj] = lp; WLX[sj] = lx; WLZ[sj] = lz;
-      make_q(G, j, sCl + (j & (AM_CROWS - 1)) * 64, lx, lz, WQX0[sj], WQZ0[sj]);
+      make_q(G, j, getter(i + 6, z, z, z, z, cpre[0]), lx, lz, WQX0[sj], WQZ0[sj]);
     }
     // --- row b = i+5: lambda_p' of step it, r1, lambda_p after it, imaging ---
     {
@@ -2552,7 +2581,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       const int b = i + 5;
       float2 lpv, lpo;
       bool pr[2];
-      lam_p(G, b, sCl + (b & (AM_CROWS - 1)) * 64, WLP[sb], WQX0[sb], WQZ0[(uu + 3) & 3],
+      lam_p(G, b, getter(i + 5, cpre[2], cpre[1], z, z, z), WLP[sb], WQX0[sb], WQZ0[(uu + 3) & 3],
             WQZ0[(uu + 4) & 3], WQZ0[sb], WQZ0[(uu + 6) & 3], lpv, WR1[sb], lpo, pr, &sfm0);
       if (!GG || (b >= zs && b < ze)) {
         float2 t1 = f2zero();
@@ -2573,9 +2602,9 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
     {
       constexpr int sc = (uu + 3) & 3;
       const int c = i + 3;
-      const float2* cr = sCl + (c & (AM_CROWS - 1)) * 64;
+      const auto cc = getter(i + 3, z, z, cpre[3], cpre[4], cpre[5]);
       float2 lx = WLX[sc], lz = WLZ[sc];
-      lam_v(G, c, cr, lx, lz, WR1[(uu + 2) & 3], WR1[sc], WR1[uu & 3], WR1[(uu + 1) & 3]);
+      lam_v(G, c, cc, lx, lz, WR1[(uu + 2) & 3], WR1[sc], WR1[uu & 3], WR1[(uu + 1) & 3]);
       if constexpr (GG) {
         if (it >= 1 && a.R > 0 && row_in(c) && a.rrow[c]) inject(c, it - 1, nullptr, &lx, &lz);
         // the second step's transpose of vz[f-1] = vz[f]
@@ -2589,7 +2618,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
         }
       }
       WLX1[sc] = lx; WLZ1[sc] = lz;
-      make_q(G, c, cr, lx, lz, WQX1[sc], WQZ1[sc]);
+      make_q(G, c, cc, lx, lz, WQX1[sc], WQZ1[sc]);
     }
     // --- row dr = i+2: lambda_p' of step it-1, r2, lambda_p (stored), imaging ---
     {
@@ -2597,7 +2626,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       const int dr = i + 2;
       float2 lpv, lpo, r2;
       bool pr[2];
-      lam_p(G, dr, sCl + (dr & (AM_CROWS - 1)) * 64, WLP1[sd], WQX1[sd], WQZ1[uu & 3],
+      lam_p(G, dr, getter(i + 2, cpre[7], cpre[6], z, z, z), WLP1[sd], WQX1[sd], WQZ1[uu & 3],
             WQZ1[(uu + 1) & 3], WQZ1[sd], WQZ1[(uu + 3) & 3], lpv, r2, lpo, pr, &sfm1);
       WR2[sd] = r2;
       if (!GG || (dr >= zs && dr < ze)) {
@@ -2623,7 +2652,7 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
       constexpr int se = uu & 3;
       if (!GG || (i >= zs && i < ze)) {
         float2 lx = WLX1[se], lz = WLZ1[se];
-        lam_v(G, i, sCl + (i & (AM_CROWS - 1)) * 64, lx, lz, WR2[(uu + 3) & 3], WR2[se],
+        lam_v(G, i, getter(i, z, z, cpre[8], cpre[9], z), lx, lz, WR2[(uu + 3) & 3], WR2[se],
               WR2[(uu + 1) & 3], WR2[(uu + 2) & 3]);
         if constexpr (GG) {
           if (it - 1 >= 1 && a.R > 0 && a.rrow[i]) inject(i, it - 2, nullptr, &lx, &lz);
