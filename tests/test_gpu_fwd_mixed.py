@@ -82,3 +82,39 @@ def test_two_phase_full_size_64_shots():
     groups = _run(p, K.LaunchChoice(16, 32, 3, 16, 8, 2, 1), segments=4)
     mixed = _run(p, K.LaunchChoice(16, 32, 3, 16, 32, 2, 3), segments=4)
     _assert_same(groups, mixed, "two-phase vs shot groups")
+
+
+def test_two_phase_snapshots_and_energy_chunks_match_oracle():
+    """The library's default launch on a grid that takes two-phase launches
+    (300 x 1700, 10 shots in one group: phase B groups of 8 and 2), in snapshot
+    mode (shot-0 pressure after every 7 steps, so one-step launches end each
+    chunk) and, separately, with 3 energy chunks: seismograms bit-identical to
+    the FP32 oracle, snapshots and energy maps within 1e-6 relative L2."""
+    print("[case] two-phase default launch: snapshots and energy chunks, 300x1700", flush=True)
+    import torch
+    from geobrain_amd.physics import wave
+    nz, nx, nt, S = 300, 1700, 40, 10
+    p = _problem(nz, nx, nt, S, seed=61)
+    dev = torch.device("cuda:0")
+    T = lambda a, **kw: torch.as_tensor(np.asarray(a), device=dev, **kw)  # noqa: E731
+    for snap, segments in ((7, 1), (0, 3)):
+        K.set_launch_choice(K.LaunchChoice(0, S, 0, 0, S, 2, 0))
+        try:
+            out = wave.acoustic_forward_kernel(
+                nx=nx, nz=nz, dx=5.0, dz=5.0, nt=nt, dt=5e-4, nabc=20, free_surface=False,
+                src_x=T(p["sx"]), src_z=T(p["sz"]), src_n=S, src_v=T(p["w"], dtype=torch.float32),
+                rcv_x=T(p["rx"]), rcv_z=T(p["rz"]), rcv_n=len(p["rx"]),
+                damp=T(p["damp"]), vp=T(p["vp"], dtype=torch.float32),
+                rho=T(p["rho"], dtype=torch.float32), checkpoint_segments=segments,
+                snapshot_every=snap)
+        finally:
+            K.set_launch_choice(None)
+        want = oracle.forward(nz=nz, nx=nx, nabc=20, free_surface=False, dt=5e-4, dz=5.0,
+                              src_z=p["sz"], src_x=p["sx"], src_v=p["w"], rcv_z=p["rz"],
+                              rcv_x=p["rx"], vp=p["vp"], rho=p["rho"], damp=p["damp"],
+                              chunk=chunk_len(nt, segments), snapshot_every=snap,
+                              precision="f32")
+        for k in ("p", "vx", "vz"):
+            np.testing.assert_array_equal(out[k].cpu().numpy(), want[k], err_msg=f"{k} snap={snap}")
+        for k in ENERGY + (("snapshots",) if snap else ()):
+            assert rel_l2(out[k].cpu().numpy(), want[k]) <= 1e-6, (k, snap)
