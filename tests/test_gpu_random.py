@@ -2,7 +2,8 @@
 oracle: grids from the smallest padded size up, absorbing layers of 1-8 cells,
 with and without a free surface, 1-5 shots, 1-60 steps, sources and receivers
 anywhere the reference accepts them (absorbing layers and padded edges
-included, duplicates allowed) and the launch choices drawn at random too.
+included, duplicates allowed) and the launch choices drawn at random too
+(z-marching adjoint and forward included).
 Bar: seismograms bit for bit, energy maps to 1e-6 and gradients to 1e-5
 (relative L2; the sums are associated differently across launch choices)."""
 
@@ -54,7 +55,16 @@ def case_2d(seed):
                             march=int(rng.integers(0, 3)),
                             bwd_shots_per_block=int(rng.integers(0, 3)),
                             steps_per_launch=spl)
-    return a, launch, int(rng.integers(1, 4))
+    segs = int(rng.integers(1, 4))
+    # drawn last, so the cases above keep their values: the z-marching adjoint
+    # (march 3) and / or the z-marching forward (fwd_march 2) for two-step
+    # launches (the library keeps the tiles where they do not apply)
+    extra = int(rng.integers(0, 4))
+    if extra & 1:
+        launch.march = 3
+    if extra & 2:
+        launch.fwd_march = 2
+    return a, launch, segs
 
 
 @pytest.mark.parametrize("seed", range(24))
@@ -115,7 +125,12 @@ def case_3d(seed):
              rcv_x=rng.integers(-nabc, nx + nabc - 1, R))
     spb = int(rng.integers(0, 3))
     kernel = int(rng.integers(0, 2))
-    return a, spb, kernel, int(rng.integers(1, 4))
+    segs = int(rng.integers(1, 4))
+    # drawn last, so the cases above keep their values: the fused step
+    # (step_kernel 2, forward only; the gradient's recompute keeps the split)
+    if int(rng.integers(0, 3)) == 2:
+        kernel = 2
+    return a, spb, kernel, segs
 
 
 @pytest.mark.parametrize("seed", range(16))
