@@ -1395,6 +1395,7 @@ struct AdjArgs {
   float* gw;    // [S][nt] or null
   int one;      // two-step kernel: adjoint step it only (it == 0: the problem's first step)
   int am_segs, am_seg_h;  // z-marching adjoint: z segments per strip, rows per segment
+  int am_seg0_h;          // rows of its first segment (the others: am_seg_h)
   // the batched gradient's launches (PHASED kernel): the per-problem arguments
   // come from these descriptors, the rest of this struct is unused
   const struct GradDesc* batch_descs;
@@ -2350,7 +2351,8 @@ __device__ __forceinline__ void adj2m_body(const AdjArgs& a, int group, int seg,
   const int ns = min(a.S, s_begin + a.spb) - s_begin;
   const bool active = w < ns;  // uniform per wavefront
   const int s = s_begin + (active ? w : 0);
-  const int zs = seg * a.am_seg_h, ze = min(d.NZ, zs + a.am_seg_h);
+  const int zs = seg == 0 ? 0 : a.am_seg0_h + (seg - 1) * a.am_seg_h;
+  const int ze = min(d.NZ, seg == 0 ? a.am_seg0_h : zs + a.am_seg_h);
   const int xs = strip * AM_OWN - 6;
   const int x0 = xs + 2 * lane;
   const bool lane_ok = x0 >= 0 && x0 <= d.PX - 2;
@@ -3500,6 +3502,7 @@ struct Plan {
   int mix_ta;       // their phase-A tiles
   int eparts;       // energy partial maps (shot groups; the two-phase launches' larger count)
   int am_strips, am_segs, am_seg_h;  // its column strips, z segments and rows per segment
+  int am_seg0_h;    // the adjoint's first segment (shorter with a free surface)
   bool two;         // two time steps per forward / recompute launch
   int TZ2, TX2, tiles_x2;  // tiles of the two-step kernels (16 x 128 or 24 x 96)
   int mseg, mseg_h; // z segments per strip and rows per segment for it
@@ -3619,6 +3622,23 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
     }
     pl.am_seg_h = ((d.NZ + best_ns - 1) / best_ns + 3) / 4 * 4;
     pl.am_segs = (d.NZ + pl.am_seg_h - 1) / pl.am_seg_h;
+    pl.am_seg0_h = pl.am_seg_h;
+    // With a free surface the adjoint's first segment also walks the rows
+    // above the surface in its guarded (slower) form: it takes f rows fewer
+    // than the others, so the segments end closer together (64 shots of
+    // 390 x 1740: 471 -> 455 us per launch; f/2 and 3f/2 rows fewer measured
+    // 459 and 461 us).  Only where the surface is a small part of a segment;
+    // the segment count stays.
+    if (d.fs && pl.am_segs >= 2 && 4 * d.f <= pl.am_seg_h) {
+      const int ns = pl.am_segs;
+      const int h0 = std::max(16, (d.NZ - (ns - 1) * d.f) / ns / 4 * 4);
+      const int h = ((d.NZ - h0 + ns - 2) / (ns - 1) + 3) / 4 * 4;
+      if (h0 < h && h0 + (ns - 1) * h >= d.NZ) {
+        pl.am_seg0_h = h0;
+        pl.am_seg_h = h;
+        pl.am_segs = 1 + (d.NZ - h0 + h - 1) / h;
+      }
+    }
   }
   {
     const int strips = pl.tiles_x;
@@ -3867,6 +3887,7 @@ void launch_adj2(const Plan& pl, const AdjArgs& a, hipStream_t st) {
     am.ntiles = pl.am_strips * pl.am_segs;
     am.am_segs = pl.am_segs;
     am.am_seg_h = pl.am_seg_h;
+    am.am_seg0_h = pl.am_seg0_h;
     const unsigned n = (unsigned)(((long long)am.ntiles * pl.groups + 7) / 8 * 8);
     LaunchTimer timer("adj2m_kernel", st);
     if (pl.d.fs) {
