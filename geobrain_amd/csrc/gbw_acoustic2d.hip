@@ -3623,15 +3623,19 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
     pl.am_seg_h = ((d.NZ + best_ns - 1) / best_ns + 3) / 4 * 4;
     pl.am_segs = (d.NZ + pl.am_seg_h - 1) / pl.am_seg_h;
     pl.am_seg0_h = pl.am_seg_h;
-    // With a free surface the adjoint's first segment also walks the rows
-    // above the surface in its guarded (slower) form: it takes f rows fewer
-    // than the others, so the segments end closer together (64 shots of
-    // 390 x 1740: 471 -> 455 us per launch; f/2 and 3f/2 rows fewer measured
-    // 459 and 461 us).  Only where the surface is a small part of a segment;
-    // the segment count stays.
-    if (d.fs && pl.am_segs >= 2 && 4 * d.f <= pl.am_seg_h) {
+    // The adjoint's first segment holds the rows a survey's sources and
+    // receivers usually sit on, and with a free surface the rows above the
+    // surface: blocks there run in the guarded (slower) form.  In a launch of
+    // one round, where the longest segment sets the time, it takes `shift`
+    // rows fewer than the others (64 shots of 390 x 1740, receivers and
+    // sources on row 2: 8 rows fewer 425 -> 416 us per launch, 4 and 16 fewer
+    // 421 and 422 us; with a free surface f = 20 rows fewer 471 -> 455 us,
+    // f/2 and 3f/2 fewer 459 and 461 us).  The segment count stays.
+    const int shift = d.fs ? std::max(d.f, 8) : 8;
+    if (pl.am_segs >= 2 && (long long)per_seg * pl.am_segs <= AM_TARGET_WGS &&
+        4 * shift <= pl.am_seg_h) {
       const int ns = pl.am_segs;
-      const int h0 = std::max(16, (d.NZ - (ns - 1) * d.f) / ns / 4 * 4);
+      const int h0 = std::max(16, (d.NZ - (ns - 1) * shift) / ns / 4 * 4);
       const int h = ((d.NZ - h0 + ns - 2) / (ns - 1) + 3) / 4 * 4;
       if (h0 < h && h0 + (ns - 1) * h >= d.NZ) {
         pl.am_seg0_h = h0;
