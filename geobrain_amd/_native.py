@@ -18,7 +18,7 @@ c_i32 = ctypes.c_int32
 c_f32 = ctypes.c_float
 c_vp = ctypes.c_void_p
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 # size of the device array behind gbw_acoustic2d_forward_noalloc (GBW_NOALLOC_SCRATCH_BYTES)
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -57,7 +57,7 @@ class Problem(ctypes.Structure):
         ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
         ("snapshot_every", c_i32), ("checkpoint_every", c_i32), ("checkpoint_bf16", c_i32),
         ("tile_rows", c_i32), ("shots_per_block", c_i32), ("march", c_i32),
-        ("steps_per_launch", c_i32), ("fwd_march", c_i32),
+        ("steps_per_launch", c_i32), ("fwd_march", c_i32), ("src_points", c_i32),
     ]
 
 

@@ -146,8 +146,8 @@ struct FwdArgs {
   const float* P0; const float* VX0; const float* VZ0;  // input state fields (shot-major)
   float* P1; float* VX1; float* VZ1;                    // output state fields
   const float* coef;
-  const int* src_cell;           // [S] padded cell index (row * PX + col)
-  const float* wav; long long wav_stride;  // w[s * wav_stride + it]
+  const int* src_cell;           // [S * src_k] padded cell index (row * PX + col) of each point
+  const float* wav; long long wav_stride;  // w[s * wav_stride + k * wav_kstride + it]
   int it, nt; float dt;
   const int* rhead; const int* rnext; const unsigned char* rrow; int R;
   float* rec_p; float* rec_vx; float* rec_vz;  // [S][nt][R] or null
@@ -158,6 +158,8 @@ struct FwdArgs {
   int one;                       // two-step kernel: do step it only (the last of a problem)
   int skip1;                     // two-step kernel: step it is before the problem's first
                                  // step (it == -1): run step it+1 only, from the input state
+  long long wav_kstride;         // wavelet row of point k of a shot: + k * wav_kstride
+  int src_k;                     // source points per shot (distinct cells)
 };
 
 // ---------------------------------------------------------------------------
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
 
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const int scell = a.src_cell[s];
+    const int scell = a.src_cell[s * a.src_k];
     const float supd = a.dt * a.wav[(long long)s * a.wav_stride + a.it];
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
@@ -371,6 +373,12 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
             pv = u - f4get(pc_a1[n], e) * br;
           }
           if ((p_in & bit) && (int)(base + e) == scell) pv = pv + supd;
+          // further points of the shot (acoustic_2d.py:234-237; distinct
+          // cells, so at most one adds here)
+          if (a.src_k > 1 && (p_in & bit))
+            for (int k = 1; k < a.src_k; ++k)
+              if ((int)(base + e) == a.src_cell[s * a.src_k + k])
+                pv = pv + a.dt * a.wav[(long long)s * a.wav_stride + k * a.wav_kstride + a.it];
           f4set(out, e, pv);
           f4set(dv, e, br);
         }
@@ -1401,6 +1409,7 @@ struct AdjArgs {
   const struct GradDesc* batch_descs;
   const int* batch_units;  // n + 1 prefix sums of work units
   int batch_n, batch_g, batch_parity, batch_c, batch_k;
+  int src_k;  // source points per shot (src_cell[s * src_k + k]; gw row s * src_k + k)
 };
 
 // RHO: also accumulate the density gradient (needs the recompute's vx/vz brackets)
@@ -1682,7 +1691,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
         const int4 rh = ((t_rcv >> n) & 1u) ? reinterpret_cast<const int4*>(a.rhead)[ti]
                                            : make_int4(-1, -1, -1, -1);
         float4 lpo = lp4;
-        const int scell = a.src_cell[s];
+        const int scell = a.src_cell[s * a.src_k];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const unsigned bit = 1u << (4 * n + e);
@@ -1692,7 +1701,11 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
             f4set(lpo, e, f4get(tc_omk1[n], e) * lp);
           }
           if ((int)(ti * 4u + e) == scell && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
-            a.gw[(long long)s * a.nt + a.it] = a.dt * lp;
+            a.gw[(long long)s * a.src_k * a.nt + a.it] = a.dt * lp;
+          if (a.src_k > 1 && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
+            for (int k = 1; k < a.src_k; ++k)
+              if ((int)(ti * 4u + e) == a.src_cell[s * a.src_k + k])
+                a.gw[((long long)s * a.src_k + k) * a.nt + a.it] = a.dt * lp;
           float lx = f4get(lx4, e), lz = f4get(lz4, e);
           if constexpr (RHO) {
             float g2 = f4get(gacc2[n], e);
@@ -3404,6 +3417,18 @@ __global__ void wavelet_grad_kernel(const float* __restrict__ gw, int S, int nt,
   wavelet_grad(gw, S, nt, shared, out, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// several points per shot, one wavelet per shot: the samples of its K points
+// (rows s*K .. s*K+K-1 of gw) summed in point order
+__global__ void wavelet_grad_points_kernel(const float* __restrict__ gw, int S, int K, int nt,
+                                           float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)S * nt) return;
+  const long long s = t / nt, i = t - s * nt;
+  float v = gw[s * K * nt + i];
+  for (int k = 1; k < K; ++k) v = v + gw[(s * K + k) * nt + i];
+  out[t] = v;
+}
+
 // a problem of the batched gradient is done: chain rule and pad_model fold of
 // its imaging sums (blockIdx.y = model row), then its wavelet gradient
 __global__ void grad_batch_fold_kernel(const GradDesc* __restrict__ descs, int p) {
@@ -3493,6 +3518,7 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Plan {
   Dims d;
   int S, R, nt, k, nseg, TZ, spb, groups, tiles_x, tiles_z, echunk;
+  int K;            // source points per shot
   int rspb, rgroups;  // shot groups of the gradient's tiled recompute (the adjoint's may be smaller)
   bool bf16;
   bool march;       // z-marching recompute instead of tiles
@@ -3537,6 +3563,14 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   // in-plane float4 accesses use 32-bit byte offsets
   if (d.plane >= (1LL << 30)) return fail(GBW_ERR_INVALID, "padded plane exceeds 2^30 cells");
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
+  if (p->src_points < 0) return fail(GBW_ERR_INVALID, "src_points must be >= 0");
+  pl.K = std::max(1, p->src_points);
+  if (p->wavelet_shared < 0 || p->wavelet_shared > 2)
+    return fail(GBW_ERR_INVALID, "wavelet_shared must be 0, 1 or 2");
+  if (pl.K > 1 && p->wavelet_shared == 1)
+    return fail(GBW_ERR_INVALID,
+                "several source points per shot take src_v of shape (src_n, nt) or "
+                "(src_n, src_points, nt)");
   pl.k = p->checkpoint_every > 0 ? std::min(p->checkpoint_every, p->nt) : 0;
   if (pass >= 1 && pl.k == 0) return fail(GBW_ERR_INVALID, "the gradient needs checkpoint_every > 0");
   pl.nseg = pl.k ? (p->nt + pl.k - 1) / pl.k : 0;
@@ -3564,10 +3598,12 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
   if (p->march < 0 || p->march > 3) return fail(GBW_ERR_INVALID, "march must be 0, 1, 2 or 3");
   if (p->steps_per_launch < 0 || p->steps_per_launch > 2)
     return fail(GBW_ERR_INVALID, "steps_per_launch must be 0, 1 or 2");
-  pl.two = p->steps_per_launch != 1;
+  // several source points per shot: one-step tiled launches (the two-step and
+  // z-marching kernels inject one point per shot)
+  pl.two = p->steps_per_launch != 1 && pl.K == 1;
   // z-marching recompute only on request: with coefficients reloaded per shot
   // it measured slower than tiles on 128-column strips
-  pl.march = p->march == 2;
+  pl.march = p->march == 2 && pl.K == 1;
   // z-marching two-step adjoint: one wavefront per shot of a group of at most 8;
   // chosen by default for gradients of 8 or more shots without the density
   // gradient (it fills its workgroups and measured faster than tiles: 64 shots
@@ -3680,7 +3716,7 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
       pl.off_G2 = take((size_t)pl.groups * pl.plane_bytes);
     }
     pl.off_G1 = take((size_t)pl.groups * pl.plane_bytes);
-    pl.off_GW = take((size_t)pl.S * pl.nt * sizeof(float));
+    pl.off_GW = take((size_t)pl.S * pl.K * pl.nt * sizeof(float));
   }
   {
     size_t g = 0;
@@ -3690,7 +3726,7 @@ int make_plan(const gbw_acoustic2d_problem* p, int pass, Plan& pl) {
     pl.off_rcell = gtake((size_t)std::max(1, pl.R) * sizeof(int));
     pl.off_heads = gtake(2 * (size_t)std::max(1, pl.R) * sizeof(int));
     pl.off_rrow = gtake((size_t)d.NZ);
-    pl.off_src = gtake((size_t)pl.S * sizeof(int));
+    pl.off_src = gtake((size_t)pl.S * pl.K * sizeof(int));
     pl.geo_total = g;
   }
   pl.off_geo = take(pl.geo_total);
@@ -3747,14 +3783,24 @@ int setup_geometry(const Plan& pl, const gbw_acoustic2d_inputs* in, void* ws, hi
   const Dims& d = pl.d;
   if (!in || !in->src_x || !in->src_z) return fail(GBW_ERR_INVALID, "source positions are NULL");
   if (pl.R > 0 && (!in->rcv_x || !in->rcv_z)) return fail(GBW_ERR_INVALID, "receiver positions are NULL");
-  std::vector<int> src(pl.S);
-  for (int s = 0; s < pl.S; ++s) {
-    const long long z = in->src_z[s] + d.nabc, x = in->src_x[s] + d.nabc;
+  // point k of shot s at s * K + k
+  std::vector<int> src((size_t)pl.S * pl.K);
+  for (int i = 0; i < pl.S * pl.K; ++i) {
+    const long long z = in->src_z[i] + d.nabc, x = in->src_x[i] + d.nabc;
+    const std::string who = pl.K > 1 ? "source " + std::to_string(i / pl.K) + " point " +
+                                           std::to_string(i % pl.K)
+                                     : "source " + std::to_string(i);
     if (z < 0 || z >= d.NZ || x < 0 || x >= d.NX)
-      return fail(GBW_ERR_INDEX, "index out of range: source " + std::to_string(s) + " at (z=" +
-                                     std::to_string(in->src_z[s]) + ", x=" +
-                                     std::to_string(in->src_x[s]) + ")");
-    src[s] = (int)(z * d.PX + x);
+      return fail(GBW_ERR_INDEX, "index out of range: " + who + " at (z=" +
+                                     std::to_string(in->src_z[i]) + ", x=" +
+                                     std::to_string(in->src_x[i]) + ")");
+    src[i] = (int)(z * d.PX + x);
+    // the reference's indexed update keeps one of two equal cells' sums
+    // (acoustic_2d.py:237): such shots are refused, not guessed at
+    for (int j = i - i % pl.K; j < i; ++j)
+      if (src[j] == src[i])
+        return fail(GBW_ERR_INVALID, who + " repeats the cell of point " +
+                                         std::to_string(j % pl.K) + " of its shot");
   }
   host_rcell.assign(std::max(1, pl.R), 0);
   std::vector<int> rnext(std::max(1, pl.R), -1), heads;
@@ -3939,17 +3985,17 @@ void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
 // address, with the shape each was prepared for: a call handed one checks its
 // own shape against it before any kernel reads the buffer.
 struct GeoKey {
-  int NZ, NX, PX, nabc, fs, S, R;
+  int NZ, NX, PX, nabc, fs, S, R, K;
   bool operator==(const GeoKey& o) const {
     return NZ == o.NZ && NX == o.NX && PX == o.PX && nabc == o.nabc && fs == o.fs && S == o.S &&
-           R == o.R;
+           R == o.R && K == o.K;
   }
 };
 std::mutex g_geo_mu;
 std::unordered_map<const void*, GeoKey> g_geo;
 
 GeoKey geo_key(const Plan& pl) {
-  return GeoKey{pl.d.NZ, pl.d.NX, pl.d.PX, pl.d.nabc, pl.d.fs, pl.S, pl.R};
+  return GeoKey{pl.d.NZ, pl.d.NX, pl.d.PX, pl.d.nabc, pl.d.fs, pl.S, pl.R, pl.K};
 }
 
 // the geometry region of a call: the caller's prepared buffer, or the
@@ -3985,7 +4031,10 @@ FwdArgs base_fwd_args(const Plan& pl, const gbw_acoustic2d_problem* p,
   a.coef = at<float>(ws, pl.off_coef);
   a.src_cell = at<int>(g, pl.off_src);
   a.wav = in->src_v;
-  a.wav_stride = p->wavelet_shared ? 0 : p->nt;
+  // wavelet rows: shared (nt,), per shot (src_n, nt), per point (src_n, K, nt)
+  a.wav_stride = p->wavelet_shared == 1 ? 0 : (p->wavelet_shared == 2 ? (long long)pl.K : 1) * p->nt;
+  a.wav_kstride = p->wavelet_shared == 2 ? p->nt : 0;
+  a.src_k = pl.K;
   a.nt = p->nt; a.dt = p->dt;
   a.rhead = at<int>(g, pl.off_rhead);
   a.rnext = at<int>(g, pl.off_rnext);
@@ -4031,6 +4080,10 @@ int make_batch(const gbw_acoustic2d_problem* probs, int n, BatchLayout& L) {
     if (p.checkpoint_every != 0 || p.snapshot_every != 0)
       return fail(GBW_ERR_UNSUPPORTED,
                   "batched forward: checkpoints and snapshots are not supported (problem " +
+                      std::to_string(L.order[k]) + ")");
+    if (p.src_points > 1)
+      return fail(GBW_ERR_UNSUPPORTED,
+                  "batched forward: one source point per shot (problem " +
                       std::to_string(L.order[k]) + ")");
     if ((p.steps_per_launch == 1) != (L.single != 0))
       return fail(GBW_ERR_INVALID, "batched forward: steps_per_launch must agree across problems");
@@ -4114,6 +4167,8 @@ int make_grad_batch(const gbw_acoustic2d_problem* probs, int n, GradLayout& L) {
     const std::string where = "batched gradient, problem " + std::to_string(L.order[j]) + ": ";
     if (p.snapshot_every != 0 || p.checkpoint_bf16 != 0)
       return fail(GBW_ERR_UNSUPPORTED, where + "snapshots and bf16 checkpoints are not supported");
+    if (p.src_points > 1)
+      return fail(GBW_ERR_UNSUPPORTED, where + "one source point per shot");
     if (p.steps_per_launch == 1)
       return fail(GBW_ERR_UNSUPPORTED, where + "runs two time steps per launch (steps_per_launch 0 or 2)");
     if ((p.tile_rows == 24 ? 24 : 16) != L.TZ2)
@@ -4388,7 +4443,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   const char* ck = static_cast<const char*>(adj->checkpoints);
   if ((rc = fill(G1, (long long)pl.groups * d.plane, 0.f, st))) return rc;
   if (G2 && (rc = fill(G2, (long long)pl.groups * d.plane, 0.f, st))) return rc;
-  if ((rc = fill(GW, (long long)pl.S * pl.nt, 0.f, st))) return rc;
+  if ((rc = fill(GW, (long long)pl.S * pl.K * pl.nt, 0.f, st))) return rc;
   if ((rc = fill(LA, slot, 0.f, st))) return rc;
   const bool any_g = adj->g_p || adj->g_vx || adj->g_vz;
   if (any_g && pl.R > 0) {
@@ -4401,7 +4456,7 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
   FwdArgs f = base_fwd_args(pl, prob, in, workspace, geo);
   AdjArgs b{};
   b.d = d; b.S = pl.S; b.spb = pl.spb; b.tiles_x = pl.tiles_x; b.ntiles = f.ntiles;
-  b.coef = f.coef; b.src_cell = f.src_cell; b.nt = pl.nt; b.dt = prob->dt;
+  b.coef = f.coef; b.src_cell = f.src_cell; b.src_k = pl.K; b.nt = pl.nt; b.dt = prob->dt;
   b.rhead = f.rhead; b.rnext = f.rnext; b.rrow = f.rrow; b.R = pl.R;
   b.g_p = adj->g_p; b.g_vx = adj->g_vx; b.g_vz = adj->g_vz;
   b.gA1 = G1; b.gA2 = G2; b.gw = GW;
@@ -4474,9 +4529,16 @@ int gbw_acoustic2d_backward(const gbw_acoustic2d_problem* prob, const gbw_acoust
                                    adj->grad_rho);
     LAUNCH_CHECK();
   }
-  if (adj->grad_src_v) {
-    const int n = prob->wavelet_shared ? pl.nt : pl.S * pl.nt;
-    wavelet_grad_kernel<<<(n + 255) / 256, 256, 0, st>>>(GW, pl.S, pl.nt, prob->wavelet_shared,
+  if (adj->grad_src_v && pl.K > 1 && prob->wavelet_shared == 0) {
+    const int n = pl.S * pl.nt;
+    wavelet_grad_points_kernel<<<(n + 255) / 256, 256, 0, st>>>(GW, pl.S, pl.K, pl.nt,
+                                                                adj->grad_src_v);
+    LAUNCH_CHECK();
+  } else if (adj->grad_src_v) {
+    // shared: summed over shots; per shot or per point: the samples as they are
+    const int shared = prob->wavelet_shared == 1;
+    const int n = shared ? pl.nt : pl.S * pl.K * pl.nt;
+    wavelet_grad_kernel<<<(n + 255) / 256, 256, 0, st>>>(GW, pl.S * pl.K, pl.nt, shared,
                                                          adj->grad_src_v);
     LAUNCH_CHECK();
   }
@@ -4649,7 +4711,7 @@ int gbw_acoustic2d_batch_gradient_setup(const gbw_acoustic2d_problem* probs,
     g.vp = ins[o].vp; g.rho = ins[o].rho;
     g.grad_vp = adj.grad_vp; g.grad_w = adj.grad_src_v;
     g.dz = probs[o].dz;
-    g.shared = probs[o].wavelet_shared;
+    g.shared = probs[o].wavelet_shared == 1;
     g.fstride = (long long)pl.S * pl.d.plane;
     g.slot_bytes = (long long)pl.slot_bytes;
     g.off = L.off[j];

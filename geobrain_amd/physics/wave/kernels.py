@@ -160,7 +160,7 @@ _geometry_cache: "collections.OrderedDict[tuple, Tensor]" = collections.OrderedD
 
 def _geometry(prob: _native.Problem, sx, sz, rx, rz, device: torch.device) -> Tensor:
     key = (str(device), prob.nz, prob.nx, prob.nabc, prob.free_surface, prob.src_n, prob.rcv_n,
-           sx.tobytes(), sz.tobytes(), rx.tobytes(), rz.tobytes())
+           prob.src_points, sx.tobytes(), sz.tobytes(), rx.tobytes(), rz.tobytes())
     hit = _geometry_cache.get(key)
     if hit is not None:
         _geometry_cache.move_to_end(key)
@@ -277,7 +277,9 @@ _index_cache: Dict[int, tuple] = {}
 host_index_copies = 0
 
 
-def _host_index(t, name: str) -> np.ndarray:
+def _host_index(t, name: str, points: bool = False) -> np.ndarray:
+    """Host int64 copy of a position array (1-D; with ``points``, also the
+    (src_n, K) form of several source points per shot)."""
     if torch.is_tensor(t):
         if t.device.type != "cpu":
             stamp = (t._version, t.data_ptr(), tuple(t.shape), tuple(t.stride()), t.dtype)
@@ -291,9 +293,10 @@ def _host_index(t, name: str) -> np.ndarray:
             a = t.detach().numpy()
     else:
         a = np.asarray(t)
-    if a.ndim != 1:
+    if a.ndim != 1 and not (points and a.ndim == 2):
         raise NotImplementedError(
-            f"{name}: only one point per shot (1-D positions) is supported; got shape {a.shape}")
+            f"{name}: only one point per shot (1-D positions) is supported here; got shape "
+            f"{a.shape}")
     a = np.ascontiguousarray(a, dtype=np.int64)
     if torch.is_tensor(t) and t.device.type != "cpu":
         a.setflags(write=False)
@@ -307,11 +310,24 @@ def validate_problem(nx, nz, nt, nabc, src_n, rcv_n, sx, sz, rx, rz, vp, rho, da
     """Shape and length checks of one 2D problem, made on the host before any
     device pointer is handed to the library (a short wavelet or an undersized
     model would otherwise be read past its end on the device).  ``sx .. rz``
-    are the host int64 position arrays.  Returns ``wavelet_shared`` (1 for a
-    ``(nt,)`` wavelet, 0 for ``(src_n, nt)``).  Raises ``ValueError`` or
-    ``NotImplementedError``; ``where`` prefixes the message."""
+    are the host int64 position arrays; source positions of shape
+    ``(src_n, K)`` put K points in each shot (``acoustic_2d.py:234-237``).
+    Returns ``wavelet_shared`` (1 for a ``(nt,)`` wavelet, 0 for
+    ``(src_n, nt)``, 2 for ``(src_n, K, nt)``, one wavelet per point).  Raises
+    ``ValueError`` or ``NotImplementedError``; ``where`` prefixes the
+    message."""
     if sx.shape != sz.shape or sx.shape[0] != src_n:
         raise ValueError(f"{where}src_x/src_z must both have src_n={src_n} entries")
+    if sx.ndim == 2:
+        k = sx.shape[1]
+        if k < 1:
+            raise ValueError(f"{where}src_x/src_z of shape (src_n, K) need K >= 1")
+        if src_v.dim() == 2 and tuple(src_v.shape) == (src_n, nt):
+            return 0
+        if src_v.dim() == 3 and tuple(src_v.shape) == (src_n, k, nt):
+            return 2
+        raise ValueError(f"{where}with (src_n, K) source positions src_v must have shape "
+                         f"({src_n}, {nt}) or ({src_n}, {k}, {nt})")
     if rx.shape != rz.shape or rx.shape[0] != rcv_n:
         raise ValueError(f"{where}rcv_x/rcv_z must both have rcv_n={rcv_n} entries")
     if tuple(vp.shape) != (nz, nx) or tuple(rho.shape) != (nz, nx):
@@ -388,10 +404,11 @@ def acoustic_forward_kernel(
         _on_gpu(t, name, dev)
     if not torch.cuda.is_available():
         raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
-    sx, sz = _host_index(src_x, "src_x"), _host_index(src_z, "src_z")
+    sx, sz = _host_index(src_x, "src_x", True), _host_index(src_z, "src_z", True)
     rx, rz = _host_index(rcv_x, "rcv_x"), _host_index(rcv_z, "rcv_z")
     shared = validate_problem(nx, nz, nt, nabc, src_n, rcv_n, sx, sz, rx, rz, vp, rho, damp,
                               src_v)
+    points = sx.shape[1] if sx.ndim == 2 else 1
 
     prob = _native.Problem()
     prob.nx, prob.nz = int(nx), int(nz)
@@ -400,6 +417,7 @@ def acoustic_forward_kernel(
     prob.nabc, prob.free_surface = int(nabc), int(bool(free_surface))
     prob.src_n, prob.rcv_n = int(src_n), int(rcv_n)
     prob.wavelet_shared = shared
+    prob.src_points = int(points)
     if snapshot_every > 0:
         prob.energy_chunk = int(snapshot_every)
         prob.snapshot_every = int(snapshot_every)

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define GBW_ABI_VERSION 5
+#define GBW_ABI_VERSION 6
 
 typedef enum {
   GBW_OK = 0,
@@ -57,9 +57,10 @@ typedef struct gbw_acoustic2d_problem {
   float dt;                  /* time step (s)                                          */
   int32_t nabc;              /* absorbing layers per side (>= 1)                       */
   int32_t free_surface;      /* 0 / 1                                                  */
-  int32_t src_n;             /* shots (one point source each)                          */
+  int32_t src_n;             /* shots                                                  */
   int32_t rcv_n;             /* receivers shared by every shot                         */
   int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
+                             /* (every point of a shot); 2: (src_n, src_points, nt)     */
   int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
   int32_t snapshot_every;    /* >0: copy shot-0 pressure after every chunk of this size */
   int32_t checkpoint_every;  /* k>0: keep the state every k steps for the gradient     */
@@ -84,6 +85,11 @@ typedef struct gbw_acoustic2d_problem {
                              /* groups of 8 shots); 0 auto: 3 where the grid has more   */
                              /* than 256 two-step tiles and groups of more than 8       */
                              /* shots, else 1                                           */
+  int32_t src_points;        /* point sources per shot (0 or 1: one; K > 1: src_x and   */
+                             /* src_z are (src_n, K), distinct cells within a shot, as  */
+                             /* the reference's 2-D positions, acoustic_2d.py:234-237;  */
+                             /* such problems run one-step tiled launches, and the      */
+                             /* batched calls refuse them)                              */
 } gbw_acoustic2d_problem;
 
 /* Geometry and model (acoustic_2d.py:303-312).  Positions are unpadded model
@@ -92,11 +98,11 @@ typedef struct gbw_acoustic2d_problem {
  * read: the call uses the prepared source cells and receiver maps and does no
  * host work, copy or synchronisation, so it can be captured in a HIP graph. */
 typedef struct gbw_acoustic2d_inputs {
-  const int64_t* src_x;      /* HOST (src_n,)                                          */
-  const int64_t* src_z;      /* HOST (src_n,)                                          */
+  const int64_t* src_x;      /* HOST (src_n,), or (src_n, src_points) row-major        */
+  const int64_t* src_z;      /* HOST (src_n,), or (src_n, src_points) row-major        */
   const int64_t* rcv_x;      /* HOST (rcv_n,)                                          */
   const int64_t* rcv_z;      /* HOST (rcv_n,)                                          */
-  const float* src_v;        /* DEVICE (src_n, nt) or (nt,)                            */
+  const float* src_v;        /* DEVICE (src_n, nt), (nt,) or (src_n, src_points, nt)   */
   const float* damp;         /* DEVICE (nz + 2 nabc, nx + 2 nabc)                      */
   const float* vp;           /* DEVICE (nz, nx)                                        */
   const float* rho;          /* DEVICE (nz, nx)                                        */

@@ -68,6 +68,17 @@ def _load():
             fn = getattr(lib, name)
             fn.argtypes = grad_args
             fn.restype = I
+        # several source points per shot: (..., nt, S, K, per_point, positions, ...)
+        mp_fwd = [I, I, I, I, D, D, I, I, I, I, P, P, P, I, P, P, P, P, P, I,
+                  P, P, P, P, P, P]
+        mp_grad = [I, I, I, I, D, D, I, I, I, I, P, P, P, I, P, P, P, P, P, I,
+                   P, P, P, P, P, P]
+        for name, args in (("oracle_forward_mp_f32", mp_fwd), ("oracle_forward_mp_f64", mp_fwd),
+                           ("oracle_gradient_mp_f32", mp_grad),
+                           ("oracle_gradient_mp_f64", mp_grad)):
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = I
         fwd3_args = [I, I, I, I, I, D, D, I, I, P, P, P, P, I, P, P, P, P, P, P, I,
                      P, P, P, P, P, P, P, P, P]
         for name in ("oracle3_forward_f32", "oracle3_forward_f64"):
@@ -112,6 +123,23 @@ def _common(nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp):
     return src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp
 
 
+def _multi_point(src_z, src_x, src_v):
+    """Several source points per shot (acoustic_2d.py:234-237): positions of
+    shape (S, K) with K > 1, or a per-point wavelet (S, K, nt).  Returns
+    (K, per_point, flat positions, wavelet rows) or None for one point."""
+    sz, sx = np.asarray(src_z), np.asarray(src_x)
+    w = np.asarray(src_v, dtype=np.float32)
+    if sz.ndim != 2 or (sz.shape[1] == 1 and w.ndim < 3):
+        return None
+    assert sx.shape == sz.shape and w.ndim in (2, 3), "positions (S, K), src_v (S, nt) or (S, K, nt)"
+    S, K = sz.shape
+    per_point = int(w.ndim == 3)
+    assert w.shape[:1 + per_point] == ((S, K) if per_point else (S,))
+    rows = np.ascontiguousarray(w.reshape(-1, w.shape[-1]))
+    return (K, per_point, np.ascontiguousarray(sz, np.int64).reshape(-1),
+            np.ascontiguousarray(sx, np.int64).reshape(-1), rows)
+
+
 def forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
             vp, rho, damp, chunk=0, snapshot_every=0, precision="f64",
             return_state=False):
@@ -120,7 +148,14 @@ def forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_
     ``chunk`` is the time-chunk length (``chunk_len(nt, segments)``); it only
     changes how the energy maps are summed.  ``snapshot_every > 0`` selects the
     reference's snapshot mode (fixed chunks, shot-0 pressure after each).
+    Positions of shape (S, K) put K point sources in each shot (src_v (S, nt):
+    one wavelet per shot, (S, K, nt): one per point).
     """
+    mp = _multi_point(src_z, src_x, src_v)
+    if mp is not None:
+        assert snapshot_every == 0 and not return_state
+        return _forward_mp(nz, nx, nabc, free_surface, dt, dz, mp, rcv_z, rcv_x, vp, rho,
+                           damp, chunk, precision)
     src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp = _common(
         nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp)
     S, nt = src_v.shape
@@ -152,6 +187,50 @@ def forward(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_
     return out
 
 
+def _forward_mp(nz, nx, nabc, free_surface, dt, dz, mp, rcv_z, rcv_x, vp, rho, damp, chunk,
+                precision):
+    K, per_point, sz, sx, rows = mp
+    S, nt = sz.shape[0] // K, rows.shape[1]
+    _, _, _, rcv_z, rcv_x, vp, rho, damp = _common(
+        nz, nx, nabc, sz, sx, rows, rcv_z, rcv_x, vp, rho, damp)
+    R = rcv_z.shape[0]
+    dtp = np.float32 if precision == "f32" else np.float64
+    out = {k: np.zeros((S, nt, R), dtp) for k in ("p", "vx", "vz")}
+    for k in ("forward_wavefield_p", "forward_wavefield_vx", "forward_wavefield_vz"):
+        out[k] = np.zeros((nz, nx), dtp)
+    rc = getattr(_load(), "oracle_forward_mp_" + precision)(
+        nz, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S, K, per_point,
+        _ptr(sz), _ptr(sx), _ptr(rows), R, _ptr(rcv_z), _ptr(rcv_x), _ptr(vp), _ptr(rho),
+        _ptr(damp), int(chunk), _ptr(out["p"]), _ptr(out["vx"]), _ptr(out["vz"]),
+        _ptr(out["forward_wavefield_p"]), _ptr(out["forward_wavefield_vx"]),
+        _ptr(out["forward_wavefield_vz"]))
+    if rc != 0:
+        raise ValueError(f"oracle forward rejected its inputs (code {rc})")
+    return out
+
+
+def _gradient_mp(nz, nx, nabc, free_surface, dt, dz, mp, rcv_z, rcv_x, vp, rho, damp, g_p,
+                 g_vx, g_vz, chunk, precision):
+    K, per_point, sz, sx, rows = mp
+    S, nt = sz.shape[0] // K, rows.shape[1]
+    _, _, _, rcv_z, rcv_x, vp, rho, damp = _common(
+        nz, nx, nabc, sz, sx, rows, rcv_z, rcv_x, vp, rho, damp)
+    R = rcv_z.shape[0]
+    dtp = np.float32 if precision == "f32" else np.float64
+    conv = lambda g: None if g is None else np.ascontiguousarray(g, dtype=dtp)  # noqa: E731
+    g_p, g_vx, g_vz = conv(g_p), conv(g_vx), conv(g_vz)
+    gvp, grho = np.zeros((nz, nx), dtp), np.zeros((nz, nx), dtp)
+    gw = np.zeros((S, K, nt) if per_point else (S, nt), dtp)
+    rc = getattr(_load(), "oracle_gradient_mp_" + precision)(
+        nz, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S, K, per_point,
+        _ptr(sz), _ptr(sx), _ptr(rows), R, _ptr(rcv_z), _ptr(rcv_x), _ptr(vp), _ptr(rho),
+        _ptr(damp), int(chunk), _ptr(g_p), _ptr(g_vx), _ptr(g_vz), _ptr(gvp), _ptr(grho),
+        _ptr(gw))
+    if rc != 0:
+        raise ValueError(f"oracle gradient rejected its inputs (code {rc})")
+    return gvp, grho, gw
+
+
 def gradient(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv_x,
              vp, rho, damp, g_p=None, g_vx=None, g_vz=None, chunk=0,
              precision="f64", wavelet_shared=False, checkpoint_bf16=False):
@@ -162,8 +241,14 @@ def gradient(nz, nx, nabc, free_surface, dt, dz, src_z, src_x, src_v, rcv_z, rcv
     ``grad_rho`` is the gradient for a density that is an independent input
     (the reference's ``rho_grad=True`` case).  ``checkpoint_bf16`` rounds every
     checkpoint to bf16 with a per-(field, shot) power-of-two scale before the
-    recompute, as the GPU library's bf16 checkpoint storage does.
+    recompute, as the GPU library's bf16 checkpoint storage does.  Positions
+    of shape (S, K) put K point sources in each shot, as in ``forward``.
     """
+    mp = _multi_point(src_z, src_x, src_v)
+    if mp is not None:
+        assert not wavelet_shared and not checkpoint_bf16
+        return _gradient_mp(nz, nx, nabc, free_surface, dt, dz, mp, rcv_z, rcv_x, vp, rho,
+                            damp, g_p, g_vx, g_vz, chunk, precision)
     src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp = _common(
         nz, nx, nabc, src_z, src_x, src_v, rcv_z, rcv_x, vp, rho, damp)
     S, nt = src_v.shape

@@ -110,8 +110,11 @@ struct State {
 template <typename T>
 struct Geometry {
   int S, R, nt;
-  std::vector<int> sz, sx, rz, rx;  // padded indices
-  const float* w;                   // S * nt wavelet rows
+  int K = 1;                        // source points per shot
+  int per_point = 0;                // 1: a wavelet row per point (S*K rows), else per shot
+  std::vector<int> sz, sx, rz, rx;  // padded indices (sources: S*K, point k of shot s at s*K+k)
+  const float* w;                   // wavelet rows
+  int wrow(int s, int k) const { return per_point ? s * K + k : s; }
 };
 
 template <typename T>
@@ -172,10 +175,13 @@ static void step_forward(const Grid<T>& g, const Coefs<T>& k, const Geometry<T>&
   }
   for (int s = 0; s < S; ++s) {
     T* p = st.p.data() + s * plane;
-    // source (:228-233): p[s, sz, sx] = p[s, sz, sx] + dt * w[s, it]
-    const T upd = g.dt * (T)geo.w[(size_t)s * geo.nt + it];
-    const size_t q = g.at(geo.sz[s], geo.sx[s]);
-    p[q] = p[q] + upd;
+    // source (:228-237): p[s, sz, sx] = p[s, sz, sx] + dt * w[s, it] at each
+    // point of the shot (distinct cells: every point reads the updated field)
+    for (int k = 0; k < geo.K; ++k) {
+      const T upd = g.dt * (T)geo.w[(size_t)geo.wrow(s, k) * geo.nt + it];
+      const size_t q = g.at(geo.sz[s * geo.K + k], geo.sx[s * geo.K + k]);
+      p[q] = p[q] + upd;
+    }
     // free surface for p (:239-241)
     if (g.fs)
       for (int j = 0; j < g.NX; ++j) p[g.at(g.f - 1, j)] = -p[g.at(g.f + 1, j)];
@@ -242,15 +248,16 @@ static void energy(const Grid<T>& g, int S, const std::vector<T>& fld, T* acc) {
 template <typename T>
 static int setup(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
                  const int64_t* src_z, const int64_t* src_x, const float* src_v, int R,
-                 const int64_t* rcv_z, const int64_t* rcv_x, Grid<T>& g, Geometry<T>& geo) {
-  if (nz <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0) return -1;
+                 const int64_t* rcv_z, const int64_t* rcv_x, Grid<T>& g, Geometry<T>& geo,
+                 int K = 1, int per_point = 0) {
+  if (nz <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0 || K <= 0) return -1;
   g.nz = nz; g.nx = nx; g.nabc = nabc; g.fs = fs ? 1 : 0;
   g.NZ = nz + 2 * nabc; g.NX = nx + 2 * nabc;
   g.f = fs ? nabc : 1;
   g.dt = to_T_dt<T>(dt); g.dz = to_T_dt<T>(dz);
-  geo.S = S; geo.R = R; geo.nt = nt; geo.w = src_v;
-  geo.sz.resize(S); geo.sx.resize(S); geo.rz.resize(R); geo.rx.resize(R);
-  for (int s = 0; s < S; ++s) {
+  geo.S = S; geo.R = R; geo.nt = nt; geo.w = src_v; geo.K = K; geo.per_point = per_point;
+  geo.sz.resize((size_t)S * K); geo.sx.resize((size_t)S * K); geo.rz.resize(R); geo.rx.resize(R);
+  for (int s = 0; s < S * K; ++s) {
     geo.sz[s] = (int)src_z[s] + nabc; geo.sx[s] = (int)src_x[s] + nabc;
     if (geo.sz[s] < 0 || geo.sz[s] >= g.NZ || geo.sx[s] < 0 || geo.sx[s] >= g.NX) return -2;
   }
@@ -275,9 +282,11 @@ static int forward_impl(int nz, int nx, int nabc, int fs, double dt, double dz, 
                         const int64_t* src_z, const int64_t* src_x, const float* src_v, int R,
                         const int64_t* rcv_z, const int64_t* rcv_x, const float* vp,
                         const float* rho, const float* damp, int chunk_len, T* rp, T* rvx,
-                        T* rvz, T* ep, T* evx, T* evz, T* snapshots, T* final_state) {
+                        T* rvz, T* ep, T* evx, T* evz, T* snapshots, T* final_state,
+                        int K = 1, int per_point = 0) {
   Grid<T> g; Geometry<T> geo;
-  int rc = setup<T>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x, g, geo);
+  int rc = setup<T>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x, g, geo,
+                    K, per_point);
   if (rc) return rc;
   Coefs<T> k; build_coefs(g, vp, rho, damp, k);
   State<T> st; st.init((size_t)S * g.cells());
@@ -385,11 +394,12 @@ static void step_adjoint(const Grid<T>& g, const Coefs<T>& k, const Geometry<T>&
         lp[g.at(f - 1, j)] = 0;
       }
     // 2^T: wavelet gradient
-    if (gw) {
-      const T v = g.dt * lp[g.at(geo.sz[s], geo.sx[s])];
-      if (wavelet_shared) gw[it] += v;
-      else gw[(size_t)s * geo.nt + it] += v;
-    }
+    if (gw)
+      for (int k = 0; k < geo.K; ++k) {
+        const T v = g.dt * lp[g.at(geo.sz[s * geo.K + k], geo.sx[s * geo.K + k])];
+        if (wavelet_shared) gw[it] += v;
+        else gw[(size_t)geo.wrow(s, k) * geo.nt + it] += v;
+      }
     // 1^T: r = A1 * lambda_p' on the p region; imaging condition
     T* r = qz;  // reuse
     std::fill(r, r + plane, (T)0);
@@ -458,9 +468,10 @@ static int gradient_impl(int nz, int nx, int nabc, int fs, double dt, double dz,
                          int wavelet_shared, int R, const int64_t* rcv_z, const int64_t* rcv_x,
                          const float* vp, const float* rho, const float* damp, int chunk_len,
                          const T* g_p, const T* g_vx, const T* g_vz, T* grad_vp, T* grad_rho,
-                         T* grad_w, int ckpt_bf16) {
+                         T* grad_w, int ckpt_bf16, int K = 1, int per_point = 0) {
   Grid<T> g; Geometry<T> geo;
-  int rc = setup<T>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x, g, geo);
+  int rc = setup<T>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x, g, geo,
+                    K, per_point);
   if (rc) return rc;
   Coefs<T> k; build_coefs(g, vp, rho, damp, k);
   const size_t plane = g.cells();
@@ -483,7 +494,9 @@ static int gradient_impl(int nz, int nx, int nabc, int fs, double dt, double dz,
   // adjoint sweep
   State<T> lam; lam.init(n);
   std::vector<T> gA1(plane, (T)0), gA2(plane, (T)0), scratch(2 * plane);
-  if (grad_w) std::fill(grad_w, grad_w + (wavelet_shared ? (size_t)nt : (size_t)S * nt), (T)0);
+  if (grad_w)
+    std::fill(grad_w, grad_w + (wavelet_shared ? (size_t)nt : (size_t)S * (per_point ? K : 1) * nt),
+              (T)0);
   for (int c = nseg - 1; c >= 0; --c) {
     const int t0 = bounds[c], t1 = bounds[c + 1];
     std::vector<State<T>> seg(t1 - t0 + 1);
@@ -570,6 +583,51 @@ int oracle_gradient_f64(int nz, int nx, int nabc, int fs, double dt, double dz, 
   return gradient_impl<double>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v,
                                wavelet_shared, R, rcv_z, rcv_x, vp, rho, damp, chunk_len, g_p,
                                g_vx, g_vz, grad_vp, grad_rho, grad_w, ckpt_bf16);
+}
+
+// several source points per shot: src_z / src_x hold S*K points (point k of
+// shot s at s*K+k), src_v a row per shot (per_point 0) or per point (1)
+int oracle_forward_mp_f32(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
+                          int K, int per_point, const int64_t* src_z, const int64_t* src_x,
+                          const float* src_v, int R, const int64_t* rcv_z, const int64_t* rcv_x,
+                          const float* vp, const float* rho, const float* damp, int chunk_len,
+                          float* rp, float* rvx, float* rvz, float* ep, float* evx, float* evz) {
+  return forward_impl<float>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z, rcv_x,
+                             vp, rho, damp, chunk_len, rp, rvx, rvz, ep, evx, evz, nullptr,
+                             nullptr, K, per_point);
+}
+
+int oracle_forward_mp_f64(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
+                          int K, int per_point, const int64_t* src_z, const int64_t* src_x,
+                          const float* src_v, int R, const int64_t* rcv_z, const int64_t* rcv_x,
+                          const float* vp, const float* rho, const float* damp, int chunk_len,
+                          double* rp, double* rvx, double* rvz, double* ep, double* evx,
+                          double* evz) {
+  return forward_impl<double>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, R, rcv_z,
+                              rcv_x, vp, rho, damp, chunk_len, rp, rvx, rvz, ep, evx, evz,
+                              nullptr, nullptr, K, per_point);
+}
+
+int oracle_gradient_mp_f32(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
+                           int K, int per_point, const int64_t* src_z, const int64_t* src_x,
+                           const float* src_v, int R, const int64_t* rcv_z, const int64_t* rcv_x,
+                           const float* vp, const float* rho, const float* damp, int chunk_len,
+                           const float* g_p, const float* g_vx, const float* g_vz, float* grad_vp,
+                           float* grad_rho, float* grad_w) {
+  return gradient_impl<float>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, 0, R, rcv_z,
+                              rcv_x, vp, rho, damp, chunk_len, g_p, g_vx, g_vz, grad_vp, grad_rho,
+                              grad_w, 0, K, per_point);
+}
+
+int oracle_gradient_mp_f64(int nz, int nx, int nabc, int fs, double dt, double dz, int nt, int S,
+                           int K, int per_point, const int64_t* src_z, const int64_t* src_x,
+                           const float* src_v, int R, const int64_t* rcv_z, const int64_t* rcv_x,
+                           const float* vp, const float* rho, const float* damp, int chunk_len,
+                           const double* g_p, const double* g_vx, const double* g_vz,
+                           double* grad_vp, double* grad_rho, double* grad_w) {
+  return gradient_impl<double>(nz, nx, nabc, fs, dt, dz, nt, S, src_z, src_x, src_v, 0, R, rcv_z,
+                               rcv_x, vp, rho, damp, chunk_len, g_p, g_vx, g_vz, grad_vp,
+                               grad_rho, grad_w, 0, K, per_point);
 }
 
 }  // extern "C"

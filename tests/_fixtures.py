@@ -9,6 +9,9 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 FWD_CASES = ["tiny0_fwd", "tiny1_fwd", "tiny2_fwd", "tiny3_fwd", "tiny_snap", "c1_fwd"]
 GRAD_CASES = ["tiny0_grad", "tiny1_grad", "tiny2_grad", "tiny3_grad", "c1_grad", "c2r_grad"]
+# several source points per shot ((S, K) positions): per-shot and per-point wavelets
+MP_FWD_CASES = ["mp0_fwd", "mp1_fwd"]
+MP_GRAD_CASES = ["mp0_grad", "mp1_grad"]
 OUT_KEYS = ("p", "vx", "vz", "forward_wavefield_p", "forward_wavefield_vx",
             "forward_wavefield_vz", "snapshots")
 

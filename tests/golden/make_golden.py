@@ -335,6 +335,38 @@ def tiny_cases():
     return cases
 
 
+def mp_cases():
+    """Several point sources per shot (2-D ``src_z`` / ``src_x`` of shape
+    (S, K), acoustic_2d.py:229-237): a per-shot wavelet (S, nt) on all points
+    of a shot, and a per-point wavelet (S, K, nt) with a free surface, points
+    on the surface row and on the mirrored row above it.  Points are distinct
+    within a shot."""
+    rng = np.random.default_rng(2468)
+    cases = []
+    for idx, (nz, nx, nabc, fs, S, K, nt, segs, per_point) in enumerate([
+        (9, 14, 4, False, 2, 3, 40, 3, False),
+        (10, 12, 3, True, 2, 2, 45, 4, True),
+    ]):
+        vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
+        rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+        src_z = np.stack([rng.permutation(nz)[:K] for _ in range(S)])
+        src_x = rng.integers(0, nx, (S, K))
+        if fs:
+            src_z[0, 0], src_z[1, 1] = 0, -1
+        R = 6
+        rcv_x = rng.integers(0, nx, R)
+        rcv_z = rng.integers(0, nz, R)
+        shape = (S, K, nt) if per_point else (S, nt)
+        w = rng.standard_normal(shape).astype(np.float32)
+        base = dict(nz=nz, nx=nx, nabc=nabc, fs=fs, dx=10.0, dt=1e-3, nt=nt, vp=vp, rho=rho,
+                    src_z=src_z, src_x=src_x, wavelet=w, rcv_z=rcv_z, rcv_x=rcv_x,
+                    segments=segs)
+        cases.append((f"mp{idx}_fwd", base))
+        cases.append((f"mp{idx}_grad", dict(base, grad="random", g_seed=300 + idx,
+                                            rho_grad=True, w_grad=True)))
+    return cases
+
+
 def c1_case(grad=False):
     # 100 x 200, dx 10 m, dt 1 ms, 1000 steps, 5 layers from seed 0, 10 Hz Ricker
     nz, nx, nt, dx, dt = 100, 200, 1000, 10.0, 1e-3
@@ -425,8 +457,8 @@ def main():
     ap.add_argument("--threads", type=int, default=8)
     args = ap.parse_args()
     torch.set_num_threads(args.threads)
-    cases = tiny_cases() + [c1_case(False), c1_case(True),
-                            c2_case(120, 400, 1200, "c2r_grad")]
+    cases = tiny_cases() + mp_cases() + [c1_case(False), c1_case(True),
+                                         c2_case(120, 400, 1200, "c2r_grad")]
     if args.only and "c2_full" in args.only:
         cases.append(c2_case(350, 1700, 4000, "c2_full_grad", lean=True))
     # PML arrays for several shapes (free-surface variants included)

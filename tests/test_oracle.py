@@ -9,11 +9,11 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._fixtures import (FWD_CASES, GRAD_CASES, OUT_KEYS, adjoint_sources, chunk_len,
-                             fd_c1, load, oracle_args, rel_l2)
+from tests._fixtures import (FWD_CASES, GRAD_CASES, MP_FWD_CASES, MP_GRAD_CASES, OUT_KEYS,
+                             adjoint_sources, chunk_len, fd_c1, load, oracle_args, rel_l2)
 
 
-@pytest.mark.parametrize("name", FWD_CASES + GRAD_CASES)
+@pytest.mark.parametrize("name", FWD_CASES + GRAD_CASES + MP_FWD_CASES + MP_GRAD_CASES)
 def test_oracle_fp32_forward_is_bitwise_reference(name):
     z, m = load(name)
     out = oracle.forward(**oracle_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
@@ -26,7 +26,7 @@ def test_oracle_fp32_forward_is_bitwise_reference(name):
     assert checked >= 3
 
 
-@pytest.mark.parametrize("name", FWD_CASES)
+@pytest.mark.parametrize("name", FWD_CASES + MP_FWD_CASES)
 def test_oracle_fp64_forward_close_to_reference(name):
     z, m = load(name)
     out = oracle.forward(**oracle_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
@@ -36,7 +36,7 @@ def test_oracle_fp64_forward_close_to_reference(name):
             assert rel_l2(out[k], z["out_" + k]) < 1e-5, k
 
 
-@pytest.mark.parametrize("name", GRAD_CASES)
+@pytest.mark.parametrize("name", GRAD_CASES + MP_GRAD_CASES)
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_oracle_gradient_matches_reference_autograd(name, precision):
     z, m = load(name)
@@ -130,6 +130,41 @@ def test_oracle_gradient_finite_differences_config1(name):
     assert scale > 0
     err = np.abs(fd - g[ii, jj]) / scale
     assert err.max() <= 1e-8, (err.max(), list(zip(ii, jj, fd, g[ii, jj])))
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_oracle_multipoint_superposition(fs):
+    """K point sources in one shot against K one-point shots (FP64): the
+    seismograms add (the scheme is linear in the source terms), the energy
+    maps do not (they square the summed field), and the per-point wavelet
+    gradient equals the one-point shots' wavelet gradients for the same
+    adjoint sources.  Distinct cells; a point on the surface row with a free
+    surface."""
+    rng = np.random.default_rng(17 + fs)
+    nz, nx, nabc, nt, K, R = 11, 13, 4, 50, 3, 5
+    vp = rng.uniform(1500, 3000, (nz, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, nx)).astype(np.float32)
+    damp = oracle.bc_pml(nx, nz, 10.0, 10.0, nabc, float(vp.max()), False).astype(np.float32)
+    sz, sx = np.array([[0, 4, 7]]), np.array([[2, 9, 5]])
+    w = rng.standard_normal((1, K, nt)).astype(np.float32)
+    rz, rx = rng.integers(0, nz, R), rng.integers(0, nx, R)
+    common = dict(nz=nz, nx=nx, nabc=nabc, free_surface=fs, dt=1e-3, dz=10.0, rcv_z=rz, rcv_x=rx,
+                  vp=vp, rho=rho, damp=damp)
+    multi = oracle.forward(**common, src_z=sz, src_x=sx, src_v=w, precision="f64")
+    single = oracle.forward(**common, src_z=sz[0], src_x=sx[0], src_v=w[0], precision="f64")
+    for k in ("p", "vx", "vz"):
+        np.testing.assert_allclose(multi[k][0], single[k].sum(axis=0), rtol=0,
+                                   atol=1e-12 * np.abs(single[k]).max())
+    assert rel_l2(multi["forward_wavefield_p"], single["forward_wavefield_p"]) > 1e-3
+    g = rng.standard_normal((1, nt, R))
+    _, _, gw = oracle.gradient(**common, src_z=sz, src_x=sx, src_v=w, g_p=g, precision="f64")
+    _, _, gw1 = oracle.gradient(**common, src_z=sz[0], src_x=sx[0], src_v=w[0],
+                                g_p=np.repeat(g, K, axis=0), precision="f64")
+    np.testing.assert_allclose(gw[0], gw1, rtol=1e-12, atol=1e-18)
+    # one wavelet for the shot: its gradient is the sum over the points
+    _, _, gws = oracle.gradient(**common, src_z=sz, src_x=sx, src_v=w[:, 0], g_p=g,
+                                precision="f64")
+    np.testing.assert_allclose(gws[0], gw1.sum(axis=0), rtol=1e-12, atol=1e-18)
 
 
 def test_oracle_linear_and_zero_wavelet():
