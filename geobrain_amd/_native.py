@@ -93,7 +93,7 @@ class Problem3(ctypes.Structure):
         ("dz", c_f32), ("nt", c_i32), ("dt", c_f32), ("nabc", c_i32), ("free_surface", c_i32),
         ("src_n", c_i32), ("rcv_n", c_i32), ("wavelet_shared", c_i32), ("energy_chunk", c_i32),
         ("shots_per_block", c_i32), ("checkpoint_every", c_i32), ("step_kernel", c_i32),
-        ("z_chunk", c_i32),
+        ("z_chunk", c_i32), ("src_points", c_i32),
     ]
 
 

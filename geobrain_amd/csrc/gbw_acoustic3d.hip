@@ -145,17 +145,23 @@ struct Step3Args {
   float* state;            // [field][shot][plane]
   long long fstride;       // S * plane
   const float* coef;
-  const int* src_cell;     // [S] padded cell index in the plane
+  const int* src_cell;     // [S * src_k] padded cell index in the plane of each point
   const float* wav; long long wav_stride;
   float* energy;           // [group][4][nz][ny][nx] or null
   float* dstore;           // recompute: [shot][plane] p-update bracket of this step, or null
   float* pstore;           // recompute: [shot][plane] p after this step's update, or null
+  long long wav_kstride;   // wavelet row of point k of a shot: + k * wav_kstride
+  int src_k;               // source points per shot (distinct cells)
 };
 
 // p on z in [f+1, NZ-3], y in [2, NY-3], x in [2, NX-3]; then the source.
 // In-plane offsets are 32-bit (a plane of one field of one shot is < 2^28
 // float4), so loads use a uniform base and a 32-bit lane offset.
+// MP: several source points per shot (src_k > 1); the one-point instance
+// compiles with src_k fixed at 1
+template <bool MP>
 __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
+  const int K = MP ? a.src_k : 1;
   const Dims3 d = a.d;
   int g, y, z;
   unsigned q4;  // float4 group of the plane
@@ -179,9 +185,14 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
     const float4* __restrict__ VX = reinterpret_cast<const float4*>(a.state + a.fstride + so);
     const float4* __restrict__ VY = reinterpret_cast<const float4*>(a.state + 2 * a.fstride + so);
     const float4* __restrict__ VZ = reinterpret_cast<const float4*>(a.state + 3 * a.fstride + so);
-    const int sc = a.src_cell[s];
+    const int sc = a.src_cell[s * K];
     const bool src_here = (unsigned)(sc >> 2) == q4;
-    if (!mask && !src_here && !store) continue;
+    // further points of the shot (acoustic_3d.py:171-176) in this group
+    bool more_here = false;
+    if constexpr (MP)
+      for (int k = 1; k < K; ++k)
+        more_here = more_here || (unsigned)(a.src_cell[s * K + k] >> 2) == q4;
+    if (!mask && !src_here && !more_here && !store) continue;
     float4 p = ld4u(P, q4);
     float4 dsum = make_float4(0.f, 0.f, 0.f, 0.f);
     if (mask) {
@@ -218,6 +229,13 @@ __global__ __launch_bounds__(T3) void p3_step_kernel(Step3Args a) {
       const int e = sc & 3;
       s3(p, e, g3(p, e) + a.dt * a.wav[(long long)s * a.wav_stride + a.it]);
     }
+    if (more_here)  // distinct cells: each adds to its own
+      for (int k = 1; k < K; ++k) {
+        const int sk = a.src_cell[s * K + k];
+        if ((unsigned)(sk >> 2) != q4) continue;
+        const int e = sk & 3;
+        s3(p, e, g3(p, e) + a.dt * a.wav[(long long)s * a.wav_stride + k * a.wav_kstride + a.it]);
+      }
     st4u(P, q4, p);
     if (store) {
       st4u(reinterpret_cast<float4*>(a.dstore + so), q4, dsum);
@@ -408,6 +426,7 @@ struct Adj3Args {
   const int* urcv;         // receiver numbers
   int R;
   const float* g_p; const float* g_vx; const float* g_vy; const float* g_vz;  // (S, nt, R) or null
+  int src_k;               // source points per shot (gw row s * src_k + k)
 };
 
 // float4 array addressed by a 32-bit element index (byte offset i * 16 < 2^32):
@@ -690,7 +709,11 @@ __device__ __forceinline__ float4 gather_q(const Dims3& d, const QStencil& t,
   return out;
 }
 
+// MP: several source points per shot (src_k > 1); the one-point instance
+// compiles with src_k fixed at 1
+template <bool MP>
 __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
+  const int K = MP ? a.src_k : 1;
   const Dims3 d = a.d;
   Cell3 c;
   if (!cell3(d, c)) return;
@@ -715,7 +738,7 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
   float4 acc1 = make_float4(0.f, 0.f, 0.f, 0.f);
   // source cells one shot ahead: a load at the top of the shot would be
   // waited for before anything else of the shot is issued
-  int sc_next = a.src_cell[s_begin];
+  int sc_next = a.src_cell[s_begin * K];
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
     const int sc = sc_next;
@@ -724,15 +747,15 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
       // so waited for) as soon as it is loaded
       int sn = min(s + 1, s_end - 1);
       asm volatile("" : "+v"(sn));
-      sc_next = a.src_cell[sn];
+      sc_next = a.src_cell[sn * K];
     }
     const bool src_here = (unsigned)(sc >> 2) == q4;
     if (mirrored) {  // the source term on the mirrored plane is overwritten: no gradient
       if (src_here) {
         if (a.gws) a.gws[s] = 0.f;
-        if (a.gw) a.gw[(long long)s * a.nt + a.it] = 0.f;
+        if (a.gw) a.gw[(long long)s * K * a.nt + a.it] = 0.f;
       }
-      continue;
+      continue;  // further points here keep their zeroed samples
     }
     const A4<float4> LP{reinterpret_cast<float4*>(a.lam + so)};
     const A4<const float4> LX{reinterpret_cast<const float4*>(a.lam + a.fstride + so)};
@@ -762,7 +785,15 @@ __global__ __launch_bounds__(T3) void adj3_p_kernel(Adj3Args a) {
     if (src_here) {
       const float v = a.dt * g3(lp, sc & 3);
       if (a.gws) a.gws[s] = v;
-      if (a.gw) a.gw[(long long)s * a.nt + a.it] = v;
+      if (a.gw) a.gw[(long long)s * K * a.nt + a.it] = v;
+    }
+    if constexpr (MP) {  // further points: a row of samples each
+      if (a.gw)
+        for (int k = 1; k < K; ++k) {
+          const int sk = a.src_cell[s * K + k];
+          if ((unsigned)(sk >> 2) == q4)
+            a.gw[((long long)s * K + k) * a.nt + a.it] = a.dt * g3(lp, sk & 3);
+        }
     }
     if (mp) {
       const float4 D = ld4u(reinterpret_cast<const float4*>(a.dstep + so), q4);
@@ -1404,6 +1435,18 @@ __global__ __launch_bounds__(F3_NW * 64) void fwd3f_kernel(Fused3Args a) {
 }
 
 // shared wavelet: grad[it] = sum over shots of the shot's source term, in shot order
+// several points per shot, one wavelet per shot: the K points' samples of
+// each (shot, step) summed in point order
+__global__ void wavelet_points3_kernel(const float* __restrict__ gwp, int S, int K, int nt,
+                                       float* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)S * nt) return;
+  const long long s = t / nt, i = t - s * nt;
+  float v = gwp[s * K * nt + i];
+  for (int k = 1; k < K; ++k) v = v + gwp[(s * K + k) * nt + i];
+  out[t] = v;
+}
+
 __global__ void wavelet_sum3_kernel(const float* gws, int S, float* gw, int it) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   float v = 0.f;
@@ -1458,6 +1501,7 @@ inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct Plan3 {
   Dims3 d;
   int S, R, nt, spb, groups, echunk, k, nseg;
+  int K;                            // source points per shot
   bool march;                       // fused z-marching step (state in and out of place)
   bool fused;                       // fused step with rows in wavefronts (state out of place)
   int tiles_x, tiles_y, zc;         // z-marching decomposition
@@ -1465,7 +1509,7 @@ struct Plan3 {
   int f_strips, f_ytiles;           // fused step: 61 x 13 windows
   size_t off_coef, off_state, off_E, off_src, off_rcell;
   size_t off_lam, off_r, off_D, off_P1, off_gA1, off_gA2, off_gws, off_ucell, off_ustart,
-      off_urcv, off_rowstart;
+      off_urcv, off_rowstart, off_gwp;
   size_t total;
 };
 
@@ -1498,11 +1542,20 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   pl.S = p->src_n; pl.R = p->rcv_n; pl.nt = p->nt;
   if (p->step_kernel < 0 || p->step_kernel > 2)
     return fail(GBW_ERR_INVALID, "step_kernel must be 0, 1 or 2");
+  if (p->src_points < 0) return fail(GBW_ERR_INVALID, "src_points must be >= 0");
+  pl.K = std::max(1, p->src_points);
+  if (p->wavelet_shared < 0 || p->wavelet_shared > 2)
+    return fail(GBW_ERR_INVALID, "wavelet_shared must be 0, 1 or 2");
+  if (pl.K > 1 && p->wavelet_shared == 1)
+    return fail(GBW_ERR_INVALID,
+                "several source points per shot take src_v of shape (src_n, nt) or "
+                "(src_n, src_points, nt)");
   if (p->z_chunk < 0) return fail(GBW_ERR_INVALID, "z_chunk must be >= 0");
-  pl.march = p->step_kernel == 1;
+  // several points per shot: the p / v kernels (the fused steps inject one)
+  pl.march = p->step_kernel == 1 && pl.K == 1;
   // the fused step runs the forward pass; the gradient's recompute keeps the
   // in-place p / v kernels
-  pl.fused = p->step_kernel == 2 && pass == 0;
+  pl.fused = p->step_kernel == 2 && pass == 0 && pl.K == 1;
   int spb;
   if (pl.fused) {
     // its buffer accesses take 32-bit byte offsets: the six coefficient planes
@@ -1547,8 +1600,9 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
   auto take = [&](size_t bytes) { size_t at = o; o += al256(bytes); return at; };
   pl.off_coef = take(6 * (size_t)d.plane * sizeof(float));
   pl.off_state = take((pl.march || pl.fused ? 8 : 4) * fld);
-  pl.off_src = take((size_t)pl.S * sizeof(int));
+  pl.off_src = take((size_t)pl.S * pl.K * sizeof(int));
   pl.off_rcell = take(R1 * sizeof(int));
+  pl.off_gwp = 0;
   if (pass == 0) {
     pl.off_E = take((size_t)pl.groups * 4 * inner * sizeof(float));
   } else {
@@ -1563,6 +1617,8 @@ int make_plan3(const gbw_acoustic3d_problem* p, int pass, Plan3& pl) {
     pl.off_ustart = take((R1 + 1) * sizeof(int));
     pl.off_urcv = take(R1 * sizeof(int));
     pl.off_rowstart = take(((size_t)d.NZ * d.NY + 1) * sizeof(int));
+    // several points per shot: wavelet-gradient samples per point
+    if (pl.K > 1) pl.off_gwp = take((size_t)pl.S * pl.K * pl.nt * sizeof(float));
   }
   pl.total = o;
   return 0;
@@ -1588,13 +1644,23 @@ int geometry3(const Plan3& pl, const gbw_acoustic3d_inputs* in, std::vector<int>
   if (!in->src_x || !in->src_y || !in->src_z) return fail(GBW_ERR_INVALID, "source positions are NULL");
   if (pl.R > 0 && (!in->rcv_x || !in->rcv_y || !in->rcv_z))
     return fail(GBW_ERR_INVALID, "receiver positions are NULL");
-  src.assign(pl.S, 0);
+  // point k of shot s at s * K + k
+  src.assign((size_t)pl.S * pl.K, 0);
   rcell.assign(std::max(1, pl.R), 0);
-  for (int s = 0; s < pl.S; ++s) {
-    const long long z = in->src_z[s] + d.nabc, y = in->src_y[s] + d.nabc, x = in->src_x[s] + d.nabc;
+  for (int i = 0; i < pl.S * pl.K; ++i) {
+    const long long z = in->src_z[i] + d.nabc, y = in->src_y[i] + d.nabc, x = in->src_x[i] + d.nabc;
+    const std::string who = pl.K > 1 ? "source " + std::to_string(i / pl.K) + " point " +
+                                           std::to_string(i % pl.K)
+                                     : "source " + std::to_string(i);
     if (z < 0 || z >= d.NZ || y < 0 || y >= d.NY || x < 0 || x >= d.NX)
-      return fail(GBW_ERR_INDEX, "index out of range: source " + std::to_string(s));
-    src[s] = (int)((z * d.NY + y) * d.PX + x);
+      return fail(GBW_ERR_INDEX, "index out of range: " + who);
+    src[i] = (int)((z * d.NY + y) * d.PX + x);
+    // the reference's indexed update keeps one of two equal cells' sums
+    // (acoustic_3d.py:174): such shots are refused, not guessed at
+    for (int j = i - i % pl.K; j < i; ++j)
+      if (src[j] == src[i])
+        return fail(GBW_ERR_INVALID, who + " repeats the cell of point " +
+                                         std::to_string(j % pl.K) + " of its shot");
   }
   for (int r = 0; r < pl.R; ++r) {
     const long long z = in->rcv_z[r] + d.nabc, y = in->rcv_y[r] + d.nabc, x = in->rcv_x[r] + d.nabc;
@@ -1615,7 +1681,10 @@ unsigned step_blocks3(const Dims3& d) {
 int forward_step3(Step3Args& a, const dim3& grid, hipStream_t st) {
   {
     LaunchTimer timer("p3_step_kernel", st);
-    p3_step_kernel<<<grid, T3, 0, st>>>(a);
+    if (a.src_k > 1)
+      p3_step_kernel<true><<<grid, T3, 0, st>>>(a);
+    else
+      p3_step_kernel<false><<<grid, T3, 0, st>>>(a);
   }
   LAUNCH_CHECK();
   {
@@ -1752,7 +1821,12 @@ int gbw_acoustic3d_forward(const gbw_acoustic3d_problem* prob, const gbw_acousti
   a.d = d; a.S = pl.S; a.spb = pl.spb; a.nt = pl.nt; a.dt = prob->dt;
   a.state = state; a.fstride = fstride; a.coef = coef;
   a.src_cell = at3<int>(workspace, pl.off_src);
-  a.wav = in->src_v; a.wav_stride = prob->wavelet_shared ? 0 : prob->nt;
+  // wavelet rows: shared (nt,), per shot (src_n, nt), per point (src_n, K, nt)
+  a.wav = in->src_v;
+  a.wav_stride = prob->wavelet_shared == 1 ? 0
+                                           : (prob->wavelet_shared == 2 ? (long long)pl.K : 1) * prob->nt;
+  a.wav_kstride = prob->wavelet_shared == 2 ? prob->nt : 0;
+  a.src_k = pl.K;
   a.energy = energy ? E : nullptr;
   const dim3 grid(step_blocks3(d), pl.groups);
   float* cur = state;
@@ -1859,15 +1933,20 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   if ((rc = fill3(r, fstride, 0.f, st))) return rc;
   if ((rc = fill3(gA1, (long long)pl.groups * d.plane, 0.f, st))) return rc;
   if (gA2 && (rc = fill3(gA2, (long long)pl.groups * d.plane, 0.f, st))) return rc;
-  const bool shared = prob->wavelet_shared != 0;
-  if (adj->grad_src_v &&
-      (rc = fill3(adj->grad_src_v, shared ? pl.nt : (long long)pl.S * pl.nt, 0.f, st)))
-    return rc;
+  const bool shared = prob->wavelet_shared == 1;
+  const long long nw = shared ? pl.nt : (long long)pl.S * (prob->wavelet_shared == 2 ? pl.K : 1) * pl.nt;
+  if (adj->grad_src_v && (rc = fill3(adj->grad_src_v, nw, 0.f, st))) return rc;
+  // several points per shot: per-point samples, reduced to the wavelet's shape at the end
+  float* gwp = (pl.K > 1 && adj->grad_src_v) ? at3<float>(workspace, pl.off_gwp) : nullptr;
+  if (gwp && (rc = fill3(gwp, (long long)pl.S * pl.K * pl.nt, 0.f, st))) return rc;
   Step3Args a{};
   a.d = d; a.S = pl.S; a.spb = pl.spb; a.nt = pl.nt; a.dt = prob->dt;
   a.state = state; a.fstride = fstride; a.coef = coef;
   a.src_cell = at3<int>(workspace, pl.off_src);
-  a.wav = in->src_v; a.wav_stride = shared ? 0 : prob->nt;
+  a.wav = in->src_v;
+  a.wav_stride = shared ? 0 : (prob->wavelet_shared == 2 ? (long long)pl.K : 1) * prob->nt;
+  a.wav_kstride = prob->wavelet_shared == 2 ? prob->nt : 0;
+  a.src_k = pl.K;
   March3Args m{};
   m.d = d; m.S = pl.S; m.nt = pl.nt; m.dt = prob->dt; m.fstride = fstride; m.coef = coef;
   m.src_cell = a.src_cell; m.wav = a.wav; m.wav_stride = a.wav_stride;
@@ -1875,7 +1954,8 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
   b.d = d; b.S = pl.S; b.spb = pl.spb; b.nt = pl.nt; b.dt = prob->dt;
   b.lam = lam; b.fstride = fstride; b.r = r; b.coef = coef;
   b.gA1 = gA1; b.gA2 = gA2; b.src_cell = a.src_cell;
-  b.gw = (adj->grad_src_v && !shared) ? adj->grad_src_v : nullptr;
+  b.gw = gwp ? gwp : (adj->grad_src_v && !shared) ? adj->grad_src_v : nullptr;
+  b.src_k = pl.K;
   b.gws = (adj->grad_src_v && shared) ? at3<float>(workspace, pl.off_gws) : nullptr;
   const dim3 grid(step_blocks3(d), pl.groups);
   if (pl.R > 0 && (adj->g_p || adj->g_vx || adj->g_vy || adj->g_vz)) {
@@ -1918,13 +1998,26 @@ int gbw_acoustic3d_backward(const gbw_acoustic3d_problem* prob, const gbw_acoust
       LAUNCH_CHECK();
       {
         LaunchTimer timer("adj3_p_kernel", st);
-        adj3_p_kernel<<<grid, T3, 0, st>>>(b);
+        if (b.src_k > 1)
+          adj3_p_kernel<true><<<grid, T3, 0, st>>>(b);
+        else
+          adj3_p_kernel<false><<<grid, T3, 0, st>>>(b);
       }
       LAUNCH_CHECK();
       if (b.gws) {
         wavelet_sum3_kernel<<<1, 64, 0, st>>>(b.gws, pl.S, adj->grad_src_v, it);
         LAUNCH_CHECK();
       }
+    }
+  }
+  if (gwp) {  // per-point samples -> (src_n, K, nt) as they are, or summed per shot
+    if (prob->wavelet_shared == 2)
+      HIP_TRY(hipMemcpyAsync(adj->grad_src_v, gwp, (size_t)nw * sizeof(float),
+                             hipMemcpyDeviceToDevice, st));
+    else {
+      wavelet_points3_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(gwp, pl.S, pl.K, pl.nt,
+                                                                         adj->grad_src_v);
+      LAUNCH_CHECK();
     }
   }
   if (adj->grad_vp || adj->grad_rho) {

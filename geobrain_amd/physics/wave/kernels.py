@@ -648,8 +648,8 @@ def acoustic_forward_kernel_3d(
         _on_gpu(t, name, dev)
     if not torch.cuda.is_available():
         raise RuntimeError("geobrain_amd: no ROCm GPU is visible")
-    sx, sy, sz = (_host_index(a, n) for a, n in ((src_x, "src_x"), (src_y, "src_y"),
-                                                 (src_z, "src_z")))
+    sx, sy, sz = (_host_index(a, n, True) for a, n in ((src_x, "src_x"), (src_y, "src_y"),
+                                                       (src_z, "src_z")))
     rx, ry, rz = (_host_index(a, n) for a, n in ((rcv_x, "rcv_x"), (rcv_y, "rcv_y"),
                                                  (rcv_z, "rcv_z")))
     if not (sx.shape == sy.shape == sz.shape) or sx.shape[0] != src_n:
@@ -661,7 +661,17 @@ def acoustic_forward_kernel_3d(
     padded = (nz + 2 * nabc, ny + 2 * nabc, nx + 2 * nabc)
     if tuple(damp.shape) != padded:
         raise ValueError(f"damp must have shape {padded}")
-    if src_v.dim() == 1:
+    # (src_n, K) positions: K point sources per shot (acoustic_3d.py:171-176)
+    points = sx.shape[1] if sx.ndim == 2 else 1
+    if sx.ndim == 2:
+        if src_v.dim() == 2 and tuple(src_v.shape) == (src_n, nt):
+            shared = 0
+        elif src_v.dim() == 3 and tuple(src_v.shape) == (src_n, points, nt):
+            shared = 2
+        else:
+            raise ValueError(f"with (src_n, K) source positions src_v must have shape "
+                             f"({src_n}, {nt}) or ({src_n}, {points}, {nt})")
+    elif src_v.dim() == 1:
         shared = 1
         if src_v.shape[0] != nt:
             raise ValueError(f"src_v has {src_v.shape[0]} samples, nt={nt}")
@@ -678,6 +688,7 @@ def acoustic_forward_kernel_3d(
     prob.nabc, prob.free_surface = int(nabc), int(bool(free_surface))
     prob.src_n, prob.rcv_n = int(src_n), int(rcv_n)
     prob.wavelet_shared = shared
+    prob.src_points = int(points)
     prob.energy_chunk = chunk_length(nt, checkpoint_segments)
     prob.shots_per_block = _launch3.shots_per_block
     prob.step_kernel = _launch3.step_kernel

@@ -38,9 +38,10 @@ typedef struct gbw_acoustic3d_problem {
   float dt;                  /* time step (s)                                          */
   int32_t nabc;              /* absorbing layers per side (>= 1)                       */
   int32_t free_surface;      /* 0 / 1                                                  */
-  int32_t src_n;             /* shots (one point source each)                          */
+  int32_t src_n;             /* shots                                                  */
   int32_t rcv_n;             /* receivers shared by every shot                         */
   int32_t wavelet_shared;    /* 1: src_v is (nt,) and feeds every shot; 0: (src_n, nt) */
+                             /* (every point of a shot); 2: (src_n, src_points, nt)     */
   int32_t energy_chunk;      /* steps per chunk whose energy sums are added (0 = nt)   */
   int32_t shots_per_block;   /* launch choice: shots per workgroup (0 = all); 1, 2 or 4
                                 with the z-marching kernel (0 = 4); 1 to 8 with the
@@ -51,17 +52,21 @@ typedef struct gbw_acoustic3d_problem {
                                 2 fused step with rows in wavefronts (forward pass;
                                 the gradient's recompute keeps the p and v kernels)   */
   int32_t z_chunk;           /* launch choice: z planes per z-marching workgroup (0 = auto) */
+  int32_t src_points;        /* point sources per shot (0 or 1: one; K > 1: src_x, src_y
+                                and src_z are (src_n, K), distinct cells within a shot,
+                                as the reference's 2-D positions, acoustic_3d.py:171-176;
+                                such problems run the separate p and v step kernels) */
 } gbw_acoustic3d_problem;
 
 /* Geometry and model (acoustic_3d.py:286-297); positions are unpadded indices. */
 typedef struct gbw_acoustic3d_inputs {
-  const int64_t* src_x;      /* HOST (src_n,)                                          */
-  const int64_t* src_y;      /* HOST (src_n,)                                          */
-  const int64_t* src_z;      /* HOST (src_n,)                                          */
+  const int64_t* src_x;      /* HOST (src_n,), or (src_n, src_points) row-major        */
+  const int64_t* src_y;      /* HOST (src_n,), or (src_n, src_points) row-major        */
+  const int64_t* src_z;      /* HOST (src_n,), or (src_n, src_points) row-major        */
   const int64_t* rcv_x;      /* HOST (rcv_n,)                                          */
   const int64_t* rcv_y;      /* HOST (rcv_n,)                                          */
   const int64_t* rcv_z;      /* HOST (rcv_n,)                                          */
-  const float* src_v;        /* DEVICE (src_n, nt) or (nt,)                            */
+  const float* src_v;        /* DEVICE (src_n, nt), (nt,) or (src_n, src_points, nt)   */
   const float* damp;         /* DEVICE (nz + 2 nabc, ny + 2 nabc, nx + 2 nabc)         */
   const float* vp;           /* DEVICE (nz, ny, nx)                                    */
   const float* rho;          /* DEVICE (nz, ny, nx)                                    */

@@ -91,6 +91,17 @@ def _load():
             fn = getattr(lib, name)
             fn.argtypes = grad3_args
             fn.restype = I
+        # several source points per shot: (..., nt, S, K, per_point, positions, ...)
+        mp3_fwd = [I, I, I, I, I, D, D, I, I, I, I, P, P, P, P, I, P, P, P, P, P, P, I,
+                   P, P, P, P, P, P, P, P]
+        mp3_grad = [I, I, I, I, I, D, D, I, I, I, I, P, P, P, P, I, P, P, P, P, P, P, I,
+                    P, P, P, P, P, P, P]
+        for name, args in (("oracle3_forward_mp_f32", mp3_fwd), ("oracle3_forward_mp_f64", mp3_fwd),
+                           ("oracle3_gradient_mp_f32", mp3_grad),
+                           ("oracle3_gradient_mp_f64", mp3_grad)):
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = I
         _lib = lib
     return _lib
 
@@ -329,17 +340,32 @@ def gardner(vp):
     return (310.0 * np.power(np.asarray(vp, dtype=np.float32), 0.25)).astype(np.float32)
 
 
+def _points3(src_z, src_v):
+    """(K, per_point) of 3D source positions: (S, K) positions hold K points per
+    shot; a 3-D src_v (S, K, nt) has a wavelet per point."""
+    sz = np.asarray(src_z)
+    K = sz.shape[1] if sz.ndim == 2 else 1
+    per_point = int(np.ndim(src_v) == 3)
+    if per_point:
+        assert np.shape(src_v)[:2] == sz.shape, "src_v (S, K, nt) needs (S, K) positions"
+    return K, per_point
+
+
 def forward3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_v, rcv_z, rcv_y,
               rcv_x, vp, rho, damp, chunk=0, precision="f64", return_state=False):
     """3D forward modelling; returns a dict with the reference's output keys
-    (``acoustic_3d.py:470-476``).  ``chunk`` is the ``torch.chunk`` length."""
+    (``acoustic_3d.py:470-476``).  ``chunk`` is the ``torch.chunk`` length.
+    Positions of shape (S, K) put K point sources in each shot
+    (``acoustic_3d.py:171-176``; src_v (S, nt) per shot or (S, K, nt) per
+    point)."""
+    K, per_point = _points3(src_z, src_v)
     idx = [np.ascontiguousarray(a, dtype=np.int64).reshape(-1)
            for a in (src_z, src_y, src_x, rcv_z, rcv_y, rcv_x)]
-    S, R = idx[0].shape[0], idx[3].shape[0]
+    S, R = idx[0].shape[0] // K, idx[3].shape[0]
     src_v = np.asarray(src_v, dtype=np.float32)
     if src_v.ndim == 1:
         src_v = np.broadcast_to(src_v, (S, src_v.shape[0]))
-    src_v = np.ascontiguousarray(src_v, dtype=np.float32)
+    src_v = np.ascontiguousarray(src_v.reshape(-1, src_v.shape[-1]), dtype=np.float32)
     nt = src_v.shape[1]
     vp = np.ascontiguousarray(vp, dtype=np.float32)
     rho = np.ascontiguousarray(rho, dtype=np.float32)
@@ -351,12 +377,21 @@ def forward3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_v
     eng = {k: np.zeros((nz, ny, nx), dt_np) for k in ("p", "vx", "vy", "vz")}
     n = S * (nz + 2 * nabc) * (ny + 2 * nabc) * (nx + 2 * nabc)
     state = np.zeros((4, n), dt_np) if return_state else None
-    fn = getattr(_load(), "oracle3_forward_" + precision)
-    rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+    if K > 1 or per_point:
+        assert not return_state
+        rc = getattr(_load(), "oracle3_forward_mp_" + precision)(
+            nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S, K, per_point,
             *[_ptr(a) for a in idx[:3]], _ptr(src_v), R, *[_ptr(a) for a in idx[3:]],
             _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
             *[_ptr(rec[k]) for k in ("p", "vx", "vy", "vz")],
-            *[_ptr(eng[k]) for k in ("p", "vx", "vy", "vz")], _ptr(state))
+            *[_ptr(eng[k]) for k in ("p", "vx", "vy", "vz")])
+    else:
+        fn = getattr(_load(), "oracle3_forward_" + precision)
+        rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+                *[_ptr(a) for a in idx[:3]], _ptr(src_v), R, *[_ptr(a) for a in idx[3:]],
+                _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
+                *[_ptr(rec[k]) for k in ("p", "vx", "vy", "vz")],
+                *[_ptr(eng[k]) for k in ("p", "vx", "vy", "vz")], _ptr(state))
     if rc == -2 or rc == -3:
         raise IndexError("oracle3: source or receiver index out of range")
     if rc:
@@ -375,10 +410,13 @@ def gradient3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_
     """Gradient of ``sum(g_p*p + g_vx*vx + g_vy*vy + g_vz*vz)`` of the 3D
     forward w.r.t. vp, rho and the wavelet (``src_v`` of shape (S, nt), or
     (nt,) shared by every shot, whose gradient then has shape (nt,)).
-    Returns ``(grad_vp, grad_rho, grad_w)``."""
+    Positions of shape (S, K) put K point sources in each shot, as in
+    ``forward3d``.  Returns ``(grad_vp, grad_rho, grad_w)``."""
+    K, per_point = _points3(src_z, src_v)
     idx = [np.ascontiguousarray(a, dtype=np.int64).reshape(-1)
            for a in (src_z, src_y, src_x, rcv_z, rcv_y, rcv_x)]
-    S, R = idx[0].shape[0], idx[3].shape[0]
+    S, R = idx[0].shape[0] // K, idx[3].shape[0]
+    w_shape = np.shape(src_v)
     src_v = np.ascontiguousarray(src_v, dtype=np.float32)
     shared = src_v.ndim == 1
     nt = src_v.shape[-1]
@@ -399,12 +437,21 @@ def gradient3d(nz, ny, nx, nabc, free_surface, dt, dz, src_z, src_y, src_x, src_
     gs = [conv(g) for g in (g_p, g_vx, g_vy, g_vz)]
     gvp = np.zeros((nz, ny, nx), dtp)
     grho = np.zeros((nz, ny, nx), dtp)
-    gw = np.zeros((nt,) if shared else (S, nt), dtp)
-    fn = getattr(_load(), "oracle3_gradient_" + precision)
-    rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
-            *[_ptr(a) for a in idx[:3]], _ptr(src_v), int(shared), R,
-            *[_ptr(a) for a in idx[3:]], _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
-            *[_ptr(g) for g in gs], _ptr(gvp), _ptr(grho), _ptr(gw))
+    if K > 1 or per_point:
+        assert not shared
+        gw = np.zeros(w_shape, dtp)
+        rc = getattr(_load(), "oracle3_gradient_mp_" + precision)(
+            nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S, K, per_point,
+            *[_ptr(a) for a in idx[:3]], _ptr(src_v), R, *[_ptr(a) for a in idx[3:]],
+            _ptr(vp), _ptr(rho), _ptr(damp), int(chunk), *[_ptr(g) for g in gs], _ptr(gvp),
+            _ptr(grho), _ptr(gw))
+    else:
+        gw = np.zeros((nt,) if shared else (S, nt), dtp)
+        fn = getattr(_load(), "oracle3_gradient_" + precision)
+        rc = fn(nz, ny, nx, nabc, int(bool(free_surface)), float(dt), float(dz), nt, S,
+                *[_ptr(a) for a in idx[:3]], _ptr(src_v), int(shared), R,
+                *[_ptr(a) for a in idx[3:]], _ptr(vp), _ptr(rho), _ptr(damp), int(chunk),
+                *[_ptr(g) for g in gs], _ptr(gvp), _ptr(grho), _ptr(gw))
     if rc == -2 or rc == -3:
         raise IndexError("oracle3: source or receiver index out of range")
     if rc:

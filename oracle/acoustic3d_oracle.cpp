@@ -123,7 +123,7 @@ struct State3 {
 template <typename T>
 static void step3(const Grid3<T>& g, const Coefs3<T>& c, int S, const std::vector<int>& sz,
                   const std::vector<int>& sy, const std::vector<int>& sx, const float* w, int nt,
-                  int it, State3<T>& st) {
+                  int it, State3<T>& st, int K = 1, int per_point = 0) {
   const size_t n = g.cells();
   const int NY = g.NY, NX = g.NX;
   const size_t sy1 = (size_t)NX, sz1 = (size_t)NY * NX;
@@ -149,10 +149,14 @@ static void step3(const Grid3<T>& g, const Coefs3<T>& c, int S, const std::vecto
     }
   for (int s = 0; s < S; ++s) {
     T* p = st.p.data() + s * n;
-    // source (:165-169): p[s, sz, sy, sx] += dt * w[s, it]
-    const T upd = g.dt * (T)w[(size_t)s * nt + it];
-    const size_t q = g.at(sz[s], sy[s], sx[s]);
-    p[q] = p[q] + upd;
+    // source (:165-176): p[s, sz, sy, sx] += dt * w[s, it] at each point of
+    // the shot (point k at s*K+k; distinct cells); a row per shot or per point
+    for (int k = 0; k < K; ++k) {
+      const size_t row = per_point ? (size_t)s * K + k : (size_t)s;
+      const T upd = g.dt * (T)w[row * nt + it];
+      const size_t q = g.at(sz[s * K + k], sy[s * K + k], sx[s * K + k]);
+      p[q] = p[q] + upd;
+    }
     if (g.fs)  // p[:, f-1] = -p[:, f+1]   (:179-180)
       for (int j = 0; j < NY; ++j)
         for (int k = 0; k < NX; ++k) p[g.at(g.f - 1, j, k)] = -p[g.at(g.f + 1, j, k)];
@@ -217,15 +221,16 @@ static int forward3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, do
                          const float* src_v, int R, const int64_t* rcv_z, const int64_t* rcv_y,
                          const int64_t* rcv_x, const float* vp, const float* rho,
                          const float* damp, int chunk_len, T* rp, T* rvx, T* rvy, T* rvz, T* ep,
-                         T* evx, T* evy, T* evz, T* final_state) {
-  if (nz <= 0 || ny <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0) return -1;
+                         T* evx, T* evy, T* evz, T* final_state, int K = 1, int per_point = 0) {
+  if (nz <= 0 || ny <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0 || K <= 0)
+    return -1;
   Grid3<T> g;
   g.nz = nz; g.ny = ny; g.nx = nx; g.nabc = nabc; g.fs = fs ? 1 : 0;
   g.NZ = nz + 2 * nabc; g.NY = ny + 2 * nabc; g.NX = nx + 2 * nabc;
   g.f = fs ? nabc : 1;
   g.dt = dt_as<T>(dt); g.dz = dt_as<T>(dz);
-  std::vector<int> sz(S), sy(S), sx(S), rz(R), ry(R), rx(R);
-  for (int s = 0; s < S; ++s) {
+  std::vector<int> sz((size_t)S * K), sy((size_t)S * K), sx((size_t)S * K), rz(R), ry(R), rx(R);
+  for (int s = 0; s < S * K; ++s) {
     sz[s] = (int)src_z[s] + nabc; sy[s] = (int)src_y[s] + nabc; sx[s] = (int)src_x[s] + nabc;
     if (sz[s] < 0 || sz[s] >= g.NZ || sy[s] < 0 || sy[s] >= g.NY || sx[s] < 0 || sx[s] >= g.NX)
       return -2;
@@ -252,7 +257,7 @@ static int forward3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, do
     const int t1 = std::min(nt, t0 + chunk_len);
     for (auto& v : part) std::fill(v.begin(), v.end(), (T)0);
     for (int it = t0; it < t1; ++it) {
-      step3(g, c, S, sz, sy, sx, src_v, nt, it, st);
+      step3(g, c, S, sz, sy, sx, src_v, nt, it, st, K, per_point);
       for (int s = 0; s < S; ++s)  // receivers (:244-248)
         for (int r = 0; r < R; ++r) {
           const size_t q = s * n + g.at(rz[r], ry[r], rx[r]);
@@ -295,7 +300,7 @@ static void step3_adjoint(const Grid3<T>& g, const Coefs3<T>& c, int S, const st
                           const std::vector<int>& sy, const std::vector<int>& sx, int nt, int it,
                           const State3<T>& st0, const State3<T>& st1, State3<T>& lam,
                           std::vector<T>& gA1, std::vector<T>& gA2, T* gw, int shared,
-                          std::vector<T>& scratch) {
+                          std::vector<T>& scratch, int K = 1, int per_point = 0) {
   const size_t n = g.cells();
   const int NZ = g.NZ, NY = g.NY, NX = g.NX, f = g.f;
   const size_t sy1 = (size_t)NX, sz1 = (size_t)NY * NX;
@@ -363,11 +368,12 @@ static void step3_adjoint(const Grid3<T>& g, const Coefs3<T>& c, int S, const st
           lp[g.at(f + 1, j, k)] -= lp[g.at(f - 1, j, k)];
           lp[g.at(f - 1, j, k)] = 0;
         }
-    if (gw) {  // transpose of the source term
-      const T v = g.dt * lp[g.at(sz[s], sy[s], sx[s])];
-      if (shared) gw[it] += v;
-      else gw[(size_t)s * nt + it] += v;
-    }
+    if (gw)  // transpose of the source terms
+      for (int k = 0; k < K; ++k) {
+        const T v = g.dt * lp[g.at(sz[s * K + k], sy[s * K + k], sx[s * K + k])];
+        if (shared) gw[it] += v;
+        else gw[(per_point ? (size_t)s * K + k : (size_t)s) * nt + it] += v;
+      }
     // transpose of the p update: r = alpha1 * lambda_p', d alpha1 -= lambda_p' * bracket(v)
     T* r = qz;
     std::fill(r, r + n, (T)0);
@@ -416,15 +422,16 @@ static int gradient3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, d
                           const int64_t* rcv_y, const int64_t* rcv_x, const float* vp,
                           const float* rho, const float* damp, int chunk_len, const T* g_p,
                           const T* g_vx, const T* g_vy, const T* g_vz, T* grad_vp, T* grad_rho,
-                          T* grad_w) {
-  if (nz <= 0 || ny <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0) return -1;
+                          T* grad_w, int K = 1, int per_point = 0) {
+  if (nz <= 0 || ny <= 0 || nx <= 0 || nabc <= 0 || nt <= 0 || S <= 0 || R < 0 || K <= 0)
+    return -1;
   Grid3<T> g;
   g.nz = nz; g.ny = ny; g.nx = nx; g.nabc = nabc; g.fs = fs ? 1 : 0;
   g.NZ = nz + 2 * nabc; g.NY = ny + 2 * nabc; g.NX = nx + 2 * nabc;
   g.f = fs ? nabc : 1;
   g.dt = dt_as<T>(dt); g.dz = dt_as<T>(dz);
-  std::vector<int> sz(S), sy(S), sx(S), rz(R), ry(R), rx(R);
-  for (int s = 0; s < S; ++s) {
+  std::vector<int> sz((size_t)S * K), sy((size_t)S * K), sx((size_t)S * K), rz(R), ry(R), rx(R);
+  for (int s = 0; s < S * K; ++s) {
     sz[s] = (int)src_z[s] + nabc; sy[s] = (int)src_y[s] + nabc; sx[s] = (int)src_x[s] + nabc;
     if (sz[s] < 0 || sz[s] >= g.NZ || sy[s] < 0 || sy[s] >= g.NY || sx[s] < 0 || sx[s] >= g.NX)
       return -2;
@@ -436,8 +443,10 @@ static int gradient3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, d
       return -3;
   }
   // the forward reads src_v as (S, nt); a shared wavelet is broadcast
-  std::vector<float> wfull((size_t)S * nt);
-  for (int s = 0; s < S; ++s)
+  // (rows per point: S*K of them)
+  const int rows = per_point ? S * K : S;
+  std::vector<float> wfull((size_t)rows * nt);
+  for (int s = 0; s < rows; ++s)
     for (int t = 0; t < nt; ++t) wfull[(size_t)s * nt + t] = src_v[shared ? t : (size_t)s * nt + t];
   Coefs3<T> c;
   build_coefs3(g, vp, rho, damp, c);
@@ -452,19 +461,20 @@ static int gradient3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, d
   st.init((size_t)S * n);
   for (int k = 0; k < nseg; ++k) {
     ckpt[k] = st;
-    for (int it = bounds[k]; it < bounds[k + 1]; ++it) step3(g, c, S, sz, sy, sx, wfull.data(), nt, it, st);
+    for (int it = bounds[k]; it < bounds[k + 1]; ++it)
+      step3(g, c, S, sz, sy, sx, wfull.data(), nt, it, st, K, per_point);
   }
   State3<T> lam;
   lam.init((size_t)S * n);
   std::vector<T> gA1(n, (T)0), gA2(n, (T)0), scratch(3 * n);
-  if (grad_w) std::fill(grad_w, grad_w + (shared ? (size_t)nt : (size_t)S * nt), (T)0);
+  if (grad_w) std::fill(grad_w, grad_w + (shared ? (size_t)nt : (size_t)rows * nt), (T)0);
   for (int k = nseg - 1; k >= 0; --k) {
     const int t0 = bounds[k], t1 = bounds[k + 1];
     std::vector<State3<T>> seg(t1 - t0 + 1);
     seg[0] = ckpt[k];
     for (int it = t0; it < t1; ++it) {
       seg[it - t0 + 1] = seg[it - t0];
-      step3(g, c, S, sz, sy, sx, wfull.data(), nt, it, seg[it - t0 + 1]);
+      step3(g, c, S, sz, sy, sx, wfull.data(), nt, it, seg[it - t0 + 1], K, per_point);
     }
     for (int it = t1 - 1; it >= t0; --it) {
       for (int s = 0; s < S; ++s)  // records of step `it` read state it+1
@@ -477,7 +487,7 @@ static int gradient3_impl(int nz, int ny, int nx, int nabc, int fs, double dt, d
           if (g_vz) lam.vz[q] += g_vz[o];
         }
       step3_adjoint(g, c, S, sz, sy, sx, nt, it, seg[it - t0], seg[it - t0 + 1], lam, gA1, gA2,
-                    grad_w, shared, scratch);
+                    grad_w, shared, scratch, K, per_point);
     }
   }
   // chain rule: alpha1 = den c c dt / dz, alpha2 = dt / (den dz), then fold pad_model_3d
@@ -551,6 +561,57 @@ int oracle3_gradient_f64(int nz, int ny, int nx, int nabc, int fs, double dt, do
   return gradient3_impl<double>(nz, ny, nx, nabc, fs, dt, dz, nt, S, src_z, src_y, src_x, src_v,
                                 shared, R, rcv_z, rcv_y, rcv_x, vp, rho, damp, chunk_len, g_p,
                                 g_vx, g_vy, g_vz, grad_vp, grad_rho, grad_w);
+}
+
+// several source points per shot: src_* hold S*K points (point k of shot s
+// at s*K+k), src_v a row per shot (per_point 0) or per point (1)
+int oracle3_forward_mp_f32(int nz, int ny, int nx, int nabc, int fs, double dt, double dz, int nt,
+                           int S, int K, int per_point, const int64_t* src_z,
+                           const int64_t* src_y, const int64_t* src_x, const float* src_v, int R,
+                           const int64_t* rcv_z, const int64_t* rcv_y, const int64_t* rcv_x,
+                           const float* vp, const float* rho, const float* damp, int chunk_len,
+                           float* rp, float* rvx, float* rvy, float* rvz, float* ep, float* evx,
+                           float* evy, float* evz) {
+  return forward3_impl<float>(nz, ny, nx, nabc, fs, dt, dz, nt, S, src_z, src_y, src_x, src_v, R,
+                              rcv_z, rcv_y, rcv_x, vp, rho, damp, chunk_len, rp, rvx, rvy, rvz,
+                              ep, evx, evy, evz, nullptr, K, per_point);
+}
+
+int oracle3_forward_mp_f64(int nz, int ny, int nx, int nabc, int fs, double dt, double dz, int nt,
+                           int S, int K, int per_point, const int64_t* src_z,
+                           const int64_t* src_y, const int64_t* src_x, const float* src_v, int R,
+                           const int64_t* rcv_z, const int64_t* rcv_y, const int64_t* rcv_x,
+                           const float* vp, const float* rho, const float* damp, int chunk_len,
+                           double* rp, double* rvx, double* rvy, double* rvz, double* ep,
+                           double* evx, double* evy, double* evz) {
+  return forward3_impl<double>(nz, ny, nx, nabc, fs, dt, dz, nt, S, src_z, src_y, src_x, src_v,
+                               R, rcv_z, rcv_y, rcv_x, vp, rho, damp, chunk_len, rp, rvx, rvy,
+                               rvz, ep, evx, evy, evz, nullptr, K, per_point);
+}
+
+int oracle3_gradient_mp_f32(int nz, int ny, int nx, int nabc, int fs, double dt, double dz, int nt,
+                            int S, int K, int per_point, const int64_t* src_z,
+                            const int64_t* src_y, const int64_t* src_x, const float* src_v, int R,
+                            const int64_t* rcv_z, const int64_t* rcv_y, const int64_t* rcv_x,
+                            const float* vp, const float* rho, const float* damp, int chunk_len,
+                            const float* g_p, const float* g_vx, const float* g_vy,
+                            const float* g_vz, float* grad_vp, float* grad_rho, float* grad_w) {
+  return gradient3_impl<float>(nz, ny, nx, nabc, fs, dt, dz, nt, S, src_z, src_y, src_x, src_v, 0,
+                               R, rcv_z, rcv_y, rcv_x, vp, rho, damp, chunk_len, g_p, g_vx, g_vy,
+                               g_vz, grad_vp, grad_rho, grad_w, K, per_point);
+}
+
+int oracle3_gradient_mp_f64(int nz, int ny, int nx, int nabc, int fs, double dt, double dz, int nt,
+                            int S, int K, int per_point, const int64_t* src_z,
+                            const int64_t* src_y, const int64_t* src_x, const float* src_v, int R,
+                            const int64_t* rcv_z, const int64_t* rcv_y, const int64_t* rcv_x,
+                            const float* vp, const float* rho, const float* damp, int chunk_len,
+                            const double* g_p, const double* g_vx, const double* g_vy,
+                            const double* g_vz, double* grad_vp, double* grad_rho,
+                            double* grad_w) {
+  return gradient3_impl<double>(nz, ny, nx, nabc, fs, dt, dz, nt, S, src_z, src_y, src_x, src_v, 0,
+                                R, rcv_z, rcv_y, rcv_x, vp, rho, damp, chunk_len, g_p, g_vx, g_vy,
+                                g_vz, grad_vp, grad_rho, grad_w, K, per_point);
 }
 
 }  // extern "C"

@@ -51,6 +51,8 @@ def adjoint_sources(z, m, fwd_p=None, obs_p=None):
 
 FWD3D_CASES = ["t3d0_fwd", "t3d1_fwd", "t3d2_fwd"]
 GRAD3D_CASES = ["t3d0_grad", "t3d1_grad", "t3d2_grad"]
+MP3D_FWD_CASES = ["t3dmp0_fwd", "t3dmp1_fwd"]   # (S, K) source positions
+MP3D_GRAD_CASES = ["t3dmp0_grad", "t3dmp1_grad"]
 OUT3D_KEYS = ("p", "vx", "vy", "vz", "forward_wavefield_p", "forward_wavefield_vx",
               "forward_wavefield_vy", "forward_wavefield_vz")
 

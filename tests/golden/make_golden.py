@@ -191,6 +191,28 @@ def cases_3d():
         name, kw = cases[idx]
         cases.append((f"t3d{idx}_grad", dict(kw, g_seed=200 + idx, rho_grad=(idx != 0),
                                              w_grad=True)))
+    # several points per shot ((S, K) positions, acoustic_3d.py:166-176): a
+    # wavelet per shot; a wavelet per point with a free surface, points on the
+    # surface plane and on the mirrored plane above it.  Distinct cells.
+    for idx, (nz, ny, nx, nabc, fs, S, K, R, nt, segs, per_point) in enumerate([
+        (9, 8, 10, 4, False, 2, 3, 6, 30, 2, False),
+        (8, 9, 9, 3, True, 2, 2, 5, 28, 1, True),
+    ]):
+        vp = (rng.uniform(1500, 3000, (nz, ny, nx))).astype(np.float32)
+        rho = rng.uniform(1800, 2400, (nz, ny, nx)).astype(np.float32)
+        src_z = np.stack([rng.permutation(nz)[:K] for _ in range(S)])
+        src_y = rng.integers(0, ny, (S, K))
+        src_x = rng.integers(0, nx, (S, K))
+        if fs:
+            src_z[0, 0], src_z[1, 1] = 0, -1
+        rcv_z, rcv_y, rcv_x = rng.integers(0, nz, R), rng.integers(0, ny, R), rng.integers(0, nx, R)
+        w = rng.standard_normal((S, K, nt) if per_point else (S, nt)).astype(np.float32)
+        kw = dict(nz=nz, ny=ny, nx=nx, nabc=nabc, fs=fs, dx=10.0, dt=1e-3, nt=nt, vp=vp,
+                  rho=rho, src_z=src_z, src_y=src_y, src_x=src_x, wavelet=w, rcv_z=rcv_z,
+                  rcv_y=rcv_y, rcv_x=rcv_x, segments=segs)
+        cases.append((f"t3dmp{idx}_fwd", kw))
+        cases.append((f"t3dmp{idx}_grad", dict(kw, g_seed=210 + idx, rho_grad=True,
+                                               w_grad=True)))
     return cases
 
 

@@ -1,11 +1,12 @@
 """Several point sources per shot (source positions of shape (src_n, K),
-acoustic_2d.py:229-237) through acoustic_forward_kernel and the C ABI: against
-the reference's golden vectors (a wavelet per shot on every point of it, and a
-wavelet per point with a free surface, points on the surface row and on the
-mirrored row above it), seismograms bit for bit and gradients within 1e-5; a
-larger problem against the FP32 oracle (seismograms bit for bit, energy maps
-and gradients within 1e-6); (src_n, 1) positions equal to 1-D ones; and the
-refusals (repeated cells within a shot, a 1-D wavelet, the batched calls)."""
+acoustic_2d.py:229-237, acoustic_3d.py:171-176) through acoustic_forward_kernel,
+acoustic_forward_kernel_3d and the C ABI: against the reference's golden
+vectors (a wavelet per shot on every point of it, and a wavelet per point with
+a free surface, points on the surface row / plane and on the mirrored one
+above it), seismograms bit for bit and gradients within 1e-5; larger problems
+against the FP32 oracle (seismograms bit for bit, energy maps and gradients
+within 1e-6); (src_n, 1) positions equal to 1-D ones; and the refusals
+(repeated cells within a shot, a 1-D wavelet, the batched calls)."""
 
 import numpy as np
 import pytest
@@ -14,8 +15,10 @@ import torch
 import oracle
 from geobrain_amd import batch as B
 from geobrain_amd.physics import wave
-from tests._fixtures import (MP_FWD_CASES, MP_GRAD_CASES, adjoint_sources, chunk_len, load,
-                             oracle_args, rel_l2)
+from tests._fixtures import (MP3D_FWD_CASES, MP3D_GRAD_CASES, MP_FWD_CASES, MP_GRAD_CASES,
+                             OUT3D_KEYS, adjoint_sources, adjoint_sources_3d, chunk_len, load,
+                             oracle3d_args, oracle_args, rel_l2)
+from tests.test_gpu_3d import grad3, run3
 from tests.test_gpu_parity import np_, run
 
 pytestmark = pytest.mark.gpu
@@ -147,3 +150,70 @@ def test_multipoint_refusals():
                 rho=T(p["rho"], dtype=torch.float32))
     with pytest.raises(NotImplementedError, match="one point per shot"):
         B.BatchedForward([prob])
+
+
+# --------------------------------------------------------------------------
+# 3D
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", MP3D_FWD_CASES)
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_multipoint_3d_forward_matches_reference(name, kernel):
+    """Every step-kernel choice (the fused ones give way to the p / v kernels
+    for several points per shot)."""
+    print(f"[case] 3d multi-point forward {name}, step_kernel {kernel}", flush=True)
+    z, m = load(name)
+    got = run3(oracle3d_args(z, m), segments=m["segments"], kernel=kernel)
+    for k in OUT3D_KEYS:
+        if k.startswith("forward_wavefield"):
+            assert rel_l2(got[k], z["out_" + k]) <= 1e-6, k
+        else:
+            np.testing.assert_array_equal(got[k], z["out_" + k], err_msg=k)
+
+
+@pytest.mark.parametrize("name", MP3D_GRAD_CASES)
+def test_multipoint_3d_gradient_matches_reference(name):
+    print(f"[case] 3d multi-point gradient {name}", flush=True)
+    z, m = load(name)
+    a = oracle3d_args(z, m)
+    gs = adjoint_sources_3d(z, m)
+    out, g = grad3(a, gs, segments=m["segments"])
+    for k in ("p", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(out[k], z["out_" + k], err_msg=k)
+    o32 = oracle.gradient3d(**a, g_p=gs[0], g_vx=gs[1], g_vy=gs[2], g_vz=gs[3],
+                            chunk=chunk_len(m["nt"], m["segments"]), precision="f32")
+    for key, zk, i in (("vp", "grad_vp", 0), ("rho", "grad_rho", 1), ("w", "grad_w", 2)):
+        assert g[key].shape == z[zk].shape, key
+        e_ref, e_or = rel_l2(g[key], z[zk]), rel_l2(g[key], o32[i])
+        print(f"  {key}: rel-L2 vs reference autograd {e_ref:.1e}, vs FP32 oracle {e_or:.1e}")
+        assert e_ref <= 1e-5 and e_or <= 1e-6, key
+
+
+def test_multipoint_3d_larger_problem_matches_oracle():
+    """30 x 28 x 40 cells, 2 shots x 3 points with a wavelet per point, free
+    surface, 60 steps in 3 checkpoint intervals: seismograms bit for bit, vp and
+    wavelet gradients within 1e-6 of the FP32 oracle."""
+    print("[case] 3d multi-point 30 x 28 x 40, 2 shots x 3 points", flush=True)
+    rng = np.random.default_rng(31)
+    nz, ny, nx, nabc, nt, S, K, R = 30, 28, 40, 6, 60, 2, 3, 12
+    vp = rng.uniform(1600, 3200, (nz, ny, nx)).astype(np.float32)
+    rho = rng.uniform(1800, 2400, (nz, ny, nx)).astype(np.float32)
+    damp = wave.bc_pml_3d(nx, ny, nz, 10.0, 10.0, 10.0, nabc, float(vp.max()), False)
+    a = dict(nz=nz, ny=ny, nx=nx, nabc=nabc, free_surface=True, dt=1e-3, dz=10.0,
+             src_z=np.array([[0, 5, 17], [-1, 9, 2]]), src_y=rng.integers(0, ny, (S, K)),
+             src_x=rng.integers(0, nx, (S, K)),
+             src_v=rng.standard_normal((S, K, nt)).astype(np.float32),
+             rcv_z=rng.integers(0, nz, R), rcv_y=rng.integers(0, ny, R),
+             rcv_x=rng.integers(0, nx, R), vp=vp, rho=rho, damp=damp.astype(np.float32))
+    gs = [rng.standard_normal((S, nt, R)).astype(np.float32), None, None, None]
+    out, g = grad3(a, gs, segments=3, want=("vp", "w"))
+    clen = chunk_len(nt, 3)
+    want = oracle.forward3d(**a, chunk=clen, precision="f32")
+    assert float(np.abs(want["p"]).max()) > 0
+    for k in ("p", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(out[k], want[k], err_msg=k)
+    gv, _, gw = oracle.gradient3d(**a, g_p=gs[0], chunk=clen, precision="f32")
+    for key, o in (("vp", gv), ("w", gw)):
+        e = rel_l2(g[key], o)
+        print(f"  grad {key}: rel-L2 vs FP32 oracle {e:.1e}")
+        assert e <= 1e-6, key

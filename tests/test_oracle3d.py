@@ -10,11 +10,12 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._fixtures import (FWD3D_CASES, GRAD3D_CASES, OUT3D_KEYS, adjoint_sources_3d,
-                             chunk_len, load, oracle3d_args, rel_l2)
+from tests._fixtures import (FWD3D_CASES, GRAD3D_CASES, MP3D_FWD_CASES, MP3D_GRAD_CASES,
+                             OUT3D_KEYS, adjoint_sources_3d, chunk_len, load, oracle3d_args,
+                             rel_l2)
 
 
-@pytest.mark.parametrize("name", FWD3D_CASES)
+@pytest.mark.parametrize("name", FWD3D_CASES + MP3D_FWD_CASES + MP3D_GRAD_CASES)
 def test_oracle3d_fp32_forward_is_bitwise_reference(name):
     z, m = load(name)
     out = oracle.forward3d(**oracle3d_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
@@ -23,7 +24,7 @@ def test_oracle3d_fp32_forward_is_bitwise_reference(name):
         np.testing.assert_array_equal(out[k], z["out_" + k], err_msg=k)
 
 
-@pytest.mark.parametrize("name", FWD3D_CASES)
+@pytest.mark.parametrize("name", FWD3D_CASES + MP3D_FWD_CASES)
 def test_oracle3d_fp64_forward_close_to_reference(name):
     z, m = load(name)
     out = oracle.forward3d(**oracle3d_args(z, m), chunk=chunk_len(m["nt"], m["segments"]),
@@ -125,7 +126,7 @@ def test_oracle3d_energy_equals_sum_of_squared_records():
         np.testing.assert_allclose(e, (out[k] ** 2).sum(axis=(0, 1)), rtol=1e-12)
 
 
-@pytest.mark.parametrize("name", GRAD3D_CASES)
+@pytest.mark.parametrize("name", GRAD3D_CASES + MP3D_GRAD_CASES)
 @pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_oracle3d_gradient_matches_reference_autograd(name, precision):
     z, m = load(name)
