@@ -356,6 +356,42 @@ def test_validate_problem_rejects_bad_shapes():
         K.validate_problem(**dict(good, src_v=torch.zeros(())))
 
 
+def test_validate_problem_several_points_per_shot():
+    """(src_n, K) source positions (acoustic_2d.py:234-237): src_v (src_n, nt)
+    feeds every point of a shot (wavelet_shared 0), (src_n, K, nt) one per
+    point (2); a 1-D or mis-shaped wavelet raises ValueError; the library's
+    plan refuses a shared wavelet and repeated points are refused at geometry
+    setup (covered on the GPU), and its workspace grows with K."""
+    from geobrain_amd.physics.wave import kernels as K
+    nz, nx, nabc, nt, S, R, P = 6, 9, 3, 20, 2, 4, 3
+    i64 = lambda *n: np.zeros(n, np.int64)  # noqa: E731
+    good = dict(nx=nx, nz=nz, nt=nt, nabc=nabc, src_n=S, rcv_n=R, sx=i64(S, P), sz=i64(S, P),
+                rx=i64(R), rz=i64(R), vp=torch.zeros(nz, nx), rho=torch.zeros(nz, nx),
+                damp=torch.zeros(nz + 2 * nabc, nx + 2 * nabc), src_v=torch.zeros(S, nt))
+    assert K.validate_problem(**good) == 0
+    assert K.validate_problem(**dict(good, src_v=torch.zeros(S, P, nt))) == 2
+    for w in (torch.zeros(nt), torch.zeros(S, P + 1, nt), torch.zeros(S, nt + 1)):
+        with pytest.raises(ValueError, match="src_v must have shape"):
+            K.validate_problem(**dict(good, src_v=w))
+    with pytest.raises(ValueError):
+        K.validate_problem(**dict(good, sz=i64(S, P + 1)))
+    assert K._host_index(np.zeros((S, P), np.int64), "src_x", True).shape == (S, P)
+    with pytest.raises(NotImplementedError, match="one point per shot"):
+        K._host_index(np.zeros((S, P), np.int64), "rcv_x")
+    lib = _native.load()
+    p1 = _problem()
+    pk = _problem()
+    pk.src_points, pk.wavelet_shared = 4, 0
+    assert (lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(pk), 1) >
+            lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(p1), 1))
+    pk.wavelet_shared = 1
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(pk), 0) == 0
+    assert "several source points per shot" in _native.last_error()
+    pk.wavelet_shared, pk.src_points = 3, 1
+    assert lib.gbw_acoustic2d_workspace_bytes(ctypes.byref(pk), 0) == 0
+    assert "wavelet_shared must be" in _native.last_error()
+
+
 def test_host_index_cache_follows_tensor_changes():
     """Positions given as tensors are read fresh when the tensor changes: the
     cached host copy is keyed by the tensor's version (CPU tensors are read
