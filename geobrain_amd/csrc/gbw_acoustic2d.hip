@@ -205,8 +205,11 @@ __device__ __forceinline__ int xcd_tile() {
 
 // RECOMP = false: forward pass (receivers, energy); true: checkpoint-interval
 // recompute for the adjoint (stores the brackets the adjoint needs instead).
-template <int TZ, bool RECOMP>
+// MP: several source points per shot (src_k > 1); the one-point instance
+// compiles with src_k fixed at 1
+template <int TZ, bool RECOMP, bool MP = false>
 __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_waves<TZ>()))) void fwd_step_kernel(FwdArgs a) {
+  const int SK = MP ? a.src_k : 1;
   constexpr int RV = TZ + 3;  // vx rows z0-1 .. z0+TZ+1
   constexpr int RZ = TZ + 6;  // vz rows z0-3 .. z0+TZ+2
   constexpr int RP = TZ + 3;  // p' rows z0-1 .. z0+TZ+1
@@ -327,7 +330,7 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
 
   for (int s = s_begin; s < s_end; ++s) {
     const long long so = (long long)s * d.plane;
-    const int scell = a.src_cell[s * a.src_k];
+    const int scell = a.src_cell[s * SK];
     const float supd = a.dt * a.wav[(long long)s * a.wav_stride + a.it];
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
@@ -375,9 +378,9 @@ __global__ __launch_bounds__(NTHREADS) __attribute__((amdgpu_waves_per_eu(fwd_wa
           if ((p_in & bit) && (int)(base + e) == scell) pv = pv + supd;
           // further points of the shot (acoustic_2d.py:234-237; distinct
           // cells, so at most one adds here)
-          if (a.src_k > 1 && (p_in & bit))
-            for (int k = 1; k < a.src_k; ++k)
-              if ((int)(base + e) == a.src_cell[s * a.src_k + k])
+          if (MP && (p_in & bit))
+            for (int k = 1; k < SK; ++k)
+              if ((int)(base + e) == a.src_cell[s * SK + k])
                 pv = pv + a.dt * a.wav[(long long)s * a.wav_stride + k * a.wav_kstride + a.it];
           f4set(out, e, pv);
           f4set(dv, e, br);
@@ -1413,8 +1416,11 @@ struct AdjArgs {
 };
 
 // RHO: also accumulate the density gradient (needs the recompute's vx/vz brackets)
-template <int TZ, bool RHO>
+// MP: several source points per shot (src_k > 1); the one-point instance
+// compiles with src_k fixed at 1
+template <int TZ, bool RHO, bool MP = false>
 __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
+  const int SK = MP ? a.src_k : 1;
   constexpr int RV = TZ + 3;  // lambda_vx rows z0-1 ..
   constexpr int RZ = TZ + 6;  // lambda_vz rows z0-3 ..
   constexpr int RP = TZ + 3;  // lambda_p' rows z0-1 ..
@@ -1691,7 +1697,7 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
         const int4 rh = ((t_rcv >> n) & 1u) ? reinterpret_cast<const int4*>(a.rhead)[ti]
                                            : make_int4(-1, -1, -1, -1);
         float4 lpo = lp4;
-        const int scell = a.src_cell[s * a.src_k];
+        const int scell = a.src_cell[s * SK];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const unsigned bit = 1u << (4 * n + e);
@@ -1701,11 +1707,11 @@ __global__ __launch_bounds__(NTHREADS) void adj_step_kernel(AdjArgs a) {
             f4set(lpo, e, f4get(tc_omk1[n], e) * lp);
           }
           if ((int)(ti * 4u + e) == scell && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
-            a.gw[(long long)s * a.src_k * a.nt + a.it] = a.dt * lp;
-          if (a.src_k > 1 && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
-            for (int k = 1; k < a.src_k; ++k)
-              if ((int)(ti * 4u + e) == a.src_cell[s * a.src_k + k])
-                a.gw[((long long)s * a.src_k + k) * a.nt + a.it] = a.dt * lp;
+            a.gw[(long long)s * SK * a.nt + a.it] = a.dt * lp;
+          if (MP && a.gw && (t_ok >> n & 1u) && x0 + 4 * h + e < d.NX)
+            for (int k = 1; k < SK; ++k)
+              if ((int)(ti * 4u + e) == a.src_cell[s * SK + k])
+                a.gw[((long long)s * SK + k) * a.nt + a.it] = a.dt * lp;
           float lx = f4get(lx4, e), lz = f4get(lz4, e);
           if constexpr (RHO) {
             float g2 = f4get(gacc2[n], e);
@@ -3923,10 +3929,17 @@ void launch_steps(const Plan& pl, const FwdArgs& a_in, int steps, hipStream_t st
       fwd2_step_kernel<24, 96, FWD2_THREADS, RECOMP><<<g2, FWD2_THREADS, 0, st>>>(a2);
     else
       fwd2_step_kernel<16, 128, FWD2_THREADS, RECOMP><<<g2, FWD2_THREADS, 0, st>>>(a2);
-  } else if (pl.TZ == 32)
-    fwd_step_kernel<32, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
-  else
-    fwd_step_kernel<16, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
+  } else if (pl.TZ == 32) {
+    if (pl.K > 1)
+      fwd_step_kernel<32, RECOMP, true><<<grid, NTHREADS, 0, st>>>(a);
+    else
+      fwd_step_kernel<32, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
+  } else {
+    if (pl.K > 1)
+      fwd_step_kernel<16, RECOMP, true><<<grid, NTHREADS, 0, st>>>(a);
+    else
+      fwd_step_kernel<16, RECOMP><<<grid, NTHREADS, 0, st>>>(a);
+  }
 }
 
 // two adjoint steps (a.it, a.it-1) in one launch; 16-row tiles, or z-marching
@@ -3975,10 +3988,16 @@ template <int TZ>
 void launch_adj(const Plan& pl, const AdjArgs& a, hipStream_t st) {
   dim3 grid(tile_grid(pl), pl.groups);
   LaunchTimer timer("adj_step_kernel", st);
-  if (a.gA2)
+  if (pl.K > 1) {
+    if (a.gA2)
+      adj_step_kernel<TZ, true, true><<<grid, NTHREADS, 0, st>>>(a);
+    else
+      adj_step_kernel<TZ, false, true><<<grid, NTHREADS, 0, st>>>(a);
+  } else if (a.gA2) {
     adj_step_kernel<TZ, true><<<grid, NTHREADS, 0, st>>>(a);
-  else
+  } else {
     adj_step_kernel<TZ, false><<<grid, NTHREADS, 0, st>>>(a);
+  }
 }
 
 // Prepared geometry buffers (gbw_acoustic2d_geometry_setup), by device
